@@ -1,0 +1,162 @@
+"""Benchmark: fwd+adj+grad of the single-shot production configuration on one GPU.
+
+    python bench.py [--gpus 1] [--steps K] [--warmup W] [--workload cfg2] [--nt NT]
+                    [--no-cpu-baseline] [--profile]
+
+A "step" is one gradient evaluation of one shot: the forward loop, the misfit
+and adjoint source, and the backward loop (reverse reconstruction + adjoint +
+lambda/mu/rho imaging) over all nt time steps of the 200x200x120 (padded
+240x240x160), 3000-step workload. Inputs (model, observed data) are resident
+in HBM before the timed region. Prints one JSON line.
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import os
+import sys
+import time
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, os.path.join(ROOT, "efwi-3d_amd"))
+
+HBM_PEAK_GBS = 8000.0
+# Algorithmic FP32 traffic per updated cell of each kernel, with an ideal
+# cache (fields read + written once): the reference's kernel decomposition.
+ALG_FLOATS = {"stress_fwd": 17, "velocity_fwd": 13, "velocity_rev": 18, "stress_rev": 27,
+              "velocity_adj": 15, "stress_adj": 18}
+
+
+def alg_bytes_per_launch(kernel, grid):
+    units = grid.interior_cells() if kernel in ("velocity_rev", "stress_rev") else grid.ncells
+    return 4.0 * ALG_FLOATS[kernel] * units
+
+
+def alg_bytes_pipeline(grid):
+    npad, nint, nt = grid.ncells, grid.interior_cells(), grid.nt
+    return 120.0 * npad * (nt - 1) + 180.0 * nint * (nt - 1) + 132.0 * npad * nt
+
+
+def cpu_baseline(w, nsteps):
+    """The CPU oracle (fp32, OpenMP over all host threads) on the first
+    `nsteps` time steps of the same workload (fwd + adjoint + gradient)."""
+    sys.path.insert(0, os.path.join(ROOT, "oracle"))
+    import numpy as np
+    import oracle
+    g = w.grid
+    og = oracle.Grid(g.nz, g.nx, g.ny, g.npml, nsteps, g.dz, g.dx, g.dy, g.dt, g.f0)
+    sh = w.shots[0]
+    osh = oracle.Shots(src=np.array([sh.src]), stf=sh.stf[None, :nsteps], rec=[sh.rec])
+    obs = [np.zeros((3, sh.nrec, nsteps), np.float32)]
+    r = oracle.run(og, osh, *w.model, obs=obs, calc_id=1, precision="fp32")
+    cores = int(os.environ.get("OMP_NUM_THREADS", "0")) or len(os.sched_getaffinity(0))
+    gcell = g.ncells * nsteps / r.seconds / 1e9
+    return {"value": gcell, "unit": "Gcell-updates/s", "cores": cores, "kind": "port",
+            "sample": f"{w.name} grid {g.nz}x{g.nx}x{g.ny}, first {nsteps} of {g.nt} time steps, "
+                      f"fwd+adj+grad, 1 shot, fp32 oracle, {r.seconds:.1f} s"}
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=3)
+    ap.add_argument("--warmup", type=int, default=1)
+    ap.add_argument("--workload", default="cfg2")
+    ap.add_argument("--nt", type=int, default=None, help="override the number of time steps")
+    ap.add_argument("--cpu-steps", type=int, default=6)
+    ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--no-profile", action="store_true", help="skip the per-kernel event timing")
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print("bench.py measures one GPU: every box of this deployment has one GPU and the path "
+              "shards by shot across GPUs (see the multi-GPU design); run with --gpus 1.",
+              file=sys.stderr)
+        sys.exit(2)
+
+    import numpy as np
+    import torch
+    from efwi import synth
+    from efwi.propagator import Propagator
+
+    w = synth.WORKLOADS[args.workload](**({} if args.nt is None else {"nt": args.nt}))
+    g = w.grid
+    shot = w.shots[0]
+    print(f"[bench] {w.name}: padded grid {g.shape}, nt={g.nt}, nrec={shot.nrec}", file=sys.stderr, flush=True)
+    P = Propagator(g, 0)
+    # observed data from the true model (untimed set-up)
+    P.set_model(*(torch.from_numpy(a) for a in w.true))
+    _, obs, _ = P.run_shot(shot, 2)
+    P.set_model(*(torch.from_numpy(a) for a in w.model))
+    torch.cuda.synchronize()
+
+    def step():
+        P.zero_gradient()
+        j, _, _ = P.run_shot(shot, 1, obs)
+        return j
+
+    for _ in range(args.warmup):
+        step()
+    torch.cuda.synchronize()
+    print(f"[bench] warmup done, device bytes {P.device_bytes / 2**30:.1f} GiB", file=sys.stderr, flush=True)
+    if not args.no_profile:
+        P.profile(True)
+        P.profile_read(reset=True)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for k in range(args.steps):
+        j = step()
+        print(f"[bench] step {k} misfit {j:.6e}", file=sys.stderr, flush=True)
+    torch.cuda.synchronize()
+    t1 = time.perf_counter()
+    prof = P.profile_read(reset=True) if not args.no_profile else {}
+    dt_step = (t1 - t0) / args.steps
+    gcell = g.ncells * g.nt / dt_step / 1e9
+    pipe_gbs = alg_bytes_pipeline(g) / dt_step / 1e9
+
+    roof = None
+    kernels = {}
+    for k, (ms, n) in prof.items():
+        if k in ALG_FLOATS and n:
+            avg_ms = ms / n
+            ach = alg_bytes_per_launch(k, g) / (avg_ms * 1e-3) / 1e9
+            kernels[k] = {"avg_us": round(avg_ms * 1e3, 2), "launches": n, "share": ms / (dt_step * args.steps * 1e3),
+                          "alg_GBps": round(ach, 1)}
+    if kernels:
+        dom = max(kernels, key=lambda k: prof[k][0])
+        ach = kernels[dom]["alg_GBps"]
+        roof = {"bound": "hbm", "kernel": dom, "achieved": ach, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                "frac": round(ach / HBM_PEAK_GBS, 4), "traffic": None}
+
+    cpu = None
+    if not args.no_cpu_baseline:
+        print("[bench] cpu baseline ...", file=sys.stderr, flush=True)
+        cpu = cpu_baseline(w, args.cpu_steps)
+
+    out = {
+        "metric": "fwd+adj+grad Gcell-updates/s & %HBM peak",
+        "value": round(gcell, 3),
+        "unit": "Gcell-updates/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": round(dt_step * 1e3, 2),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "fp32",
+        "data": "synthetic",
+        "config": {"workload": f"{w.name}: {g.nz - 2 * g.npml}x{g.nx - 2 * g.npml}x{g.ny - 2 * g.npml} physical "
+                               f"(z,x,y) + {g.npml} PML cells, padded {g.nz}x{g.nx}x{g.ny}, nt={g.nt}, "
+                               f"1 shot, {shot.nrec} receivers, fwd+adj+grad, ring reconstruction",
+                   "pipeline_alg_GBps": round(pipe_gbs, 1),
+                   "pipeline_hbm_frac": round(pipe_gbs / HBM_PEAK_GBS, 4),
+                   "kernels": kernels},
+        "roofline": roof,
+        "cpu_baseline": cpu,
+    }
+    print(json.dumps(out), flush=True)
+    P.close()
+
+
+if __name__ == "__main__":
+    main()

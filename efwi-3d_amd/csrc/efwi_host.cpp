@@ -1,0 +1,581 @@
+// Host orchestration and C ABI of the gfx950 elastic FWI library.
+//
+// One efwi_ctx owns the device state of one grid: the forward and adjoint
+// wavefields, the 18 C-PML memory arrays, the material arrays, the gradient
+// accumulators, the 1-D C-PML profiles and (for gradients) the boundary-band
+// ring of the reverse reconstruction. efwi_run_shot drives the time loops of
+// the reference driver (Src/libCUFD.cu:220-613) on one HIP stream.
+#include <hip/hip_runtime.h>
+
+#include <cmath>
+#include <cstdio>
+#include <cstring>
+#include <string>
+#include <vector>
+
+#include "../../include/efwi.h"
+#include "efwi_common.h"
+#include "efwi_host.h"
+#include "efwi_kernels.h"
+
+namespace efwi {
+
+static thread_local std::string g_err;
+
+int fail(int code, const std::string& msg) {
+  g_err = msg;
+  return code;
+}
+
+#define HIP_TRY(call)                                                                   \
+  do {                                                                                  \
+    hipError_t e_ = (call);                                                             \
+    if (e_ != hipSuccess)                                                               \
+      return fail(EFWI_ERR_HIP, std::string(#call) + ": " + hipGetErrorString(e_));     \
+  } while (0)
+
+// ---------------------------------------------------------------------------
+// 1-D C-PML profile of one axis, Src/utilities.cu:204-320. The float
+// variables of the reference (d0_h, depth, normalised depth, damp, K, alpha,
+// a, b) are rounded to float where the reference stores them; the libm
+// pow/log/exp are evaluated in double except where the reference calls
+// expf/powf.
+// ---------------------------------------------------------------------------
+void cpml_profile_1d(int n, int npml, float h, float f0, float dt, float* K, float* a, float* b,
+                     float* Kh, float* ah, float* bh) {
+  const double PI = 3.141592653589793238462643383279502884197169;
+  const float R = 0.0008f, KMAX = 2.0f, NP = 8.0f;
+  const float AMAX = (float)(2.0 * PI * (f0 / 2.0));
+  const float thick = npml * h;
+  const float vref = 3000.0f;
+  const float d0 = (float)(-(NP + 1) * vref * std::log((double)R) / (2.0 * thick));
+  std::vector<float> damp(n, 0.0f), damph(n, 0.0f), alpha(n, 0.0f), alphah(n, 0.0f);
+  for (int i = 0; i < n; i++) {
+    K[i] = 1.0f; Kh[i] = 1.0f; a[i] = 0.0f; ah[i] = 0.0f;
+  }
+  auto prof = [&](float q, float& dmp, float& kk, float& al, bool kpowf) {
+    dmp = (float)(d0 * (0.25f * (double)q + 0.75f * std::pow((double)q, (double)NP) +
+                        0.0f * std::pow((double)q, 2.0 * NP)));
+    kk = kpowf ? (float)(1.0 + (KMAX - 1.0) * powf(q, NP))
+               : (float)(1.0 + (KMAX - 1.0) * std::pow((double)q, (double)NP));
+    al = (float)(AMAX * (1.0 - q));
+  };
+  for (int i = 0; i < n; i++) {
+    float dep = (float)(npml - i) * h;
+    if (dep >= 0.0f) prof(dep / thick, damp[i], K[i], alpha[i], false);
+    dep = (float)((npml - i - 0.5) * h);
+    if (dep >= 0.0f) prof(dep / thick, damph[i], Kh[i], alphah[i], false);
+    dep = (float)(npml - n + i) * h;
+    if (dep >= 0.0f) prof(dep / thick, damp[i], K[i], alpha[i], false);
+    dep = (float)((npml - n + i + 0.5) * h);
+    if (dep >= 0.0f) prof(dep / thick, damph[i], Kh[i], alphah[i], true);
+    if (alpha[i] < 0.0f) alpha[i] = 0.0f;
+    if (alphah[i] < 0.0f) alphah[i] = 0.0f;
+    b[i] = expf(-(damp[i] / K[i] + alpha[i]) * dt);
+    bh[i] = expf(-(damph[i] / Kh[i] + alphah[i]) * dt);
+    if (std::fabs(damp[i]) > 1.0e-6)
+      a[i] = (float)(damp[i] * (b[i] - 1.0) / (K[i] * (damp[i] + K[i] * alpha[i])));
+    if (std::fabs(damph[i]) > 1.0e-6)
+      ah[i] = (float)(damph[i] * (bh[i] - 1.0) / (Kh[i] * (damph[i] + Kh[i] * alphah[i])));
+  }
+}
+
+// w(t) of the time tapers, Src/utilities.cu:578-614 (plain) and :524-576
+// (per trace). `valid` is false where the reference returns early and leaves
+// the data unchanged.
+static float taper_w(float t, float t0, float t1, float t2, float t3) {
+  const double PI = 3.141592653589793238462643383279502884197169;
+  if (t >= t0 && t < t1) return (float)std::sin(PI / 2.0 * (t - t0) / (t1 - t0));
+  if (t >= t1 && t < t2) return 1.0f;
+  if (t >= t2 && t < t3) return (float)std::cos(PI / 2.0 * (t - t2) / (t3 - t2));
+  return 0.0f;
+}
+
+float taper_plain_w2(int it, int nt, float dt, float ratio) {
+  const float t = it * dt, t0 = 0, t3 = nt * dt, off = nt * dt * ratio;
+  if (2.0 * off >= t3 - t0) return 1.0f;
+  const float w = taper_w(t, t0, t0 + off, t3 - off, t3);
+  return w * w;
+}
+
+float taper_window_fac(int it, int nt, float dt, float ratio, float t0, float t3, float weight,
+                       float src_weight) {
+  const float t = it * dt;
+  if (ratio > 0.5) return 1.0f;
+  const float tmax = nt * dt;
+  if (t0 < 0.0f) t0 = 0.0f;
+  if (t0 > tmax) t0 = tmax;
+  if (t3 < 0.0f) t3 = 0.0f;
+  if (t3 > tmax) t3 = tmax;
+  const float off = (t3 - t0) * ratio;
+  if (off <= 0.0) return 1.0f;
+  const float w = taper_w(t, t0, t0 + off, t3 - off, t3);
+  return w * w * weight * src_weight;
+}
+
+// ---------------------------------------------------------------------------
+// Profiler: an event pair around each timed launch.
+// ---------------------------------------------------------------------------
+void Profiler::begin(int k, hipStream_t s) {
+  if (!on) return;
+  if (2 * (used + 1) > pool.size()) {
+    for (int j = 0; j < 2; j++) {
+      hipEvent_t e;
+      hipEventCreate(&e);
+      pool.push_back(e);
+    }
+    kind.push_back(0);
+  }
+  kind[used] = k;
+  hipEventRecord(pool[2 * used], s);
+}
+
+void Profiler::end(hipStream_t s) {
+  if (!on) return;
+  hipEventRecord(pool[2 * used + 1], s);
+  used++;
+}
+
+int Profiler::collect() {
+  if (!used) return EFWI_OK;
+  if (hipEventSynchronize(pool[2 * used - 1]) != hipSuccess) return fail(EFWI_ERR_HIP, "event synchronize");
+  for (size_t u = 0; u < used; u++) {
+    float t = 0.0f;
+    if (hipEventElapsedTime(&t, pool[2 * u], pool[2 * u + 1]) != hipSuccess)
+      return fail(EFWI_ERR_HIP, "event elapsed time");
+    ms[kind[u]] += t;
+    launches[kind[u]]++;
+  }
+  used = 0;
+  return EFWI_OK;
+}
+
+void Profiler::release() {
+  for (hipEvent_t e : pool) hipEventDestroy(e);
+  pool.clear();
+  kind.clear();
+  used = 0;
+}
+
+// ---------------------------------------------------------------------------
+// Context.
+// ---------------------------------------------------------------------------
+Ctx::~Ctx() {
+  int prev = 0;
+  hipGetDevice(&prev);
+  hipSetDevice(cfg.device);
+  prof.release();
+  for (void* p : allocs) hipFree(p);
+  if (ring) hipFree(ring);
+  hipSetDevice(prev);
+}
+
+int Ctx::alloc(void** p, size_t bytes) {
+  hipError_t e = hipMalloc(p, bytes);
+  if (e != hipSuccess) {
+    *p = nullptr;
+    return fail(EFWI_ERR_NOMEM, "hipMalloc(" + std::to_string(bytes) + " B) failed: " +
+                                    hipGetErrorString(e));
+  }
+  allocs.push_back(*p);
+  bytes_held += (long long)bytes;
+  return EFWI_OK;
+}
+
+static int grow(Ctx* c, void** p, size_t* cap, size_t need) {
+  if (*cap >= need) return EFWI_OK;
+  if (*p) {
+    for (auto& q : c->allocs)
+      if (q == *p) q = nullptr;
+    hipFree(*p);
+    c->bytes_held -= (long long)*cap;
+  }
+  int rc = c->alloc(p, need);
+  *cap = rc == EFWI_OK ? need : 0;
+  return rc;
+}
+
+int Ctx::init() {
+  const efwi_config& c = cfg;
+  if (c.nz < 8 || c.nx < 8 || c.ny < 8 || c.npml < 2 || c.nt < 2)
+    return fail(EFWI_ERR_ARG, "grid too small (need nz, nx, ny >= 8, npml >= 2, nt >= 2)");
+  if (c.nz < 2 * c.npml + 6 || c.nx < 2 * c.npml + 6 || c.ny < 2 * c.npml + 6)
+    return fail(EFWI_ERR_ARG, "each padded axis must exceed 2*npml + 6 cells");
+  if (!(c.dz > 0 && c.dx > 0 && c.dy > 0 && c.dt > 0))
+    return fail(EFWI_ERR_ARG, "grid steps and dt must be positive");
+  if (c.formulation != 0 && c.formulation != 1) return fail(EFWI_ERR_ARG, "formulation must be 0 or 1");
+  int ndev = 0;
+  HIP_TRY(hipGetDeviceCount(&ndev));
+  if (c.device < 0 || c.device >= ndev) return fail(EFWI_ERR_ARG, "device ordinal out of range");
+  HIP_TRY(hipSetDevice(c.device));
+  G.nz = c.nz; G.nx = c.nx; G.ny = c.ny; G.npml = c.npml;
+  G.py = c.ny;
+  G.pxy = (long)c.nx * c.ny;
+  G.dz = c.dz; G.dx = c.dx; G.dy = c.dy; G.dt = c.dt;
+  N = (long)c.nz * G.pxy;
+  const size_t fb = (size_t)N * sizeof(float);
+  float** wf[9] = {&W.vz, &W.vx, &W.vy, &W.szz, &W.sxx, &W.sxz, &W.syy, &W.sxy, &W.syz};
+  float** af[9] = {&A.vz, &A.vx, &A.vy, &A.szz, &A.sxx, &A.sxz, &A.syy, &A.sxy, &A.syz};
+  for (int k = 0; k < 9; k++) {
+    int rc = alloc((void**)wf[k], fb);
+    if (rc) return rc;
+    rc = alloc((void**)af[k], fb);
+    if (rc) return rc;
+  }
+  float** pf[18] = {&P.vz_z, &P.vz_x, &P.vz_y, &P.vx_z, &P.vx_x, &P.vx_y, &P.vy_z, &P.vy_x, &P.vy_y,
+                    &P.szz_z, &P.sxz_x, &P.sxz_z, &P.sxx_x, &P.sxy_x, &P.sxy_y, &P.syy_y, &P.syz_y,
+                    &P.syz_z};
+  for (int k = 0; k < 18; k++) {
+    int rc = alloc((void**)pf[k], fb);
+    if (rc) return rc;
+  }
+  int rc;
+  if ((rc = alloc((void**)&lam, fb)) || (rc = alloc((void**)&mu, fb)) || (rc = alloc((void**)&byc, fb)) ||
+      (rc = alloc((void**)&Gr.glam, fb)) || (rc = alloc((void**)&Gr.gmu, fb)) ||
+      (rc = alloc((void**)&Gr.grho, fb)))
+    return rc;
+  M.lam = lam; M.mu = mu; M.byc = byc;
+  // C-PML profiles: 3 axes x 6 rows.
+  const int n3[3] = {c.nz, c.nx, c.ny};
+  const float h3[3] = {c.dz, c.dx, c.dy};
+  const int nmax = std::max(c.nz, std::max(c.nx, c.ny));
+  float* prof = nullptr;
+  if ((rc = alloc((void**)&prof, sizeof(float) * 18 * nmax))) return rc;
+  std::vector<float> host(18 * (size_t)nmax, 0.0f);
+  for (int ax = 0; ax < 3; ax++) {
+    float* base = host.data() + (size_t)ax * 6 * nmax;
+    cpml_profile_1d(n3[ax], c.npml, h3[ax], c.f0, c.dt, base, base + nmax, base + 2 * nmax,
+                    base + 3 * nmax, base + 4 * nmax, base + 5 * nmax);
+    const float* d = prof + (size_t)ax * 6 * nmax;
+    C.K[ax] = d; C.a[ax] = d + nmax; C.b[ax] = d + 2 * nmax;
+    C.Kh[ax] = d + 3 * nmax; C.ah[ax] = d + 4 * nmax; C.bh[ax] = d + 5 * nmax;
+  }
+  HIP_TRY(hipMemcpy(prof, host.data(), sizeof(float) * host.size(), hipMemcpyHostToDevice));
+  // boundary band of the reverse reconstruction (Src/Boundary.cu:14-20)
+  for (int ax = 0; ax < 3; ax++) {
+    B.lo[ax] = c.npml - 2;
+    B.hi[ax] = n3[ax] - c.npml + 1;
+  }
+  const long nzB = B.hi[0] - B.lo[0] + 1, nxB = B.hi[1] - B.lo[1] + 1, nyB = B.hi[2] - B.lo[2] + 1;
+  B.nzs = 2 * 5 * nxB * nyB;
+  B.nxs = 2 * 5 * (nzB - 10) * nyB;
+  B.nys = 2 * 5 * (nzB - 10) * (nxB - 10);
+  B.n = B.nzs + B.nxs + B.nys;
+  if ((rc = alloc((void**)&msum, sizeof(double) * 8))) return rc;
+  if ((rc = alloc((void**)&partials, sizeof(double) * kMaxPartials))) return rc;
+  if ((rc = alloc((void**)&stf_dev, sizeof(float) * c.nt))) return rc;
+  if ((rc = alloc((void**)&w2_dev, sizeof(float) * c.nt))) return rc;
+  // plain taper of the data (ratio 0.005, Src/libCUFD.cu:64)
+  std::vector<float> w2(c.nt);
+  for (int it = 0; it < c.nt; it++) w2[it] = taper_plain_w2(it, c.nt, c.dt, 0.005f);
+  HIP_TRY(hipMemcpy(w2_dev, w2.data(), sizeof(float) * c.nt, hipMemcpyHostToDevice));
+  HIP_TRY(hipDeviceSynchronize());
+  return EFWI_OK;
+}
+
+static void fields9(const Wave& W, Fields9& F) {
+  F.f[0] = W.vz; F.f[1] = W.vx; F.f[2] = W.vy; F.f[3] = W.szz; F.f[4] = W.sxx;
+  F.f[5] = W.sxz; F.f[6] = W.syy; F.f[7] = W.sxy; F.f[8] = W.syz;
+}
+
+static void zero_wave(const Wave& W, long n, hipStream_t s) {
+  Fields9 F;
+  fields9(W, F);
+  for (int k = 0; k < 9; k++) launch_fill(F.f[k], n, 0.0f, s);
+}
+
+static void zero_psi(const Psi& P, long n, hipStream_t s) {
+  float* const* p = reinterpret_cast<float* const*>(&P);
+  for (int k = 0; k < 18; k++) launch_fill(p[k], n, 0.0f, s);
+}
+
+int Ctx::ensure_ring() {
+  const size_t need = (size_t)B.n * 9 * (size_t)(cfg.nt - 1) * sizeof(float);
+  if (ring && ring_bytes >= need) return EFWI_OK;
+  if (ring) {
+    hipFree(ring);
+    bytes_held -= (long long)ring_bytes;
+    ring = nullptr;
+  }
+  hipError_t e = hipMalloc((void**)&ring, need);
+  if (e != hipSuccess) {
+    ring = nullptr;
+    ring_bytes = 0;
+    return fail(EFWI_ERR_NOMEM, "boundary ring of " + std::to_string(need) +
+                                    " B does not fit in device memory: " + hipGetErrorString(e));
+  }
+  ring_bytes = need;
+  bytes_held += (long long)need;
+  return EFWI_OK;
+}
+
+int Ctx::set_model(const float* l, const float* m, const float* rho, int on_host, hipStream_t s) {
+  HIP_TRY(hipSetDevice(cfg.device));
+  const size_t fb = (size_t)N * sizeof(float);
+  const hipMemcpyKind k = on_host ? hipMemcpyHostToDevice : hipMemcpyDeviceToDevice;
+  HIP_TRY(hipMemcpyAsync(lam, l, fb, k, s));
+  HIP_TRY(hipMemcpyAsync(mu, m, fb, k, s));
+  float* rho_dev = const_cast<float*>(rho);
+  if (on_host) {
+    // stage rho through the gradient buffer, which is zeroed before use
+    HIP_TRY(hipMemcpyAsync(Gr.grho, rho, fb, hipMemcpyHostToDevice, s));
+    rho_dev = Gr.grho;
+  }
+  launch_byc(G, rho_dev, byc, s);
+  HIP_TRY(hipGetLastError());
+  if (on_host) launch_fill(Gr.grho, N, 0.0f, s);
+  model_set = true;
+  return EFWI_OK;
+}
+
+int Ctx::zero_gradient(hipStream_t s) {
+  HIP_TRY(hipSetDevice(cfg.device));
+  launch_fill(Gr.glam, N, 0.0f, s);
+  launch_fill(Gr.gmu, N, 0.0f, s);
+  launch_fill(Gr.grho, N, 0.0f, s);
+  HIP_TRY(hipGetLastError());
+  return EFWI_OK;
+}
+
+int Ctx::prepare_shot(const efwi_shot* sh, int calc_id) {
+  const int nt = cfg.nt;
+  if (!sh || !sh->stf) return fail(EFWI_ERR_ARG, "shot or its stf is NULL");
+  if (sh->nrec < 0) return fail(EFWI_ERR_ARG, "negative nrec");
+  if (sh->nrec > 0 && !sh->rec) return fail(EFWI_ERR_ARG, "receiver array is NULL");
+  auto inside = [&](int z, int x, int y) {
+    return z >= 0 && z < cfg.nz && x >= 0 && x < cfg.nx && y >= 0 && y < cfg.ny;
+  };
+  if (!inside(sh->src_z, sh->src_x, sh->src_y)) return fail(EFWI_ERR_ARG, "source outside the padded grid");
+  std::vector<long> cells(sh->nrec);
+  for (int r = 0; r < sh->nrec; r++) {
+    const int z = sh->rec[3 * r], x = sh->rec[3 * r + 1], y = sh->rec[3 * r + 2];
+    if (!inside(z, x, y))
+      return fail(EFWI_ERR_ARG, "receiver " + std::to_string(r) + " outside the padded grid");
+    cells[r] = (long)z * G.pxy + (long)x * G.py + y;
+  }
+  nrec = sh->nrec;
+  src_cell = (long)sh->src_z * G.pxy + (long)sh->src_x * G.py + sh->src_y;
+  const int p = cfg.npml;
+  src_interior = sh->src_z >= p && sh->src_z <= cfg.nz - 1 - p && sh->src_x >= p &&
+                 sh->src_x <= cfg.nx - 1 - p && sh->src_y >= p && sh->src_y <= cfg.ny - 1 - p;
+  // stf tapered with ratio 0.001 (Src/Src_Rec.cu:146)
+  std::vector<float> stf(nt);
+  for (int it = 0; it < nt; it++) stf[it] = sh->stf[it] * taper_plain_w2(it, nt, cfg.dt, 0.001f);
+  HIP_TRY(hipMemcpy(stf_dev, stf.data(), sizeof(float) * nt, hipMemcpyHostToDevice));
+  const size_t nrb = (size_t)std::max(nrec, 1);
+  int rc;
+  if ((rc = grow(this, (void**)&rec_dev, &rec_cap, sizeof(long) * nrb))) return rc;
+  if (nrec) HIP_TRY(hipMemcpy(rec_dev, cells.data(), sizeof(long) * nrec, hipMemcpyHostToDevice));
+  const size_t seis = sizeof(float) * (size_t)nt * 3 * nrb;
+  if ((rc = grow(this, (void**)&syn_t, &syn_cap, seis))) return rc;
+  if (calc_id != EFWI_CALC_SYNTHETIC) {
+    if ((rc = grow(this, (void**)&obs_t, &obs_cap, seis))) return rc;
+    if ((rc = grow(this, (void**)&res_t, &res_cap, seis))) return rc;
+    per_trace = cfg.if_win != 0;
+    if (per_trace) {
+      if (!sh->win_start || !sh->win_end) return fail(EFWI_ERR_ARG, "if_win needs win_start and win_end");
+      std::vector<float> fac((size_t)nt * nrb);
+      for (int it = 0; it < nt; it++)
+        for (int r = 0; r < nrec; r++)
+          fac[(size_t)it * nrec + r] =
+              taper_window_fac(it, nt, cfg.dt, 0.005f, sh->win_start[r], sh->win_end[r],
+                               sh->weights ? sh->weights[r] : 1.0f, sh->src_weight);
+      if ((rc = grow(this, (void**)&fac_dev, &fac_cap, sizeof(float) * fac.size()))) return rc;
+      HIP_TRY(hipMemcpy(fac_dev, fac.data(), sizeof(float) * fac.size(), hipMemcpyHostToDevice));
+    }
+  }
+  return EFWI_OK;
+}
+
+int Ctx::run_shot(const efwi_shot* sh, int calc_id, const float* obs, float* syn, float* res,
+                  double* misfit, hipStream_t s) {
+  if (calc_id < 0 || calc_id > 2) return fail(EFWI_ERR_ARG, "invalid calc_id " + std::to_string(calc_id));
+  if (!model_set) return fail(EFWI_ERR_STATE, "efwi_set_model was not called");
+  if (calc_id != EFWI_CALC_SYNTHETIC && !obs) return fail(EFWI_ERR_ARG, "observed data required");
+  HIP_TRY(hipSetDevice(cfg.device));
+  // Pageable copies below synchronise with the host; finish earlier work on s first.
+  HIP_TRY(hipStreamSynchronize(s));
+  int rc = prepare_shot(sh, calc_id);
+  if (rc) return rc;
+  const bool grad = calc_id == EFWI_CALC_GRADIENT;
+  if (grad && (rc = ensure_ring())) return rc;
+  const int nt = cfg.nt, form = cfg.formulation;
+  Fields9 F;
+  fields9(W, F);
+
+  zero_wave(W, N, s);
+  zero_wave(A, N, s);
+  zero_psi(P, N, s);
+  launch_fill(syn_t, (long)nt * 3 * std::max(nrec, 1), 0.0f, s);
+
+  // forward loop, Src/libCUFD.cu:312-354
+  for (int it = 0; it <= nt - 2; it++) {
+    if (grad) {
+      prof.begin(K_RING, s);
+      launch_ring(B, G, F, ring, it, 0, 9, 0, s);
+      prof.end(s);
+    }
+    prof.begin(K_STRESS_FWD, s);
+    launch_stress_fwd(form, G, W, P, M, C, s);
+    prof.end(s);
+    launch_add_source(W.szz, W.sxx, W.syy, src_cell, stf_dev, it, G.dt, +1, s);
+    prof.begin(K_VELOCITY_FWD, s);
+    launch_velocity_fwd(form, G, W, P, M, C, s);
+    prof.end(s);
+    launch_record(W, rec_dev, nrec, it + 1, syn_t, s);
+  }
+  HIP_TRY(hipGetLastError());
+  if (syn && nrec > 0) launch_from_time_major(syn_t, syn, nrec, nt, s);
+
+  if (calc_id != EFWI_CALC_SYNTHETIC && nrec > 0) {
+    // misfit and adjoint source, Src/libCUFD.cu:371-441
+    launch_to_time_major(obs, obs_t, nrec, nt, s);
+    const int nb = launch_misfit(obs_t, syn_t, res_t, per_trace ? fac_dev : w2_dev, per_trace ? 1 : 0,
+                                 nt, nrec, partials, kMaxPartials, s);
+    launch_sum_partials(partials, nb, msum, s);
+    HIP_TRY(hipGetLastError());
+    if (res) launch_from_time_major(res_t, res, nrec, nt, s);
+    double j = 0.0;
+    HIP_TRY(hipMemcpyAsync(&j, msum, sizeof(double), hipMemcpyDeviceToHost, s));
+    HIP_TRY(hipStreamSynchronize(s));
+    if (misfit) *misfit += j;
+  }
+
+  if (grad) {
+    // backward loop, Src/libCUFD.cu:469-613
+    zero_wave(A, N, s);
+    zero_psi(P, N, s);
+    Source S;
+    S.cell = src_interior ? src_cell : -1;
+    S.stf = stf_dev;
+    launch_velocity_adj(form, G, A, P, M, C, s);
+    launch_inject(A, rec_dev, nrec, nt - 1, res_t, s);
+    launch_stress_adj(G, A, P, M, C, s);
+    for (int it = nt - 2; it >= 0; it--) {
+      prof.begin(K_VELOCITY_REV, s);
+      launch_velocity_rev(form, G, W, A, M, Gr, s);
+      prof.end(s);
+      launch_ring(B, G, F, ring, it, 0, 3, 1, s);
+      if (!src_interior) launch_add_source(W.szz, W.sxx, W.syy, src_cell, stf_dev, it, G.dt, -1, s);
+      prof.begin(K_STRESS_REV, s);
+      launch_stress_rev(G, W, A, M, Gr, S, it, s);
+      prof.end(s);
+      launch_ring(B, G, F, ring, it, 3, 9, 1, s);
+      prof.begin(K_VELOCITY_ADJ, s);
+      launch_velocity_adj(form, G, A, P, M, C, s);
+      prof.end(s);
+      launch_inject(A, rec_dev, nrec, it, res_t, s);
+      prof.begin(K_STRESS_ADJ, s);
+      launch_stress_adj(G, A, P, M, C, s);
+      prof.end(s);
+    }
+    HIP_TRY(hipGetLastError());
+  }
+  if (prof.on && (rc = prof.collect())) return rc;
+  return EFWI_OK;
+}
+
+int Ctx::get_gradient(float* gl, float* gm, float* gr, int on_host, hipStream_t s) {
+  HIP_TRY(hipSetDevice(cfg.device));
+  const size_t fb = (size_t)N * sizeof(float);
+  const hipMemcpyKind k = on_host ? hipMemcpyDeviceToHost : hipMemcpyDeviceToDevice;
+  if (gl) HIP_TRY(hipMemcpyAsync(gl, Gr.glam, fb, k, s));
+  if (gm) HIP_TRY(hipMemcpyAsync(gm, Gr.gmu, fb, k, s));
+  if (gr) HIP_TRY(hipMemcpyAsync(gr, Gr.grho, fb, k, s));
+  if (on_host) HIP_TRY(hipStreamSynchronize(s));
+  return EFWI_OK;
+}
+
+}  // namespace efwi
+
+// ===========================================================================
+// C ABI
+// ===========================================================================
+using efwi::Ctx;
+using efwi::fail;
+
+extern "C" {
+
+int efwi_version(void) { return 100; }
+
+const char* efwi_last_error(void) { return efwi::g_err.c_str(); }
+
+int efwi_create(const efwi_config* cfg, efwi_ctx** out) {
+  if (!cfg || !out) return fail(EFWI_ERR_ARG, "NULL argument");
+  *out = nullptr;
+  Ctx* c = new Ctx();
+  c->cfg = *cfg;
+  int rc = c->init();
+  if (rc) {
+    delete c;
+    return rc;
+  }
+  *out = reinterpret_cast<efwi_ctx*>(c);
+  return EFWI_OK;
+}
+
+void efwi_destroy(efwi_ctx* ctx) { delete reinterpret_cast<Ctx*>(ctx); }
+
+int efwi_set_model(efwi_ctx* ctx, const float* lambda, const float* mu, const float* rho, int on_host,
+                   void* stream) {
+  if (!ctx || !lambda || !mu || !rho) return fail(EFWI_ERR_ARG, "NULL argument");
+  return reinterpret_cast<Ctx*>(ctx)->set_model(lambda, mu, rho, on_host, (hipStream_t)stream);
+}
+
+int efwi_zero_gradient(efwi_ctx* ctx, void* stream) {
+  if (!ctx) return fail(EFWI_ERR_ARG, "NULL context");
+  return reinterpret_cast<Ctx*>(ctx)->zero_gradient((hipStream_t)stream);
+}
+
+int efwi_run_shot(efwi_ctx* ctx, const efwi_shot* shot, int calc_id, const float* obs, float* syn,
+                  float* res, double* misfit, void* stream) {
+  if (!ctx) return fail(EFWI_ERR_ARG, "NULL context");
+  return reinterpret_cast<Ctx*>(ctx)->run_shot(shot, calc_id, obs, syn, res, misfit, (hipStream_t)stream);
+}
+
+int efwi_get_gradient(efwi_ctx* ctx, float* g_lambda, float* g_mu, float* g_rho, int on_host,
+                      void* stream) {
+  if (!ctx) return fail(EFWI_ERR_ARG, "NULL context");
+  return reinterpret_cast<Ctx*>(ctx)->get_gradient(g_lambda, g_mu, g_rho, on_host, (hipStream_t)stream);
+}
+
+int efwi_profile(efwi_ctx* ctx, int enable) {
+  if (!ctx) return fail(EFWI_ERR_ARG, "NULL context");
+  Ctx* c = reinterpret_cast<Ctx*>(ctx);
+  c->prof.on = enable != 0;
+  return EFWI_OK;
+}
+
+int efwi_profile_read(efwi_ctx* ctx, double* ms, long long* launches, int reset) {
+  if (!ctx || !ms || !launches) return fail(EFWI_ERR_ARG, "NULL argument");
+  Ctx* c = reinterpret_cast<Ctx*>(ctx);
+  int rc = c->prof.collect();
+  if (rc) return rc;
+  for (int k = 0; k < efwi::K_COUNT; k++) {
+    ms[k] = c->prof.ms[k];
+    launches[k] = c->prof.launches[k];
+    if (reset) {
+      c->prof.ms[k] = 0.0;
+      c->prof.launches[k] = 0;
+    }
+  }
+  return EFWI_OK;
+}
+
+long long efwi_device_bytes(const efwi_ctx* ctx) {
+  return ctx ? reinterpret_cast<const Ctx*>(ctx)->bytes_held : 0;
+}
+
+int efwi_cpml_profile(int n, int npml, float h, float f0, float dt, float* out) {
+  if (n <= 0 || !out) return fail(EFWI_ERR_ARG, "bad argument");
+  efwi::cpml_profile_1d(n, npml, h, f0, dt, out, out + n, out + 2 * n, out + 3 * n, out + 4 * n, out + 5 * n);
+  return EFWI_OK;
+}
+
+int efwi_taper_w2(int nt, float dt, float ratio, float* out) {
+  if (nt <= 0 || !out) return fail(EFWI_ERR_ARG, "bad argument");
+  for (int it = 0; it < nt; it++) out[it] = efwi::taper_plain_w2(it, nt, dt, ratio);
+  return EFWI_OK;
+}
+
+}  // extern "C"

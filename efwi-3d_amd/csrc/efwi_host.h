@@ -1,0 +1,90 @@
+// Context of the gfx950 elastic FWI library (host side).
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <string>
+#include <vector>
+
+#include "../../include/efwi.h"
+#include "efwi_common.h"
+#include "efwi_kernels.h"
+
+namespace efwi {
+
+int fail(int code, const std::string& msg);
+void cpml_profile_1d(int n, int npml, float h, float f0, float dt, float* K, float* a, float* b,
+                     float* Kh, float* ah, float* bh);
+float taper_plain_w2(int it, int nt, float dt, float ratio);
+float taper_window_fac(int it, int nt, float dt, float ratio, float t0, float t3, float weight,
+                       float src_weight);
+
+// Optional per-kernel timing with HIP events on the launch stream.
+enum KernelId { K_STRESS_FWD, K_VELOCITY_FWD, K_STRESS_REV, K_VELOCITY_REV, K_VELOCITY_ADJ, K_STRESS_ADJ,
+                K_RING, K_OTHER, K_COUNT };
+
+struct Profiler {
+  bool on = false;
+  std::vector<hipEvent_t> pool;
+  std::vector<int> kind;  // kernel id of each recorded pair
+  size_t used = 0;        // pairs recorded since the last collect
+  double ms[K_COUNT] = {0};
+  long long launches[K_COUNT] = {0};
+  void begin(int k, hipStream_t s);
+  void end(hipStream_t s);
+  int collect();  // synchronises on the recorded events and accumulates
+  void release();
+};
+
+struct Ctx {
+  static constexpr int kMaxPartials = 4096;
+  Profiler prof;
+  efwi_config cfg{};
+  Geom G{};
+  long N = 0;
+  Wave W{}, A{};
+  Psi P{};
+  Model M{};
+  Grad Gr{};
+  Cpml C{};
+  Band B{};
+  float *lam = nullptr, *mu = nullptr, *byc = nullptr;
+  bool model_set = false;
+  // ring of the reverse reconstruction: [nt-1][9][B.n]
+  float* ring = nullptr;
+  size_t ring_bytes = 0;
+  // per-shot buffers
+  float* stf_dev = nullptr;
+  float* w2_dev = nullptr;
+  long* rec_dev = nullptr;
+  size_t rec_cap = 0;
+  float* syn_t = nullptr;
+  size_t syn_cap = 0;
+  float* obs_t = nullptr;
+  size_t obs_cap = 0;
+  float* res_t = nullptr;
+  size_t res_cap = 0;
+  float* fac_dev = nullptr;
+  size_t fac_cap = 0;
+  double* partials = nullptr;
+  double* msum = nullptr;
+  int nrec = 0;
+  long src_cell = 0;
+  bool src_interior = true;
+  bool per_trace = false;
+  std::vector<void*> allocs;
+  long long bytes_held = 0;
+
+  ~Ctx();
+  int alloc(void** p, size_t bytes);
+  int init();
+  int ensure_ring();
+  int set_model(const float* l, const float* m, const float* rho, int on_host, hipStream_t s);
+  int zero_gradient(hipStream_t s);
+  int prepare_shot(const efwi_shot* sh, int calc_id);
+  int run_shot(const efwi_shot* sh, int calc_id, const float* obs, float* syn, float* res,
+               double* misfit, hipStream_t s);
+  int get_gradient(float* gl, float* gm, float* gr, int on_host, hipStream_t s);
+};
+
+}  // namespace efwi

@@ -1,0 +1,58 @@
+// Host-side launchers of the gfx950 kernels (defined in the .hip files).
+#pragma once
+
+#include "efwi_common.h"
+
+namespace efwi {
+
+// Boundary band of the reverse reconstruction: the box [lo, hi] (inclusive,
+// per axis z, x, y) minus its inner box [lo+5, hi-5], stored as three groups
+// of non-overlapping slabs (z slabs, then x slabs, then y slabs), each in
+// (z, x, y) row-major order so that consecutive cells are consecutive in y.
+struct Band {
+  int lo[3], hi[3];
+  long nzs, nxs, nys;  // cells in the z-, x- and y-slab groups
+  long n;              // total cells per field
+};
+
+// Nine field pointers in the ring order vz, vx, vy, szz, sxx, sxz, syy, sxy, syz.
+struct Fields9 {
+  float* f[9];
+};
+
+// stencil kernels (efwi_stencil.hip)
+void launch_stress_fwd(int form, const Geom& G, const Wave& W, const Psi& P, const Model& M,
+                       const Cpml& C, hipStream_t s);
+void launch_velocity_fwd(int form, const Geom& G, const Wave& W, const Psi& P, const Model& M,
+                         const Cpml& C, hipStream_t s);
+void launch_stress_rev(const Geom& G, const Wave& W, const Wave& A, const Model& M, const Grad& Gr,
+                       const Source& S, int it, hipStream_t s);
+void launch_velocity_rev(int form, const Geom& G, const Wave& W, const Wave& A, const Model& M,
+                         const Grad& Gr, hipStream_t s);
+void launch_velocity_adj(int form, const Geom& G, const Wave& A, const Psi& P, const Model& M,
+                         const Cpml& C, hipStream_t s);
+void launch_stress_adj(const Geom& G, const Wave& A, const Psi& P, const Model& M, const Cpml& C,
+                       hipStream_t s);
+
+// auxiliary kernels (efwi_aux.hip)
+void launch_fill(float* p, long n, float v, hipStream_t s);
+void launch_byc(const Geom& G, const float* rho, float* byc, hipStream_t s);
+void launch_add_source(float* szz, float* sxx, float* syy, long cell, const float* stf, int it,
+                       float dt, int sign, hipStream_t s);
+void launch_record(const Wave& W, const long* rec_cell, int nrec, int it, float* syn_t,
+                   hipStream_t s);
+void launch_inject(const Wave& A, const long* rec_cell, int nrec, int it, const float* res_t,
+                   hipStream_t s);
+void launch_ring(const Band& B, const Geom& G, const Fields9& F, float* ring, long slot, int k0,
+                 int k1, int restore, hipStream_t s);
+// obs_t/syn_t/res_t: [nt][3][nrec]; fac: [nt] (per_trace = 0) or [nt][nrec].
+// Writes per-block double partial sums of r^2 to partials (nblocks entries) and
+// returns nblocks.
+int launch_misfit(const float* obs_t, const float* syn_t, float* res_t, const float* fac,
+                  int per_trace, int nt, int nrec, double* partials, int max_blocks, hipStream_t s);
+void launch_sum_partials(const double* partials, int n, double* out, hipStream_t s);
+// [3][nrec][nt] <-> [nt][3][nrec]
+void launch_to_time_major(const float* src, float* dst, int nrec, int nt, hipStream_t s);
+void launch_from_time_major(const float* src, float* dst, int nrec, int nt, hipStream_t s);
+
+}  // namespace efwi

@@ -1,0 +1,190 @@
+"""Device-resident elastic propagator: a thin Python face of the efwi_ctx C API.
+
+Tensors stay on the GPU: the model, observed data, seismograms and gradients
+are torch CUDA tensors in the torch layout [nz][nx][ny] (padded grid) and
+[3][nrec][nt] (x, y, z) for seismograms. All work is queued on torch's
+current stream.
+"""
+from __future__ import annotations
+
+import ctypes as C
+from dataclasses import dataclass, field
+
+import numpy as np
+import torch
+
+from . import _native as N
+
+
+@dataclass(frozen=True)
+class GridSpec:
+    """Padded grid, time stepping and options (the para JSON of the reference)."""
+    nz: int
+    nx: int
+    ny: int
+    npml: int
+    nt: int
+    dz: float
+    dx: float
+    dy: float
+    dt: float
+    f0: float
+    formulation: int = 0     # 0 reference, 1 consistent
+    if_win: bool = False
+
+    @property
+    def shape(self):
+        return (self.nz, self.nx, self.ny)
+
+    @property
+    def ncells(self):
+        return self.nz * self.nx * self.ny
+
+    def interior_cells(self):
+        p = self.npml
+        return (self.nz - 2 * p) * (self.nx - 2 * p) * (self.ny - 2 * p)
+
+
+@dataclass
+class ShotSpec:
+    """One shot on the padded grid (the survey JSON value plus npml)."""
+    src: tuple                     # (z, x, y)
+    stf: np.ndarray                # [nt] untapered source time function
+    rec: np.ndarray                # [nrec, 3] (z, x, y)
+    win_start: np.ndarray | None = None
+    win_end: np.ndarray | None = None
+    weights: np.ndarray | None = None
+    src_weight: float = 1.0
+    _keep: list = field(default_factory=list, repr=False)
+
+    @property
+    def nrec(self):
+        return int(len(self.rec))
+
+    def c_struct(self) -> N.Shot:
+        stf = np.ascontiguousarray(self.stf, dtype=np.float32)
+        rec = np.ascontiguousarray(np.asarray(self.rec, dtype=np.int32).reshape(-1, 3))
+        keep = [stf, rec]
+
+        def fp(a):
+            if a is None:
+                return None
+            a = np.ascontiguousarray(a, dtype=np.float32)
+            keep.append(a)
+            return a.ctypes.data_as(C.POINTER(C.c_float))
+
+        s = N.Shot(int(self.src[0]), int(self.src[1]), int(self.src[2]),
+                   stf.ctypes.data_as(C.POINTER(C.c_float)), rec.shape[0],
+                   rec.ctypes.data_as(C.POINTER(C.c_int)), fp(self.win_start), fp(self.win_end),
+                   fp(self.weights), float(self.src_weight))
+        self._keep = keep
+        return s
+
+
+class Propagator:
+    """Owns one efwi_ctx (device state of one grid on one GPU)."""
+
+    def __init__(self, grid: GridSpec, device: int | None = None):
+        if not torch.cuda.is_available():
+            raise N.EfwiError("no HIP device visible: the propagator runs on the GPU only")
+        self.grid = grid
+        self.device = torch.cuda.current_device() if device is None else int(device)
+        cfg = N.Config(grid.nz, grid.nx, grid.ny, grid.npml, grid.nt, grid.dz, grid.dx, grid.dy,
+                       grid.dt, grid.f0, int(grid.formulation), int(bool(grid.if_win)), self.device)
+        h = C.c_void_p()
+        with torch.cuda.device(self.device):
+            N.check(N.lib().efwi_create(C.byref(cfg), C.byref(h)), "efwi_create")
+        self._h = h
+
+    # -- lifetime ---------------------------------------------------------
+    def close(self):
+        if getattr(self, "_h", None) is not None and self._h.value:
+            torch.cuda.synchronize(self.device)
+            N.lib().efwi_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *a):
+        self.close()
+
+    @property
+    def device_bytes(self) -> int:
+        return int(N.lib().efwi_device_bytes(self._h))
+
+    def profile(self, enable: bool = True):
+        """Time every stencil/ring launch with HIP events on the launch stream."""
+        N.check(N.lib().efwi_profile(self._h, int(enable)), "efwi_profile")
+
+    def profile_read(self, reset: bool = True) -> dict:
+        """{kernel: (total_ms, launches)} since the last reset."""
+        ms = (C.c_double * len(N.KERNELS))()
+        n = (C.c_longlong * len(N.KERNELS))()
+        N.check(N.lib().efwi_profile_read(self._h, ms, n, int(reset)), "efwi_profile_read")
+        return {k: (ms[i], int(n[i])) for i, k in enumerate(N.KERNELS)}
+
+    def _stream(self):
+        return N.stream_handle(self.device)
+
+    def _dev(self, t, shape=None):
+        t = torch.as_tensor(t)
+        if shape is not None and tuple(t.shape) != tuple(shape):
+            raise ValueError(f"expected shape {tuple(shape)}, got {tuple(t.shape)}")
+        return t.to(device=f"cuda:{self.device}", dtype=torch.float32).contiguous()
+
+    # -- model / gradient -------------------------------------------------
+    def set_model(self, lam, mu, rho):
+        """lambda, mu and rho (g/cc) on the padded grid [nz][nx][ny]."""
+        g = self.grid
+        l, m, r = (self._dev(a, g.shape) for a in (lam, mu, rho))
+        with torch.cuda.device(self.device):
+            N.check(N.lib().efwi_set_model(self._h, N.vptr(l), N.vptr(m), N.vptr(r), 0, self._stream()),
+                    "efwi_set_model")
+            # the copies are queued on the stream; keep the sources alive until then
+            torch.cuda.current_stream(self.device).synchronize()
+
+    def zero_gradient(self):
+        with torch.cuda.device(self.device):
+            N.check(N.lib().efwi_zero_gradient(self._h, self._stream()), "efwi_zero_gradient")
+
+    def gradient(self):
+        """(g_lambda, g_mu, g_rho) accumulated since the last zero_gradient()."""
+        g = self.grid
+        out = [torch.empty(g.shape, dtype=torch.float32, device=f"cuda:{self.device}") for _ in range(3)]
+        with torch.cuda.device(self.device):
+            N.check(N.lib().efwi_get_gradient(self._h, *(N.vptr(t) for t in out), 0, self._stream()),
+                    "efwi_get_gradient")
+        return tuple(out)
+
+    # -- shots ------------------------------------------------------------
+    def run_shot(self, shot: ShotSpec, calc_id: int, obs=None, want_syn=False, want_res=False):
+        """Forward (+ misfit, + backward for calc_id 1) of one shot.
+
+        Returns (misfit, syn, res): misfit is the shot's FP64 sum of squared
+        windowed residuals (None for calc_id 2); syn/res are [3][nrec][nt]
+        CUDA tensors when requested.
+        """
+        g = self.grid
+        nrec = shot.nrec
+        dev = f"cuda:{self.device}"
+        shp = (3, nrec, g.nt)
+        obs_t = None
+        if calc_id != N.CALC_SYNTHETIC:
+            if obs is None:
+                raise ValueError("observed data required for calc_id 0/1")
+            obs_t = self._dev(obs, shp)
+        syn = torch.empty(shp, dtype=torch.float32, device=dev) if (want_syn or calc_id == N.CALC_SYNTHETIC) else None
+        res = torch.empty(shp, dtype=torch.float32, device=dev) if want_res else None
+        j = C.c_double(0.0)
+        cs = shot.c_struct()
+        with torch.cuda.device(self.device):
+            N.check(N.lib().efwi_run_shot(self._h, C.byref(cs), int(calc_id), N.vptr(obs_t), N.vptr(syn),
+                                          N.vptr(res), C.byref(j), self._stream()), "efwi_run_shot")
+        return (None if calc_id == N.CALC_SYNTHETIC else j.value), syn, res

@@ -1,0 +1,134 @@
+/* C ABI of the MI355X (gfx950) elastic FWI propagation library, libefwi.so.
+ *
+ * Drop-in boundary for the reference's native path (guoketing/EFWI-3D,
+ * Ops/FWI/Src_3d). Plain C types only; every entry point returns 0 on success
+ * and a negative code on error, with the message in efwi_last_error(). No
+ * entry point exits the process.
+ *
+ * Volumes use the torch layout [nz][nx][ny] (y fastest) of the PML-padded
+ * grid. Seismograms use [3][nrec][nt] with component order x, y, z, the
+ * layout of the reference's Shot_x/Shot_y/Shot_z files. "Device" pointers are
+ * HIP device memory of the context's GPU; a hipStream_t is passed as void*.
+ */
+#ifndef EFWI_H_
+#define EFWI_H_
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define EFWI_OK 0
+#define EFWI_ERR_ARG (-1)
+#define EFWI_ERR_HIP (-2)
+#define EFWI_ERR_IO (-3)
+#define EFWI_ERR_STATE (-4)
+#define EFWI_ERR_NOMEM (-5)
+
+/* calc_id of the reference (Src/libCUFD.cu:25-28, Src/Parameter.cpp:132-144). */
+#define EFWI_CALC_MISFIT 0
+#define EFWI_CALC_GRADIENT 1
+#define EFWI_CALC_SYNTHETIC 2
+
+typedef struct efwi_ctx efwi_ctx;
+
+/* Grid, time stepping and options. Replaces the para JSON that Parameter
+ * reads (Src/Parameter.cpp:41-184): nz/nx/ny are padded sizes, f0 only sets
+ * the C-PML alpha profile (Src/utilities.cu:215). */
+typedef struct {
+  int nz, nx, ny;     /* padded grid */
+  int npml;           /* nPoints_pml */
+  int nt;             /* nSteps */
+  float dz, dx, dy;   /* m */
+  float dt;           /* s */
+  float f0;           /* Hz */
+  int formulation;    /* 0 = reference (default), 1 = consistent (quirks corrected) */
+  int if_win;         /* per-trace windows and weights (Src/utilities.cu:524-576) */
+  int device;         /* HIP device ordinal */
+} efwi_config;
+
+/* One shot. Replaces one "shot<i>" object of the survey JSON
+ * (Src/Src_Rec.cu:74-206); coordinates are on the padded grid (the JSON value
+ * plus npml). Host pointers. */
+typedef struct {
+  int src_z, src_x, src_y;
+  const float* stf;       /* nt samples, untapered (the row stf[shot_id]) */
+  int nrec;
+  const int* rec;         /* [nrec][3] (z, x, y) */
+  const float* win_start; /* [nrec] seconds, if_win only */
+  const float* win_end;   /* [nrec] seconds, if_win only */
+  const float* weights;   /* [nrec] or NULL (= 1), if_win only */
+  float src_weight;       /* if_win only (1 when absent from the JSON) */
+} efwi_shot;
+
+/* Library / error reporting. */
+int efwi_version(void);
+const char* efwi_last_error(void);
+
+/* Context lifetime. Allocates the wavefield, C-PML and gradient state on the
+ * device; no reference counterpart (cufd allocates and frees per call,
+ * Src/libCUFD.cu:154-196,756-795). */
+int efwi_create(const efwi_config* cfg, efwi_ctx** out);
+void efwi_destroy(efwi_ctx* ctx);
+
+/* Model in the op's units (lambda, mu in the Mu/Lambda units of FWI_3d, rho
+ * in g/cc), [nz][nx][ny]. Copies lambda and mu and derives the buoyancy
+ * (Src/Model.cu:15-80). on_host = 1 for host pointers. Replaces the model
+ * half of cufd's set-up (Src/libCUFD.cu:67-88). */
+int efwi_set_model(efwi_ctx* ctx, const float* lambda, const float* mu, const float* rho,
+                   int on_host, void* stream);
+
+/* Zero the gradient accumulators (Src/Model.cu:62-64). */
+int efwi_zero_gradient(efwi_ctx* ctx, void* stream);
+
+/* Run one shot: the forward loop (Src/libCUFD.cu:312-354), and for calc_id
+ * 0/1 the misfit and adjoint source (:371-441), and for calc_id 1 the
+ * backward loop with reverse reconstruction and imaging (:469-613), which
+ * accumulates into the gradients.
+ *   obs     device [3][nrec][nt] observed data (calc_id 0/1), else NULL
+ *   syn     device [3][nrec][nt] out: raw recorded data, or NULL
+ *   res     device [3][nrec][nt] out: windowed residual (adjoint source), or NULL
+ *   misfit  host, the shot's sum of squared residuals is ADDED to *misfit (FP64)
+ */
+int efwi_run_shot(efwi_ctx* ctx, const efwi_shot* shot, int calc_id, const float* obs,
+                  float* syn, float* res, double* misfit, void* stream);
+
+/* Copy the accumulated gradients out ([nz][nx][ny]); on_host = 1 for host
+ * pointers. Replaces Src/libCUFD.cu:636-652. */
+int efwi_get_gradient(efwi_ctx* ctx, float* g_lambda, float* g_mu, float* g_rho, int on_host,
+                      void* stream);
+
+/* Per-kernel timing with HIP events recorded on the launch stream around
+ * every stencil and ring launch of efwi_run_shot (no reference counterpart;
+ * the reference only prints wall-clock totals, Src/libCUFD.cu:210-216).
+ * efwi_profile_read fills EFWI_NKERNEL totals in milliseconds and launch
+ * counts, indexed stress_fwd, velocity_fwd, stress_rev, velocity_rev,
+ * velocity_adj, stress_adj, ring, other. */
+#define EFWI_NKERNEL 8
+int efwi_profile(efwi_ctx* ctx, int enable);
+int efwi_profile_read(efwi_ctx* ctx, double* ms, long long* launches, int reset);
+
+/* Bytes of device memory held by the context (state + ring). */
+long long efwi_device_bytes(const efwi_ctx* ctx);
+
+/* Host utilities exposed for tests: the 1-D C-PML profile of one axis as six
+ * rows K, a, b, K_half, a_half, b_half (Src/utilities.cu:204-320), and the
+ * squared time taper w(t)^2 (Src/utilities.cu:578-614). */
+int efwi_cpml_profile(int n, int npml, float h, float f0, float dt, float* out6n);
+int efwi_taper_w2(int nt, float dt, float ratio, float* out);
+
+/* The reference's native entry, cufd (Src/libCUFD.h:6-9, Src/libCUFD.cu:37),
+ * with a C string for the para file name and a status return instead of
+ * exit(). Host pointers throughout, models in [nz][nx][ny]; reads the para
+ * and survey JSON and, for calc_id 0/1, data_dir/Shot_{x,y,z}<id>.bin; for
+ * calc_id 2 writes them. Gradients and misfit are written (not accumulated)
+ * over the shots of this call. gpu_id selects the HIP device. */
+int efwi_cufd(float* misfit, float* grad_lambda, float* grad_mu, float* grad_rho,
+              const float* lambda, const float* mu, const float* rho, const float* stf,
+              int calc_id, int gpu_id, int group_size, const int* shot_ids,
+              const char* para_fname);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* EFWI_H_ */
