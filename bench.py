@@ -21,15 +21,18 @@ ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, os.path.join(ROOT, "efwi-3d_amd"))
 
 HBM_PEAK_GBS = 8000.0
-# Algorithmic FP32 traffic per updated cell of each kernel, with an ideal
-# cache (fields read + written once): the reference's kernel decomposition.
-ALG_FLOATS = {"stress_fwd": 17, "velocity_fwd": 13, "velocity_rev": 18, "stress_rev": 27,
-              "velocity_adj": 15, "stress_adj": 18}
+# Algorithmic FP32 traffic with an ideal cache (every field read and written
+# once) of the reference's kernels, in floats per cell: (per padded cell, per
+# interior cell). The backward kernels here fuse the reverse and the adjoint
+# kernel of the reference: velocity_bwd = velocity(rev) 18 + velocity_adj 15,
+# stress_bwd = stress(rev) 27 + stress_adj 18.
+ALG_FLOATS = {"stress_fwd": (17, 0), "velocity_fwd": (13, 0), "velocity_bwd": (15, 18),
+              "stress_bwd": (18, 27), "velocity_adj": (15, 0), "stress_adj": (18, 0)}
 
 
 def alg_bytes_per_launch(kernel, grid):
-    units = grid.interior_cells() if kernel in ("velocity_rev", "stress_rev") else grid.ncells
-    return 4.0 * ALG_FLOATS[kernel] * units
+    per_pad, per_int = ALG_FLOATS[kernel]
+    return 4.0 * (per_pad * grid.ncells + per_int * grid.interior_cells())
 
 
 def alg_bytes_pipeline(grid):

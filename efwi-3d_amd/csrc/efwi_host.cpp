@@ -358,6 +358,8 @@ int Ctx::prepare_shot(const efwi_shot* sh, int calc_id) {
   const int p = cfg.npml;
   src_interior = sh->src_z >= p && sh->src_z <= cfg.nz - 1 - p && sh->src_x >= p &&
                  sh->src_x <= cfg.nx - 1 - p && sh->src_y >= p && sh->src_y <= cfg.ny - 1 - p;
+  src_stencil = sh->src_z >= 2 && sh->src_z <= cfg.nz - 3 && sh->src_x >= 2 && sh->src_x <= cfg.nx - 3 &&
+                sh->src_y >= 2 && sh->src_y <= cfg.ny - 3;
   // stf tapered with ratio 0.001 (Src/Src_Rec.cu:146)
   std::vector<float> stf(nt);
   for (int it = 0; it < nt; it++) stf[it] = sh->stf[it] * taper_plain_w2(it, nt, cfg.dt, 0.001f);
@@ -409,6 +411,9 @@ int Ctx::run_shot(const efwi_shot* sh, int calc_id, const float* obs, float* syn
   launch_fill(syn_t, (long)nt * 3 * std::max(nrec, 1), 0.0f, s);
 
   // forward loop, Src/libCUFD.cu:312-354
+  Source Sf;
+  Sf.cell = src_stencil ? src_cell : -1;
+  Sf.stf = stf_dev;
   for (int it = 0; it <= nt - 2; it++) {
     if (grad) {
       prof.begin(K_RING, s);
@@ -416,9 +421,9 @@ int Ctx::run_shot(const efwi_shot* sh, int calc_id, const float* obs, float* syn
       prof.end(s);
     }
     prof.begin(K_STRESS_FWD, s);
-    launch_stress_fwd(form, G, W, P, M, C, s);
+    launch_stress_fwd(G, W, P, M, C, Sf, it, s);
     prof.end(s);
-    launch_add_source(W.szz, W.sxx, W.syy, src_cell, stf_dev, it, G.dt, +1, s);
+    if (!src_stencil) launch_add_source(W.szz, W.sxx, W.syy, src_cell, stf_dev, it, G.dt, +1, s);
     prof.begin(K_VELOCITY_FWD, s);
     launch_velocity_fwd(form, G, W, P, M, C, s);
     prof.end(s);
@@ -442,7 +447,12 @@ int Ctx::run_shot(const efwi_shot* sh, int calc_id, const float* obs, float* syn
   }
 
   if (grad) {
-    // backward loop, Src/libCUFD.cu:469-613
+    // backward loop, Src/libCUFD.cu:469-613. Per step the reference runs
+    //   velocity(rev) ; restore v ; source(-) ; stress(rev) ; restore s ;
+    //   velocity_adj ; inject ; stress_adj
+    // Here velocity_adj joins the reverse velocity and stress_adj the reverse
+    // stress: neither reads what the kernels moved past it write (v, s and
+    // their rings), and inject only has to precede stress_adj.
     zero_wave(A, N, s);
     zero_psi(P, N, s);
     Source S;
@@ -452,22 +462,16 @@ int Ctx::run_shot(const efwi_shot* sh, int calc_id, const float* obs, float* syn
     launch_inject(A, rec_dev, nrec, nt - 1, res_t, s);
     launch_stress_adj(G, A, P, M, C, s);
     for (int it = nt - 2; it >= 0; it--) {
-      prof.begin(K_VELOCITY_REV, s);
-      launch_velocity_rev(form, G, W, A, M, Gr, s);
+      prof.begin(K_VELOCITY_BWD, s);
+      launch_velocity_bwd(form, G, W, A, P, M, C, Gr, s);
       prof.end(s);
       launch_ring(B, G, F, ring, it, 0, 3, 1, s);
       if (!src_interior) launch_add_source(W.szz, W.sxx, W.syy, src_cell, stf_dev, it, G.dt, -1, s);
-      prof.begin(K_STRESS_REV, s);
-      launch_stress_rev(G, W, A, M, Gr, S, it, s);
+      launch_inject(A, rec_dev, nrec, it, res_t, s);
+      prof.begin(K_STRESS_BWD, s);
+      launch_stress_bwd(G, W, A, P, M, C, Gr, S, it, s);
       prof.end(s);
       launch_ring(B, G, F, ring, it, 3, 9, 1, s);
-      prof.begin(K_VELOCITY_ADJ, s);
-      launch_velocity_adj(form, G, A, P, M, C, s);
-      prof.end(s);
-      launch_inject(A, rec_dev, nrec, it, res_t, s);
-      prof.begin(K_STRESS_ADJ, s);
-      launch_stress_adj(G, A, P, M, C, s);
-      prof.end(s);
     }
     HIP_TRY(hipGetLastError());
   }

@@ -20,7 +20,7 @@ float taper_window_fac(int it, int nt, float dt, float ratio, float t0, float t3
                        float src_weight);
 
 // Optional per-kernel timing with HIP events on the launch stream.
-enum KernelId { K_STRESS_FWD, K_VELOCITY_FWD, K_STRESS_REV, K_VELOCITY_REV, K_VELOCITY_ADJ, K_STRESS_ADJ,
+enum KernelId { K_STRESS_FWD, K_VELOCITY_FWD, K_STRESS_BWD, K_VELOCITY_BWD, K_VELOCITY_ADJ, K_STRESS_ADJ,
                 K_RING, K_OTHER, K_COUNT };
 
 struct Profiler {
@@ -70,7 +70,8 @@ struct Ctx {
   double* msum = nullptr;
   int nrec = 0;
   long src_cell = 0;
-  bool src_interior = true;
+  bool src_interior = true;   // inside [npml, n-1-npml]^3: subtraction fused into the reverse stress
+  bool src_stencil = true;    // inside [2, n-3]^3: injection fused into the forward stress
   bool per_trace = false;
   std::vector<void*> allocs;
   long long bytes_held = 0;

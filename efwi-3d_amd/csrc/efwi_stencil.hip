@@ -1,15 +1,27 @@
-// Staggered-grid elastic stencil kernels for gfx950 (forward, reverse
-// reconstruction with imaging condition, adjoint).
+// Staggered-grid elastic stencil kernels for gfx950: forward, reverse
+// reconstruction with imaging condition, and adjoint.
 //
-// Every kernel evaluates, per cell, the same expression tree as the reference
+// Arithmetic: every cell evaluates the same expression tree as the reference
 // kernel it replaces, with IEEE division and no fused multiply-add (this file
-// is compiled with -ffp-contract=off), so that the float results are the
-// correctly rounded results of the reference's float expressions.
+// is compiled with -ffp-contract=off), so the float results are the correctly
+// rounded results of the reference's float expressions. The only liberty
+// taken is to skip a C-PML term a*d(psi) where the profile coefficient a is
+// exactly zero (the reference multiplies by zero there), which can change the
+// sign of an exact zero and nothing else.
 //
-// Launch shape: 64 lanes along y (the contiguous axis, so every wavefront
-// load is one coalesced row segment) x 4 rows along x per workgroup; one
-// workgroup row per z plane. Neighbours along x and z come through the L1/L2
-// caches.
+// Work distribution ("sweep"): a 1-D grid of one-wavefront workgroups, each a
+// 64-cell segment of one y-row. Workgroup b runs on XCD b % 8 (round-robin
+// dispatch); the kernels map b so that each XCD owns a contiguous strip of x
+// rows and walks it plane by plane in z. The x and z neighbours a cell reads
+// were then fetched into that XCD's L2 by the segments just before it, so
+// every field is read from HBM about once per launch.
+//
+// Fusion: the source injection is folded into the forward stress kernel; the
+// backward pass runs two kernels per step, {reverse velocity + density
+// imaging + adjoint velocity} and {source subtraction + reverse stress +
+// lambda/mu imaging + adjoint stress}. Each cell reads its own adjoint field
+// before updating it, as the reference's separate launches do, and no kernel
+// reads a neighbour value that it writes.
 //
 // FORM = 0: reference formulation (typo in dsxy/dx, Src/velocity.cu:79;
 //           off-by-one x/y C-PML predicate, Src/velocity.cu:46-90; swapped
@@ -44,6 +56,13 @@ __device__ __forceinline__ float Dsxy_x(const float* __restrict__ f, long i, lon
   if (FORM == 1) return Db(f, i, s, h);
   return (C1 * (f[i] - f[i - s]) - C2 * (f[i - s] - f[i - 2 * s])) / h;
 }
+// a * d(psi), skipped where a == 0 (outside the PML along that axis).
+__device__ __forceinline__ float apsi_f(float a, const float* __restrict__ f, long i, long s, float h) {
+  return a != 0.0f ? a * Af(f, i, s, h) : 0.0f;
+}
+__device__ __forceinline__ float apsi_b(float a, const float* __restrict__ f, long i, long s, float h) {
+  return a != 0.0f ? a * Ab(f, i, s, h) : 0.0f;
+}
 
 __device__ __forceinline__ bool in_pml(int i, int n, int p) { return i < p || i > n - p - 1; }
 // x/y predicate of the forward velocity kernel.
@@ -65,25 +84,35 @@ struct Cell {
   long i;
 };
 
-// Map the launch (64 x 4 workgroup, grid (ny/64, nx/4, nz)) to a cell inside
-// [lo, n-1-lo] on every axis.
-__device__ __forceinline__ bool cell_of(const Geom& G, int lo_z, int hi_z, int lo_x, int hi_x,
-                                        int lo_y, int hi_y, Cell& c) {
-  c.y = blockIdx.x * 64 + threadIdx.x;
-  c.x = blockIdx.y * 4 + threadIdx.y;
-  c.z = blockIdx.z;
-  if (c.z < lo_z || c.z > hi_z || c.x < lo_x || c.x > hi_x || c.y < lo_y || c.y > hi_y) return false;
+__device__ __forceinline__ bool sweep_cell(const Sweep& S, const Geom& G, Cell& c) {
+  const int b = blockIdx.x;
+  const int xcd = b & 7, j = b >> 3;
+  const int per_plane = S.rows_max * S.nyb;
+  const int zz = j / per_plane;
+  const int r = j - zz * per_plane;
+  const int row = r / S.nyb;
+  const int yb = r - row * S.nyb;
+  const int rows = S.rows_base + (xcd < S.rows_rem ? 1 : 0);
+  if (row >= rows) return false;
+  c.x = S.x0 + xcd * S.rows_base + min(xcd, S.rows_rem) + row;
+  c.z = S.z0 + zz;
+  c.y = yb * 64 + (int)threadIdx.x;
+  if (c.y < S.y0 || c.y > S.y1) return false;
   c.i = (long)c.z * G.pxy + (long)c.x * G.py + c.y;
   return true;
 }
 
+__device__ __forceinline__ bool interior(const Geom& G, const Cell& c) {
+  const int p = G.npml;
+  return c.z >= p && c.z <= G.nz - 1 - p && c.x >= p && c.x <= G.nx - 1 - p && c.y >= p &&
+         c.y <= G.ny - 1 - p;
+}
+
 // ---------------------------------------------------------------------------
-// Forward stress, Src/stress.cu:36-101.
+// Forward stress of one cell, Src/stress.cu:36-101.
 // ---------------------------------------------------------------------------
-template <int FORM>
-__global__ __launch_bounds__(256) void k_stress_fwd(Geom G, Wave W, Psi P, Model M, Cpml C) {
-  Cell c;
-  if (!cell_of(G, 2, G.nz - 3, 2, G.nx - 3, 2, G.ny - 3, c)) return;
+__device__ __forceinline__ void stress_fwd_cell(const Geom& G, const Wave& W, const Psi& P, const Model& M,
+                                                const Cpml& C, const Cell& c) {
   const long i = c.i, SZ = G.pxy, SX = G.py;
   const int z = c.z, x = c.x, y = c.y, p = G.npml;
   float dvz_dz = Db(W.vz, i, SZ, G.dz);
@@ -122,12 +151,11 @@ __global__ __launch_bounds__(256) void k_stress_fwd(Geom G, Wave W, Psi P, Model
 }
 
 // ---------------------------------------------------------------------------
-// Forward velocity, Src/velocity.cu:175-240.
+// Forward velocity of one cell, Src/velocity.cu:175-240.
 // ---------------------------------------------------------------------------
 template <int FORM>
-__global__ __launch_bounds__(256) void k_velocity_fwd(Geom G, Wave W, Psi P, Model M, Cpml C) {
-  Cell c;
-  if (!cell_of(G, 2, G.nz - 3, 2, G.nx - 3, 2, G.ny - 3, c)) return;
+__device__ __forceinline__ void velocity_fwd_cell(const Geom& G, const Wave& W, const Psi& P, const Model& M,
+                                                  const Cpml& C, const Cell& c) {
   const long i = c.i, SZ = G.pxy, SX = G.py;
   const int z = c.z, x = c.x, y = c.y, p = G.npml;
   const float byc = M.byc[i], dt = G.dt;
@@ -160,15 +188,13 @@ __global__ __launch_bounds__(256) void k_velocity_fwd(Geom G, Wave W, Psi P, Mod
 }
 
 // ---------------------------------------------------------------------------
-// Reverse stress + lambda/mu imaging, Src/stress.cu:103-139. The source
-// subtraction of the same step (Src/libCUFD.cu:547-549) is folded in: it
-// precedes the update at the source cell, as in the reference's launch order.
+// Reverse stress + lambda/mu imaging of one interior cell, Src/stress.cu:103-139,
+// preceded at the source cell by the source subtraction of the same step
+// (Src/libCUFD.cu:547-549). Reads the adjoint stress of the cell as it was
+// before this step's adjoint update.
 // ---------------------------------------------------------------------------
-__global__ __launch_bounds__(256) void k_stress_rev(Geom G, Wave W, Wave A, Model M, Grad Gr,
-                                                    Source S, int it) {
-  Cell c;
-  const int p = G.npml;
-  if (!cell_of(G, p, G.nz - 1 - p, p, G.nx - 1 - p, p, G.ny - 1 - p, c)) return;
+__device__ __forceinline__ void stress_rev_cell(const Geom& G, const Wave& W, const Wave& A, const Model& M,
+                                                const Grad& Gr, const Source& S, int it, const Cell& c) {
   const long i = c.i, SZ = G.pxy, SX = G.py;
   const float lam = M.lam[i], mu = M.mu[i], dt = G.dt;
   float szz = W.szz[i], sxx = W.sxx[i], syy = W.syy[i];
@@ -200,13 +226,11 @@ __global__ __launch_bounds__(256) void k_stress_rev(Geom G, Wave W, Wave A, Mode
 }
 
 // ---------------------------------------------------------------------------
-// Reverse velocity + density imaging, Src/velocity.cu:242-275.
+// Reverse velocity + density imaging of one interior cell, Src/velocity.cu:242-275.
 // ---------------------------------------------------------------------------
 template <int FORM>
-__global__ __launch_bounds__(256) void k_velocity_rev(Geom G, Wave W, Wave A, Model M, Grad Gr) {
-  Cell c;
-  const int p = G.npml;
-  if (!cell_of(G, p, G.nz - 1 - p, p, G.nx - 1 - p, p, G.ny - 1 - p, c)) return;
+__device__ __forceinline__ void velocity_rev_cell(const Geom& G, const Wave& W, const Wave& A, const Model& M,
+                                                  const Grad& Gr, const Cell& c) {
   const long i = c.i, SZ = G.pxy, SX = G.py;
   const float byc = M.byc[i], dt = G.dt;
   const float dszz_dz = Df(W.szz, i, SZ, G.dz);
@@ -229,34 +253,35 @@ __global__ __launch_bounds__(256) void k_velocity_rev(Geom G, Wave W, Wave A, Mo
 }
 
 // ---------------------------------------------------------------------------
-// Adjoint velocity, Src/velocity_adj.cu:149-296. Reads the adjoint stress and
-// the psi arrays written by the adjoint stress kernel; writes v-dagger and the
-// psi arrays the adjoint stress kernel reads.
+// Adjoint velocity of one cell, Src/velocity_adj.cu:149-296: reads the adjoint
+// stress and the psi arrays written by the adjoint stress kernel; writes
+// v-dagger and the psi arrays the adjoint stress kernel reads.
 // ---------------------------------------------------------------------------
 template <int FORM>
-__global__ __launch_bounds__(256) void k_velocity_adj(Geom G, Wave A, Psi P, Model M, Cpml C) {
-  Cell c;
-  if (!cell_of(G, 2, G.nz - 3, 2, G.nx - 3, 2, G.ny - 3, c)) return;
+__device__ __forceinline__ void velocity_adj_cell(const Geom& G, const Wave& A, const Psi& P, const Model& M,
+                                                  const Cpml& C, const Cell& c) {
   const long i = c.i, SZ = G.pxy, SX = G.py;
   const int z = c.z, x = c.x, y = c.y, p = G.npml;
   const float lambda = M.lam[i], mu = M.mu[i], byc = M.byc[i], dt = G.dt;
   const float l2m = lambda + 2.0f * mu;
   const bool pz = in_pml(z, G.nz, p), px = in_pml(x, G.nx, p), py = in_pml(y, G.ny, p);
+  const float ax = C.a[1][x], az = C.a[0][z], ay = C.a[2][y];
+  const float axh = C.ah[1][x], azh = C.ah[0][z], ayh = C.ah[2][y];
+  const float Kx = C.K[1][x], Kz = C.K[0][z], Ky = C.K[2][y];
+  const float Kxh = C.Kh[1][x], Kzh = C.Kh[0][z], Kyh = C.Kh[2][y];
   // vx
   {
-    const float dpsixx_dx = Af(P.vx_x, i, SX, G.dx);
+    const float t1 = apsi_f(ax, P.vx_x, i, SX, G.dx);
     const float dszz_dx = Af(A.szz, i, SX, G.dx);
     const float dsxx_dx = Af(A.sxx, i, SX, G.dx);
     const float dsyy_dx = Af(A.syy, i, SX, G.dx);
-    const float dpsixz_dz = Ab(P.vx_z, i, SZ, G.dz);
+    const float t6 = apsi_b(azh, P.vx_z, i, SZ, G.dz);
     const float dsxz_dz = Ab(A.sxz, i, SZ, G.dz);
-    const float dpsixy_dy = Ab(P.vx_y, i, 1, G.dy);
+    const float t8 = apsi_b(ayh, P.vx_y, i, 1, G.dy);
     const float dsxy_dy = Ab(A.sxy, i, 1, G.dy);
-    const float Kx = C.K[1][x];
-    const float v = A.vx[i] + (C.a[1][x] * dpsixx_dx + l2m * dsxx_dx / Kx * dt +
-                               lambda * dszz_dx / Kx * dt + lambda * dsyy_dx / Kx * dt +
-                               mu / C.Kh[0][z] * dsxz_dz * dt + C.ah[0][z] * dpsixz_dz +
-                               mu / C.Kh[2][y] * dsxy_dy * dt + C.ah[2][y] * dpsixy_dy);
+    const float v = A.vx[i] + (t1 + l2m * dsxx_dx / Kx * dt + lambda * dszz_dx / Kx * dt +
+                               lambda * dsyy_dx / Kx * dt + mu / Kzh * dsxz_dz * dt + t6 +
+                               mu / Kyh * dsxy_dy * dt + t8);
     A.vx[i] = v;
     if (px) P.sxx_x[i] = C.bh[1][x] * P.sxx_x[i] + byc * v * dt;
     if (pz) P.sxz_z[i] = C.b[0][z] * P.sxz_z[i] + byc * v * dt;
@@ -264,19 +289,17 @@ __global__ __launch_bounds__(256) void k_velocity_adj(Geom G, Wave A, Psi P, Mod
   }
   // vz
   {
-    const float dpsizz_dz = Af(P.vz_z, i, SZ, G.dz);
+    const float t1 = apsi_f(az, P.vz_z, i, SZ, G.dz);
     const float dszz_dz = Af(A.szz, i, SZ, G.dz);
     const float dsxx_dz = Af(A.sxx, i, SZ, G.dz);
     const float dsyy_dz = Af(A.syy, i, SZ, G.dz);
-    const float dpsizx_dx = Ab(P.vz_x, i, SX, G.dx);
+    const float t8 = apsi_b(axh, P.vz_x, i, SX, G.dx);
     const float dsxz_dx = Ab(A.sxz, i, SX, G.dx);
-    const float dpsizy_dy = Ab(P.vz_y, i, 1, G.dy);
+    const float t6 = apsi_b(ayh, P.vz_y, i, 1, G.dy);
     const float dsyz_dy = Ab(A.syz, i, 1, G.dy);
-    const float Kz = C.K[0][z];
-    const float v = A.vz[i] + (C.a[0][z] * dpsizz_dz + l2m * dszz_dz / Kz * dt +
-                               lambda * dsxx_dz / Kz * dt + lambda * dsyy_dz / Kz * dt +
-                               mu / C.Kh[2][y] * dsyz_dy * dt + C.ah[2][y] * dpsizy_dy +
-                               mu / C.Kh[1][x] * dsxz_dx * dt + C.ah[1][x] * dpsizx_dx);
+    const float v = A.vz[i] + (t1 + l2m * dszz_dz / Kz * dt + lambda * dsxx_dz / Kz * dt +
+                               lambda * dsyy_dz / Kz * dt + mu / Kyh * dsyz_dy * dt + t6 +
+                               mu / Kxh * dsxz_dx * dt + t8);
     A.vz[i] = v;
     if (px) P.sxz_x[i] = C.b[1][x] * P.sxz_x[i] + byc * v * dt;
     if (pz) P.szz_z[i] = C.bh[0][z] * P.szz_z[i] + byc * v * dt;
@@ -284,19 +307,17 @@ __global__ __launch_bounds__(256) void k_velocity_adj(Geom G, Wave A, Psi P, Mod
   }
   // vy
   {
-    const float dpsiyy_dy = Af(P.vy_y, i, 1, G.dy);
+    const float t1 = apsi_f(ay, P.vy_y, i, 1, G.dy);
     const float dsyy_dy = Af(A.syy, i, 1, G.dy);
     const float dsxx_dy = Af(A.sxx, i, 1, G.dy);
     const float dszz_dy = Af(A.szz, i, 1, G.dy);
-    const float dpsiyx_dx = Ab(P.vy_x, i, SX, G.dx);
+    const float t6 = apsi_b(axh, P.vy_x, i, SX, G.dx);
     const float dsxy_dx = Ab(A.sxy, i, SX, G.dx);
-    const float dpsiyz_dz = Ab(P.vy_z, i, SZ, G.dz);
+    const float t8 = apsi_b(azh, P.vy_z, i, SZ, G.dz);
     const float dsyz_dz = Ab(A.syz, i, SZ, G.dz);
-    const float Ky = C.K[2][y];
-    const float v = A.vy[i] + (C.a[2][y] * dpsiyy_dy + l2m * dsyy_dy / Ky * dt +
-                               lambda * dsxx_dy / Ky * dt + lambda * dszz_dy / Ky * dt +
-                               mu / C.Kh[1][x] * dsxy_dx * dt + C.ah[1][x] * dpsiyx_dx +
-                               mu / C.Kh[0][z] * dsyz_dz * dt + C.ah[0][z] * dpsiyz_dz);
+    const float v = A.vy[i] + (t1 + l2m * dsyy_dy / Ky * dt + lambda * dsxx_dy / Ky * dt +
+                               lambda * dszz_dy / Ky * dt + mu / Kxh * dsxy_dx * dt + t6 +
+                               mu / Kzh * dsyz_dz * dt + t8);
     A.vy[i] = v;
     if (px) P.sxy_x[i] = C.b[1][x] * P.sxy_x[i] + byc * v * dt;
     if (pz) P.syz_z[i] = (FORM == 1 ? C.b[0][z] : C.bh[0][z]) * P.syz_z[i] + byc * v * dt;
@@ -305,62 +326,60 @@ __global__ __launch_bounds__(256) void k_velocity_adj(Geom G, Wave A, Psi P, Mod
 }
 
 // ---------------------------------------------------------------------------
-// Adjoint stress, Src/stress_adj.cu:1-146.
+// Adjoint stress of one cell, Src/stress_adj.cu:1-146.
 // ---------------------------------------------------------------------------
-__global__ __launch_bounds__(256) void k_stress_adj(Geom G, Wave A, Psi P, Model M, Cpml C) {
-  Cell c;
-  if (!cell_of(G, 2, G.nz - 3, 2, G.nx - 3, 2, G.ny - 3, c)) return;
+__device__ __forceinline__ void stress_adj_cell(const Geom& G, const Wave& A, const Psi& P, const Model& M,
+                                                const Cpml& C, const Cell& c) {
   const long i = c.i, SZ = G.pxy, SX = G.py;
   const int z = c.z, x = c.x, y = c.y, p = G.npml;
   const float lambda = M.lam[i], mu = M.mu[i], byc = M.byc[i], dt = G.dt;
   const bool pz = in_pml(z, G.nz, p), px = in_pml(x, G.nx, p), py = in_pml(y, G.ny, p);
+  const float ax = C.a[1][x], az = C.a[0][z], ay = C.a[2][y];
+  const float Kx = C.K[1][x], Kz = C.K[0][z], Ky = C.K[2][y];
   // sxz
   {
-    const float dphi_x = Af(P.sxz_x, i, SX, G.dx);
+    const float t1 = apsi_f(ax, P.sxz_x, i, SX, G.dx);
     const float dvz_dx = Af(A.vz, i, SX, G.dx);
-    const float dphi_z = Af(P.sxz_z, i, SZ, G.dz);
+    const float t3 = apsi_f(az, P.sxz_z, i, SZ, G.dz);
     const float dvx_dz = Af(A.vx, i, SZ, G.dz);
-    const float s = A.sxz[i] + (C.a[1][x] * dphi_x + dvz_dx / C.K[1][x] * byc * dt +
-                                C.a[0][z] * dphi_z + dvx_dz / C.K[0][z] * byc * dt);
+    const float s = A.sxz[i] + (t1 + dvz_dx / Kx * byc * dt + t3 + dvx_dz / Kz * byc * dt);
     A.sxz[i] = s;
     if (px) P.vz_x[i] = C.bh[1][x] * P.vz_x[i] + s * mu * dt;
     if (pz) P.vx_z[i] = C.bh[0][z] * P.vx_z[i] + s * mu * dt;
   }
   // sxy
   {
-    const float dphi_x = Af(P.sxy_x, i, SX, G.dx);
+    const float t1 = apsi_f(ax, P.sxy_x, i, SX, G.dx);
     const float dvy_dx = Af(A.vy, i, SX, G.dx);
-    const float dphi_y = Af(P.sxy_y, i, 1, G.dy);
+    const float t3 = apsi_f(ay, P.sxy_y, i, 1, G.dy);
     const float dvx_dy = Af(A.vx, i, 1, G.dy);
-    const float s = A.sxy[i] + (C.a[1][x] * dphi_x + dvy_dx / C.K[1][x] * byc * dt +
-                                C.a[2][y] * dphi_y + dvx_dy / C.K[2][y] * byc * dt);
+    const float s = A.sxy[i] + (t1 + dvy_dx / Kx * byc * dt + t3 + dvx_dy / Ky * byc * dt);
     A.sxy[i] = s;
     if (py) P.vx_y[i] = C.bh[2][y] * P.vx_y[i] + s * mu * dt;
     if (px) P.vy_x[i] = C.bh[1][x] * P.vy_x[i] + s * mu * dt;
   }
   // syz
   {
-    const float dphi_y = Af(P.syz_y, i, 1, G.dy);
+    const float t1 = apsi_f(ay, P.syz_y, i, 1, G.dy);
     const float dvz_dy = Af(A.vz, i, 1, G.dy);
-    const float dphi_z = Af(P.syz_z, i, SZ, G.dz);
+    const float t3 = apsi_f(az, P.syz_z, i, SZ, G.dz);
     const float dvy_dz = Af(A.vy, i, SZ, G.dz);
-    const float s = A.syz[i] + (C.a[2][y] * dphi_y + dvz_dy / C.K[2][y] * byc * dt +
-                                C.a[0][z] * dphi_z + dvy_dz / C.K[0][z] * byc * dt);
+    const float s = A.syz[i] + (t1 + dvz_dy / Ky * byc * dt + t3 + dvy_dz / Kz * byc * dt);
     A.syz[i] = s;
     if (py) P.vz_y[i] = C.bh[2][y] * P.vz_y[i] + s * mu * dt;
     if (pz) P.vy_z[i] = C.bh[0][z] * P.vy_z[i] + s * mu * dt;
   }
   // normal stresses
   {
-    const float dphi_xx = Ab(P.sxx_x, i, SX, G.dx);
+    const float txx = apsi_b(C.ah[1][x], P.sxx_x, i, SX, G.dx);
     const float dvx_dx = Ab(A.vx, i, SX, G.dx);
-    const float dphi_zz = Ab(P.szz_z, i, SZ, G.dz);
+    const float tzz = apsi_b(C.ah[0][z], P.szz_z, i, SZ, G.dz);
     const float dvz_dz = Ab(A.vz, i, SZ, G.dz);
-    const float dphi_yy = Ab(P.syy_y, i, 1, G.dy);
+    const float tyy = apsi_b(C.ah[2][y], P.syy_y, i, 1, G.dy);
     const float dvy_dy = Ab(A.vy, i, 1, G.dy);
-    const float sxx = A.sxx[i] + (C.ah[1][x] * dphi_xx + byc * dvx_dx / C.Kh[1][x] * dt);
-    const float szz = A.szz[i] + (C.ah[0][z] * dphi_zz + byc * dvz_dz / C.Kh[0][z] * dt);
-    const float syy = A.syy[i] + (C.ah[2][y] * dphi_yy + byc * dvy_dy / C.Kh[2][y] * dt);
+    const float sxx = A.sxx[i] + (txx + byc * dvx_dx / C.Kh[1][x] * dt);
+    const float szz = A.szz[i] + (tzz + byc * dvz_dz / C.Kh[0][z] * dt);
+    const float syy = A.syy[i] + (tyy + byc * dvy_dy / C.Kh[2][y] * dt);
     A.sxx[i] = sxx;
     A.szz[i] = szz;
     A.syy[i] = syy;
@@ -371,51 +390,123 @@ __global__ __launch_bounds__(256) void k_stress_adj(Geom G, Wave A, Psi P, Model
   }
 }
 
+// ---------------------------------------------------------------------------
+// Kernels.
+// ---------------------------------------------------------------------------
+// Forward stress, then the source of this step at its cell (Src/libCUFD.cu:319-330).
+__global__ __launch_bounds__(64) void k_stress_fwd(Sweep Sw, Geom G, Wave W, Psi P, Model M, Cpml C, Source S,
+                                                   int it) {
+  Cell c;
+  if (!sweep_cell(Sw, G, c)) return;
+  stress_fwd_cell(G, W, P, M, C, c);
+  if (c.i == S.cell) {
+    const float a = 2250000.0f * S.stf[it] * G.dt * 1.0f;
+    W.szz[c.i] += a; W.sxx[c.i] += a; W.syy[c.i] += a;
+  }
+}
+
+template <int FORM>
+__global__ __launch_bounds__(64) void k_velocity_fwd(Sweep Sw, Geom G, Wave W, Psi P, Model M, Cpml C) {
+  Cell c;
+  if (!sweep_cell(Sw, G, c)) return;
+  velocity_fwd_cell<FORM>(G, W, P, M, C, c);
+}
+
+// Backward velocity step: reverse velocity + density imaging (interior), then
+// the adjoint velocity ([2, n-3]).
+template <int FORM>
+__global__ __launch_bounds__(64) void k_velocity_bwd(Sweep Sw, Geom G, Wave W, Wave A, Psi P, Model M, Cpml C,
+                                                     Grad Gr) {
+  Cell c;
+  if (!sweep_cell(Sw, G, c)) return;
+  if (interior(G, c)) velocity_rev_cell<FORM>(G, W, A, M, Gr, c);
+  velocity_adj_cell<FORM>(G, A, P, M, C, c);
+}
+
+// Backward stress step: (source subtraction +) reverse stress + lambda/mu
+// imaging (interior), then the adjoint stress ([2, n-3]).
+__global__ __launch_bounds__(64) void k_stress_bwd(Sweep Sw, Geom G, Wave W, Wave A, Psi P, Model M, Cpml C,
+                                                   Grad Gr, Source S, int it) {
+  Cell c;
+  if (!sweep_cell(Sw, G, c)) return;
+  if (interior(G, c)) stress_rev_cell(G, W, A, M, Gr, S, it, c);
+  stress_adj_cell(G, A, P, M, C, c);
+}
+
+template <int FORM>
+__global__ __launch_bounds__(64) void k_velocity_adj(Sweep Sw, Geom G, Wave A, Psi P, Model M, Cpml C) {
+  Cell c;
+  if (!sweep_cell(Sw, G, c)) return;
+  velocity_adj_cell<FORM>(G, A, P, M, C, c);
+}
+
+__global__ __launch_bounds__(64) void k_stress_adj(Sweep Sw, Geom G, Wave A, Psi P, Model M, Cpml C) {
+  Cell c;
+  if (!sweep_cell(Sw, G, c)) return;
+  stress_adj_cell(G, A, P, M, C, c);
+}
+
 }  // namespace
 
 // ---------------------------------------------------------------------------
 // Launchers.
 // ---------------------------------------------------------------------------
-static inline dim3 stencil_grid(const Geom& G) {
-  return dim3((G.ny + 63) / 64, (G.nx + 3) / 4, G.nz);
+Sweep make_sweep(int z0, int z1, int x0, int x1, int y0, int y1) {
+  Sweep S;
+  S.z0 = z0; S.nz = z1 - z0 + 1;
+  S.x0 = x0;
+  const int nxr = x1 - x0 + 1;
+  S.rows_base = nxr / 8;
+  S.rows_rem = nxr % 8;
+  S.rows_max = S.rows_base + (S.rows_rem ? 1 : 0);
+  S.y0 = y0; S.y1 = y1;
+  S.nyb = (y1 + 64) / 64;  // segments of 64 aligned at y = 0
+  S.blocks = 8L * S.rows_max * S.nyb * S.nz;
+  return S;
 }
-static const dim3 kStencilBlock(64, 4, 1);
 
-void launch_stress_fwd(int form, const Geom& G, const Wave& W, const Psi& P, const Model& M,
-                       const Cpml& C, hipStream_t s) {
-  if (form == 1)
-    hipLaunchKernelGGL(k_stress_fwd<1>, stencil_grid(G), kStencilBlock, 0, s, G, W, P, M, C);
-  else
-    hipLaunchKernelGGL(k_stress_fwd<0>, stencil_grid(G), kStencilBlock, 0, s, G, W, P, M, C);
+Sweep sweep_full(const Geom& G) { return make_sweep(2, G.nz - 3, 2, G.nx - 3, 2, G.ny - 3); }
+
+static inline dim3 grid_of(const Sweep& S) { return dim3((unsigned)S.blocks); }
+
+void launch_stress_fwd(const Geom& G, const Wave& W, const Psi& P, const Model& M, const Cpml& C,
+                       const Source& S, int it, hipStream_t s) {
+  const Sweep Sw = sweep_full(G);
+  hipLaunchKernelGGL(k_stress_fwd, grid_of(Sw), dim3(64), 0, s, Sw, G, W, P, M, C, S, it);
 }
 void launch_velocity_fwd(int form, const Geom& G, const Wave& W, const Psi& P, const Model& M,
                          const Cpml& C, hipStream_t s) {
+  const Sweep Sw = sweep_full(G);
   if (form == 1)
-    hipLaunchKernelGGL(k_velocity_fwd<1>, stencil_grid(G), kStencilBlock, 0, s, G, W, P, M, C);
+    hipLaunchKernelGGL(k_velocity_fwd<1>, grid_of(Sw), dim3(64), 0, s, Sw, G, W, P, M, C);
   else
-    hipLaunchKernelGGL(k_velocity_fwd<0>, stencil_grid(G), kStencilBlock, 0, s, G, W, P, M, C);
+    hipLaunchKernelGGL(k_velocity_fwd<0>, grid_of(Sw), dim3(64), 0, s, Sw, G, W, P, M, C);
 }
-void launch_stress_rev(const Geom& G, const Wave& W, const Wave& A, const Model& M, const Grad& Gr,
-                       const Source& S, int it, hipStream_t s) {
-  hipLaunchKernelGGL(k_stress_rev, stencil_grid(G), kStencilBlock, 0, s, G, W, A, M, Gr, S, it);
-}
-void launch_velocity_rev(int form, const Geom& G, const Wave& W, const Wave& A, const Model& M,
-                         const Grad& Gr, hipStream_t s) {
+void launch_velocity_bwd(int form, const Geom& G, const Wave& W, const Wave& A, const Psi& P, const Model& M,
+                         const Cpml& C, const Grad& Gr, hipStream_t s) {
+  const Sweep Sw = sweep_full(G);
   if (form == 1)
-    hipLaunchKernelGGL(k_velocity_rev<1>, stencil_grid(G), kStencilBlock, 0, s, G, W, A, M, Gr);
+    hipLaunchKernelGGL(k_velocity_bwd<1>, grid_of(Sw), dim3(64), 0, s, Sw, G, W, A, P, M, C, Gr);
   else
-    hipLaunchKernelGGL(k_velocity_rev<0>, stencil_grid(G), kStencilBlock, 0, s, G, W, A, M, Gr);
+    hipLaunchKernelGGL(k_velocity_bwd<0>, grid_of(Sw), dim3(64), 0, s, Sw, G, W, A, P, M, C, Gr);
+}
+void launch_stress_bwd(const Geom& G, const Wave& W, const Wave& A, const Psi& P, const Model& M,
+                       const Cpml& C, const Grad& Gr, const Source& S, int it, hipStream_t s) {
+  const Sweep Sw = sweep_full(G);
+  hipLaunchKernelGGL(k_stress_bwd, grid_of(Sw), dim3(64), 0, s, Sw, G, W, A, P, M, C, Gr, S, it);
 }
 void launch_velocity_adj(int form, const Geom& G, const Wave& A, const Psi& P, const Model& M,
                          const Cpml& C, hipStream_t s) {
+  const Sweep Sw = sweep_full(G);
   if (form == 1)
-    hipLaunchKernelGGL(k_velocity_adj<1>, stencil_grid(G), kStencilBlock, 0, s, G, A, P, M, C);
+    hipLaunchKernelGGL(k_velocity_adj<1>, grid_of(Sw), dim3(64), 0, s, Sw, G, A, P, M, C);
   else
-    hipLaunchKernelGGL(k_velocity_adj<0>, stencil_grid(G), kStencilBlock, 0, s, G, A, P, M, C);
+    hipLaunchKernelGGL(k_velocity_adj<0>, grid_of(Sw), dim3(64), 0, s, Sw, G, A, P, M, C);
 }
 void launch_stress_adj(const Geom& G, const Wave& A, const Psi& P, const Model& M, const Cpml& C,
                        hipStream_t s) {
-  hipLaunchKernelGGL(k_stress_adj, stencil_grid(G), kStencilBlock, 0, s, G, A, P, M, C);
+  const Sweep Sw = sweep_full(G);
+  hipLaunchKernelGGL(k_stress_adj, grid_of(Sw), dim3(64), 0, s, Sw, G, A, P, M, C);
 }
 
 }  // namespace efwi

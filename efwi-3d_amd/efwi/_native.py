@@ -18,7 +18,7 @@ _HERE = os.path.dirname(os.path.abspath(__file__))
 LIB_PATH = os.path.join(_HERE, "libefwi.so")
 
 OK = 0
-KERNELS = ("stress_fwd", "velocity_fwd", "stress_rev", "velocity_rev", "velocity_adj", "stress_adj",
+KERNELS = ("stress_fwd", "velocity_fwd", "stress_bwd", "velocity_bwd", "velocity_adj", "stress_adj",
            "ring", "other")
 CALC_MISFIT, CALC_GRADIENT, CALC_SYNTHETIC = 0, 1, 2
 

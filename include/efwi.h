@@ -101,8 +101,9 @@ int efwi_get_gradient(efwi_ctx* ctx, float* g_lambda, float* g_mu, float* g_rho,
  * every stencil and ring launch of efwi_run_shot (no reference counterpart;
  * the reference only prints wall-clock totals, Src/libCUFD.cu:210-216).
  * efwi_profile_read fills EFWI_NKERNEL totals in milliseconds and launch
- * counts, indexed stress_fwd, velocity_fwd, stress_rev, velocity_rev,
- * velocity_adj, stress_adj, ring, other. */
+ * counts, indexed stress_fwd, velocity_fwd, stress_bwd (reverse stress +
+ * adjoint stress), velocity_bwd (reverse velocity + adjoint velocity),
+ * velocity_adj, stress_adj (first backward step only), ring, other. */
 #define EFWI_NKERNEL 8
 int efwi_profile(efwi_ctx* ctx, int enable);
 int efwi_profile_read(efwi_ctx* ctx, double* ms, long long* launches, int reset);
