@@ -328,6 +328,7 @@ extern "C" int efwi_cufd(float* misfit, float* grad_lambda, float* grad_mu, floa
   cfg.formulation = 0;
   cfg.if_win = P.if_win ? 1 : 0;
   cfg.device = gpu_id;
+  cfg.memory_mode = -1;  // the full ring when it fits, checkpointed segments otherwise
   std::vector<ShotDesc> shots(group_size);
   for (int k = 0; k < group_size; k++)
     if ((rc = read_shot(survey, shot_ids[k], P.npml, P.if_win, shots[k]))) return rc;

@@ -167,6 +167,7 @@ Ctx::~Ctx() {
   prof.release();
   for (void* p : allocs) hipFree(p);
   if (ring) hipFree(ring);
+  if (ckpt) hipFree(ckpt);
   hipSetDevice(prev);
 }
 
@@ -261,6 +262,7 @@ int Ctx::init() {
   B.nxs = 2 * 5 * (nzB - 10) * nyB;
   B.nys = 2 * 5 * (nzB - 10) * (nxB - 10);
   B.n = B.nzs + B.nxs + B.nys;
+  if ((rc = plan_memory())) return rc;
   if ((rc = alloc((void**)&msum, sizeof(double) * 8))) return rc;
   if ((rc = alloc((void**)&partials, sizeof(double) * kMaxPartials))) return rc;
   if ((rc = alloc((void**)&stf_dev, sizeof(float) * c.nt))) return rc;
@@ -289,8 +291,37 @@ static void zero_psi(const Psi& P, long n, hipStream_t s) {
   for (int k = 0; k < 18; k++) launch_fill(p[k], n, 0.0f, s);
 }
 
+// Memory plan of the reverse reconstruction.
+//   mode 0: the band of every step stays in HBM ([nt-1][9][band]).
+//   mode 1: the forward state (v, s and the PML slabs of the 18 psi arrays) is
+//           kept every K steps; the backward pass replays each segment of K
+//           steps from its checkpoint into separate buffers to rebuild that
+//           segment's band ring, then reverses through it. The replayed band
+//           values are the values the forward pass produced (the kernels are
+//           deterministic), so the gradient is bit-identical to mode 0.
+int Ctx::plan_memory() {
+  const int steps = cfg.nt - 1;
+  if (cfg.memory_mode < 0) {
+    // automatic: the full ring when it fits in 90 % of the free device memory
+    size_t free_b = 0, total_b = 0;
+    HIP_TRY(hipMemGetInfo(&free_b, &total_b));
+    const double ring_b = (double)B.n * 9 * steps * sizeof(float);
+    cfg.memory_mode = ring_b < 0.9 * (double)free_b ? 0 : 1;
+  }
+  if (cfg.memory_mode == 0) {
+    seg_len = steps;
+  } else if (cfg.memory_mode == 1) {
+    seg_len = cfg.checkpoint_every > 0 ? cfg.checkpoint_every : (int)std::ceil(std::sqrt((double)steps));
+    if (seg_len > steps) seg_len = steps;
+  } else {
+    return fail(EFWI_ERR_ARG, "memory_mode must be 0 (ring) or 1 (checkpointed ring)");
+  }
+  nseg = (steps + seg_len - 1) / seg_len;
+  return EFWI_OK;
+}
+
 int Ctx::ensure_ring() {
-  const size_t need = (size_t)B.n * 9 * (size_t)(cfg.nt - 1) * sizeof(float);
+  const size_t need = (size_t)B.n * 9 * (size_t)seg_len * sizeof(float);
   if (ring && ring_bytes >= need) return EFWI_OK;
   if (ring) {
     hipFree(ring);
@@ -302,10 +333,98 @@ int Ctx::ensure_ring() {
     ring = nullptr;
     ring_bytes = 0;
     return fail(EFWI_ERR_NOMEM, "boundary ring of " + std::to_string(need) +
-                                    " B does not fit in device memory: " + hipGetErrorString(e));
+                                    " B does not fit in device memory: " + hipGetErrorString(e) +
+                                    " (memory_mode 1 keeps only sqrt(nt) steps of it)");
   }
   ring_bytes = need;
   bytes_held += (long long)need;
+  if (cfg.memory_mode == 1) return ensure_checkpoints();
+  return EFWI_OK;
+}
+
+// Bytes of one checkpoint: v and s in full, and of each psi array only the
+// slabs where its C-PML predicate can be true (z: planes [0,p) and
+// [nz-p,nz); x: rows [0,p) and [nx-p,nx) of every plane; y: the first and
+// last p cells of every row). Outside those slabs psi is never written.
+static const int kPsiAxis[18] = {0, 1, 2, 0, 1, 2, 0, 1, 2,   // vz_z vz_x vz_y vx_z vx_x vx_y vy_z vy_x vy_y
+                                 0, 1, 0, 1, 1, 2, 2, 2, 0};  // szz_z sxz_x sxz_z sxx_x sxy_x sxy_y syy_y syz_y syz_z
+
+size_t Ctx::slab_floats(int axis) const {
+  const long p = cfg.npml;
+  if (axis == 0) return (size_t)(2 * p) * G.pxy;
+  if (axis == 1) return (size_t)cfg.nz * 2 * p * G.py;
+  return (size_t)cfg.nz * cfg.nx * 2 * p;
+}
+
+int Ctx::ensure_checkpoints() {
+  size_t per = 9 * (size_t)N;
+  for (int k = 0; k < 18; k++) per += slab_floats(kPsiAxis[k]);
+  ckpt_floats = per;
+  const size_t need = per * (size_t)nseg * sizeof(float);
+  int rc;
+  if (!W2.vz) {
+    float** wf[9] = {&W2.vz, &W2.vx, &W2.vy, &W2.szz, &W2.sxx, &W2.sxz, &W2.syy, &W2.sxy, &W2.syz};
+    for (int k = 0; k < 9; k++)
+      if ((rc = alloc((void**)wf[k], (size_t)N * sizeof(float)))) return rc;
+    float** pf = reinterpret_cast<float**>(&P2);
+    for (int k = 0; k < 18; k++)
+      if ((rc = alloc((void**)&pf[k], (size_t)N * sizeof(float)))) return rc;
+    // psi outside its slabs is never written: zero it once
+    zero_psi(P2, N, nullptr);
+    HIP_TRY(hipDeviceSynchronize());
+  }
+  if (ckpt && ckpt_bytes >= need) return EFWI_OK;
+  if (ckpt) {
+    hipFree(ckpt);
+    bytes_held -= (long long)ckpt_bytes;
+    ckpt = nullptr;
+  }
+  hipError_t e = hipMalloc((void**)&ckpt, need);
+  if (e != hipSuccess) {
+    ckpt = nullptr;
+    ckpt_bytes = 0;
+    return fail(EFWI_ERR_NOMEM, "checkpoints of " + std::to_string(need) + " B do not fit: " +
+                                    hipGetErrorString(e) + " (raise checkpoint_every)");
+  }
+  ckpt_bytes = need;
+  bytes_held += (long long)need;
+  return EFWI_OK;
+}
+
+// Copy the forward state (W, P) to checkpoint k (save = true) or checkpoint k
+// to (W, P) (save = false).
+int Ctx::checkpoint(int k, const Wave& Wf, const Psi& Pf, bool save, hipStream_t s) {
+  float* base = ckpt + (size_t)k * ckpt_floats;
+  const size_t fb = (size_t)N * sizeof(float);
+  Fields9 F;
+  fields9(Wf, F);
+  for (int f = 0; f < 9; f++) {
+    if (save) HIP_TRY(hipMemcpyAsync(base, F.f[f], fb, hipMemcpyDeviceToDevice, s));
+    else HIP_TRY(hipMemcpyAsync(F.f[f], base, fb, hipMemcpyDeviceToDevice, s));
+    base += N;
+  }
+  float* const* pf = reinterpret_cast<float* const*>(&Pf);
+  const long p = cfg.npml;
+  for (int k2 = 0; k2 < 18; k2++) {
+    const int ax = kPsiAxis[k2];
+    float* a = pf[k2];
+    // two slabs per array: (offset, width in floats, rows, pitch in floats)
+    long off[2], width, rows, pitch;
+    if (ax == 0) {
+      off[0] = 0; off[1] = (long)(cfg.nz - p) * G.pxy; width = p * G.pxy; rows = 1; pitch = width;
+    } else if (ax == 1) {
+      off[0] = 0; off[1] = (long)(cfg.nx - p) * G.py; width = p * G.py; rows = cfg.nz; pitch = G.pxy;
+    } else {
+      off[0] = 0; off[1] = cfg.ny - p; width = p; rows = (long)cfg.nz * cfg.nx; pitch = G.py;
+    }
+    for (int h = 0; h < 2; h++) {
+      float* dev = a + off[h];
+      const size_t wb = width * sizeof(float), pb = pitch * sizeof(float);
+      if (save) HIP_TRY(hipMemcpy2DAsync(base, wb, dev, pb, wb, rows, hipMemcpyDeviceToDevice, s));
+      else HIP_TRY(hipMemcpy2DAsync(dev, pb, base, wb, wb, rows, hipMemcpyDeviceToDevice, s));
+      base += width * rows;
+    }
+  }
   return EFWI_OK;
 }
 
@@ -389,6 +508,19 @@ int Ctx::prepare_shot(const efwi_shot* sh, int calc_id) {
   return EFWI_OK;
 }
 
+// One forward time step on (Wf, Pf): stress + source, velocity, and (record)
+// the receiver gather of step it + 1.
+void Ctx::forward_step(const Wave& Wf, const Psi& Pf, const Source& Sf, int it, bool record, hipStream_t s) {
+  prof.begin(K_STRESS_FWD, s);
+  launch_stress_fwd(G, Wf, Pf, M, C, Sf, it, s);
+  prof.end(s);
+  if (!src_stencil) launch_add_source(Wf.szz, Wf.sxx, Wf.syy, src_cell, stf_dev, it, G.dt, +1, s);
+  prof.begin(K_VELOCITY_FWD, s);
+  launch_velocity_fwd(cfg.formulation, G, Wf, Pf, M, C, s);
+  prof.end(s);
+  if (record) launch_record(Wf, rec_dev, nrec, it + 1, syn_t, s);
+}
+
 int Ctx::run_shot(const efwi_shot* sh, int calc_id, const float* obs, float* syn, float* res,
                   double* misfit, hipStream_t s) {
   if (calc_id < 0 || calc_id > 2) return fail(EFWI_ERR_ARG, "invalid calc_id " + std::to_string(calc_id));
@@ -414,20 +546,18 @@ int Ctx::run_shot(const efwi_shot* sh, int calc_id, const float* obs, float* syn
   Source Sf;
   Sf.cell = src_stencil ? src_cell : -1;
   Sf.stf = stf_dev;
+  const int last_seg = nseg - 1;
   for (int it = 0; it <= nt - 2; it++) {
-    if (grad) {
+    const int seg = it / seg_len;
+    if (grad && cfg.memory_mode == 1 && it % seg_len == 0 && seg < last_seg)
+      if ((rc = checkpoint(seg, W, P, true, s))) return rc;
+    // the ring slots of the last segment are filled by this pass directly
+    if (grad && seg == last_seg) {
       prof.begin(K_RING, s);
-      launch_ring(B, G, F, ring, it, 0, 9, 0, s);
+      launch_ring(B, G, F, ring, it - seg * seg_len, 0, 9, 0, s);
       prof.end(s);
     }
-    prof.begin(K_STRESS_FWD, s);
-    launch_stress_fwd(G, W, P, M, C, Sf, it, s);
-    prof.end(s);
-    if (!src_stencil) launch_add_source(W.szz, W.sxx, W.syy, src_cell, stf_dev, it, G.dt, +1, s);
-    prof.begin(K_VELOCITY_FWD, s);
-    launch_velocity_fwd(form, G, W, P, M, C, s);
-    prof.end(s);
-    launch_record(W, rec_dev, nrec, it + 1, syn_t, s);
+    forward_step(W, P, Sf, it, true, s);
   }
   HIP_TRY(hipGetLastError());
   if (syn && nrec > 0) launch_from_time_major(syn_t, syn, nrec, nt, s);
@@ -461,17 +591,33 @@ int Ctx::run_shot(const efwi_shot* sh, int calc_id, const float* obs, float* syn
     launch_velocity_adj(form, G, A, P, M, C, s);
     launch_inject(A, rec_dev, nrec, nt - 1, res_t, s);
     launch_stress_adj(G, A, P, M, C, s);
-    for (int it = nt - 2; it >= 0; it--) {
-      prof.begin(K_VELOCITY_BWD, s);
-      launch_velocity_bwd(form, G, W, A, P, M, C, Gr, s);
-      prof.end(s);
-      launch_ring(B, G, F, ring, it, 0, 3, 1, s);
-      if (!src_interior) launch_add_source(W.szz, W.sxx, W.syy, src_cell, stf_dev, it, G.dt, -1, s);
-      launch_inject(A, rec_dev, nrec, it, res_t, s);
-      prof.begin(K_STRESS_BWD, s);
-      launch_stress_bwd(G, W, A, P, M, C, Gr, S, it, s);
-      prof.end(s);
-      launch_ring(B, G, F, ring, it, 3, 9, 1, s);
+    Fields9 F2;
+    fields9(W2, F2);
+    for (int seg = last_seg; seg >= 0; seg--) {
+      const int a = seg * seg_len, b = std::min(nt - 2, a + seg_len - 1);
+      if (seg < last_seg) {
+        // replay the segment from its checkpoint to rebuild its ring
+        if ((rc = checkpoint(seg, W2, P2, false, s))) return rc;
+        for (int it = a; it <= b; it++) {
+          prof.begin(K_RING, s);
+          launch_ring(B, G, F2, ring, it - a, 0, 9, 0, s);
+          prof.end(s);
+          forward_step(W2, P2, Sf, it, false, s);
+        }
+      }
+      for (int it = b; it >= a; it--) {
+        const long slot = it - a;
+        prof.begin(K_VELOCITY_BWD, s);
+        launch_velocity_bwd(form, G, W, A, P, M, C, Gr, s);
+        prof.end(s);
+        launch_ring(B, G, F, ring, slot, 0, 3, 1, s);
+        if (!src_interior) launch_add_source(W.szz, W.sxx, W.syy, src_cell, stf_dev, it, G.dt, -1, s);
+        launch_inject(A, rec_dev, nrec, it, res_t, s);
+        prof.begin(K_STRESS_BWD, s);
+        launch_stress_bwd(G, W, A, P, M, C, Gr, S, it, s);
+        prof.end(s);
+        launch_ring(B, G, F, ring, slot, 3, 9, 1, s);
+      }
     }
     HIP_TRY(hipGetLastError());
   }

@@ -50,9 +50,15 @@ struct Ctx {
   Band B{};
   float *lam = nullptr, *mu = nullptr, *byc = nullptr;
   bool model_set = false;
-  // ring of the reverse reconstruction: [nt-1][9][B.n]
+  // ring of the reverse reconstruction: [seg_len][9][B.n]
   float* ring = nullptr;
   size_t ring_bytes = 0;
+  int seg_len = 0, nseg = 1;
+  // memory_mode 1: checkpoints [nseg][ckpt_floats] and the replay state
+  float* ckpt = nullptr;
+  size_t ckpt_bytes = 0, ckpt_floats = 0;
+  Wave W2{};
+  Psi P2{};
   // per-shot buffers
   float* stf_dev = nullptr;
   float* w2_dev = nullptr;
@@ -79,7 +85,12 @@ struct Ctx {
   ~Ctx();
   int alloc(void** p, size_t bytes);
   int init();
+  int plan_memory();
   int ensure_ring();
+  int ensure_checkpoints();
+  size_t slab_floats(int axis) const;
+  int checkpoint(int k, const Wave& Wf, const Psi& Pf, bool save, hipStream_t s);
+  void forward_step(const Wave& Wf, const Psi& Pf, const Source& Sf, int it, bool record, hipStream_t s);
   int set_model(const float* l, const float* m, const float* rho, int on_host, hipStream_t s);
   int zero_gradient(hipStream_t s);
   int prepare_shot(const efwi_shot* sh, int calc_id);

@@ -30,7 +30,8 @@ class EfwiError(RuntimeError):
 class Config(C.Structure):
     _fields_ = [("nz", C.c_int), ("nx", C.c_int), ("ny", C.c_int), ("npml", C.c_int), ("nt", C.c_int),
                 ("dz", C.c_float), ("dx", C.c_float), ("dy", C.c_float), ("dt", C.c_float),
-                ("f0", C.c_float), ("formulation", C.c_int), ("if_win", C.c_int), ("device", C.c_int)]
+                ("f0", C.c_float), ("formulation", C.c_int), ("if_win", C.c_int), ("device", C.c_int),
+                ("memory_mode", C.c_int), ("checkpoint_every", C.c_int)]
 
 
 class Shot(C.Structure):

@@ -31,6 +31,8 @@ class GridSpec:
     f0: float
     formulation: int = 0     # 0 reference, 1 consistent
     if_win: bool = False
+    memory_mode: int = -1    # -1 auto, 0 boundary ring of all steps, 1 ring rebuilt per checkpointed segment
+    checkpoint_every: int = 0  # memory_mode 1 segment length (0 = ceil(sqrt(nt - 1)))
 
     @property
     def shape(self):
@@ -90,7 +92,8 @@ class Propagator:
         self.grid = grid
         self.device = torch.cuda.current_device() if device is None else int(device)
         cfg = N.Config(grid.nz, grid.nx, grid.ny, grid.npml, grid.nt, grid.dz, grid.dx, grid.dy,
-                       grid.dt, grid.f0, int(grid.formulation), int(bool(grid.if_win)), self.device)
+                       grid.dt, grid.f0, int(grid.formulation), int(bool(grid.if_win)), self.device,
+                       int(grid.memory_mode), int(grid.checkpoint_every))
         h = C.c_void_p()
         with torch.cuda.device(self.device):
             N.check(N.lib().efwi_create(C.byref(cfg), C.byref(h)), "efwi_create")

@@ -44,6 +44,12 @@ typedef struct {
   int formulation;    /* 0 = reference (default), 1 = consistent (quirks corrected) */
   int if_win;         /* per-trace windows and weights (Src/utilities.cu:524-576) */
   int device;         /* HIP device ordinal */
+  int memory_mode;    /* -1 = automatic (0 when its ring fits in 90 % of free device memory, else 1);
+                         0 = boundary ring of all nt steps in HBM (the reference's scheme);
+                         1 = the same ring rebuilt per segment from checkpoints of the forward
+                             state every checkpoint_every steps (bit-identical gradient, memory
+                             ~ sqrt(nt) instead of nt, one extra forward pass) */
+  int checkpoint_every; /* memory_mode 1 segment length; 0 = ceil(sqrt(nt - 1)) */
 } efwi_config;
 
 /* One shot. Replaces one "shot<i>" object of the survey JSON

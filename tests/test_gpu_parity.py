@@ -73,3 +73,18 @@ def test_gradient_matches_oracle(orc, small_case, formulation):
         print(f"grad {name}: rel-L2 {e:.3e} bit-exact={np.array_equal(a, b.astype(np.float32))}")
         assert np.abs(b).max() > 0
         assert e <= TOL, name
+
+
+@pytest.mark.parametrize("every", [7, 0])
+def test_checkpointed_ring_is_bit_identical(orc, small_case, every):
+    """memory_mode 1 (checkpoints every `every` steps, replayed per segment)
+    gives the same gradient as the full ring, bit for bit."""
+    import dataclasses
+    from efwi import synth
+    w0, obs = small_case
+    w = synth.small(n=32, npml=6, nt=140)
+    _, _, _, g_ring = _gpu_run(dataclasses.replace(w, grid=dataclasses.replace(w.grid, memory_mode=0)), w.model, 1, obs)
+    wc = dataclasses.replace(w, grid=dataclasses.replace(w.grid, memory_mode=1, checkpoint_every=every))
+    j, _, _, g_ck = _gpu_run(wc, w.model, 1, obs)
+    for a, b in zip(g_ck, g_ring):
+        assert np.array_equal(a, b)
