@@ -66,9 +66,12 @@ def main():
     ap.add_argument("--warmup", type=int, default=1)
     ap.add_argument("--workload", default="cfg2")
     ap.add_argument("--nt", type=int, default=None, help="override the number of time steps")
-    ap.add_argument("--cpu-steps", type=int, default=6)
+    ap.add_argument("--cpu-steps", type=int, default=100)
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-profile", action="store_true", help="skip the per-kernel event timing")
+    ap.add_argument("--no-autotune", action="store_true", help="keep the default launch shapes")
+    ap.add_argument("--tile", action="append", default=[], metavar="KERNEL=ROWSxZ",
+                    help="fix a launch shape, e.g. stress_fwd=2x4 (repeatable)")
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",
                     help="only run 5 plain device copies of MIB MiB (scale of the HBM byte counters)")
     args = ap.parse_args()
@@ -104,6 +107,16 @@ def main():
     _, obs, _ = P.run_shot(shot, 2)
     P.set_model(*(torch.from_numpy(a) for a in w.model))
     torch.cuda.synchronize()
+
+    tune = None
+    if not args.no_autotune:
+        tune = P.autotune(reps=3)
+        print("[bench] autotune:", {k: min(v, key=v.get) for k, v in tune.items()}, file=sys.stderr, flush=True)
+    for spec in args.tile:
+        k, shape = spec.split("=")
+        r, z = shape.split("x")
+        P.set_tile(k, int(r), int(z))
+    tiles = P.tiles()
 
     def step():
         P.zero_gradient()
@@ -164,6 +177,7 @@ def main():
         "config": {"workload": f"{w.name}: {g.nz - 2 * g.npml}x{g.nx - 2 * g.npml}x{g.ny - 2 * g.npml} physical "
                                f"(z,x,y) + {g.npml} PML cells, padded {g.nz}x{g.nx}x{g.ny}, nt={g.nt}, "
                                f"1 shot, {shot.nrec} receivers, fwd+adj+grad, ring reconstruction",
+                   "tiles": {k: f"{r}x{z}" for k, (r, z) in tiles.items()},
                    "pipeline_alg_GBps": round(pipe_gbs, 1),
                    "pipeline_hbm_frac": round(pipe_gbs / HBM_PEAK_GBS, 4),
                    "kernels": kernels},

@@ -512,11 +512,11 @@ int Ctx::prepare_shot(const efwi_shot* sh, int calc_id) {
 // the receiver gather of step it + 1.
 void Ctx::forward_step(const Wave& Wf, const Psi& Pf, const Source& Sf, int it, bool record, hipStream_t s) {
   prof.begin(K_STRESS_FWD, s);
-  launch_stress_fwd(G, Wf, Pf, M, C, Sf, it, s);
+  launch_stress_fwd(tiles[K_STRESS_FWD], G, Wf, Pf, M, C, Sf, it, s);
   prof.end(s);
   if (!src_stencil) launch_add_source(Wf.szz, Wf.sxx, Wf.syy, src_cell, stf_dev, it, G.dt, +1, s);
   prof.begin(K_VELOCITY_FWD, s);
-  launch_velocity_fwd(cfg.formulation, G, Wf, Pf, M, C, s);
+  launch_velocity_fwd(tiles[K_VELOCITY_FWD], cfg.formulation, G, Wf, Pf, M, C, s);
   prof.end(s);
   if (record) launch_record(Wf, rec_dev, nrec, it + 1, syn_t, s);
 }
@@ -608,13 +608,13 @@ int Ctx::run_shot(const efwi_shot* sh, int calc_id, const float* obs, float* syn
       for (int it = b; it >= a; it--) {
         const long slot = it - a;
         prof.begin(K_VELOCITY_BWD, s);
-        launch_velocity_bwd(form, G, W, A, P, M, C, Gr, s);
+        launch_velocity_bwd(tiles[K_VELOCITY_BWD], form, G, W, A, P, M, C, Gr, s);
         prof.end(s);
         launch_ring(B, G, F, ring, slot, 0, 3, 1, s);
         if (!src_interior) launch_add_source(W.szz, W.sxx, W.syy, src_cell, stf_dev, it, G.dt, -1, s);
         launch_inject(A, rec_dev, nrec, it, res_t, s);
         prof.begin(K_STRESS_BWD, s);
-        launch_stress_bwd(G, W, A, P, M, C, Gr, S, it, s);
+        launch_stress_bwd(tiles[K_STRESS_BWD], G, W, A, P, M, C, Gr, S, it, s);
         prof.end(s);
         launch_ring(B, G, F, ring, slot, 3, 9, 1, s);
       }
@@ -622,6 +622,68 @@ int Ctx::run_shot(const efwi_shot* sh, int calc_id, const float* obs, float* syn
     HIP_TRY(hipGetLastError());
   }
   if (prof.on && (rc = prof.collect())) return rc;
+  return EFWI_OK;
+}
+
+// Autotuner: time each stencil kernel of the time loop under every tile
+// shape (rows per workgroup x planes per thread) on this grid, with HIP events
+// on the stream, and keep the fastest. Runs on zeroed state (no source), which
+// the kernels leave zero; the state is zeroed again afterwards.
+int Ctx::autotune(int reps, hipStream_t s, double* report) {
+  HIP_TRY(hipSetDevice(cfg.device));
+  if (!model_set) return fail(EFWI_ERR_STATE, "efwi_set_model was not called");
+  if (reps < 1) reps = 1;
+  static const Tile cands[] = {{1, 1}, {1, 2}, {1, 4}, {1, 8}, {2, 1}, {2, 2}, {2, 4}, {2, 8},
+                               {4, 1}, {4, 2}, {4, 4}, {4, 8}};
+  const int ncand = sizeof(cands) / sizeof(cands[0]);
+  zero_wave(W, N, s);
+  zero_wave(A, N, s);
+  zero_psi(P, N, s);
+  Source none;
+  none.cell = -1;
+  none.stf = stf_dev;
+  hipEvent_t e0, e1;
+  HIP_TRY(hipEventCreate(&e0));
+  HIP_TRY(hipEventCreate(&e1));
+  const int kinds[4] = {K_STRESS_FWD, K_VELOCITY_FWD, K_VELOCITY_BWD, K_STRESS_BWD};
+  for (int q = 0; q < 4; q++) {
+    const int k = kinds[q];
+    double best = 1e30;
+    Tile bt = tiles[k];
+    for (int c = 0; c < ncand; c++) {
+      const Tile t = cands[c];
+      auto launch = [&]() {
+        if (k == K_STRESS_FWD) launch_stress_fwd(t, G, W, P, M, C, none, 0, s);
+        else if (k == K_VELOCITY_FWD) launch_velocity_fwd(t, cfg.formulation, G, W, P, M, C, s);
+        else if (k == K_VELOCITY_BWD) launch_velocity_bwd(t, cfg.formulation, G, W, A, P, M, C, Gr, s);
+        else launch_stress_bwd(t, G, W, A, P, M, C, Gr, none, 0, s);
+      };
+      launch();  // warm-up
+      HIP_TRY(hipEventRecord(e0, s));
+      for (int r = 0; r < reps; r++) launch();
+      HIP_TRY(hipEventRecord(e1, s));
+      HIP_TRY(hipEventSynchronize(e1));
+      float ms = 0.0f;
+      HIP_TRY(hipEventElapsedTime(&ms, e0, e1));
+      const double avg = ms / reps;
+      if (report) report[q * ncand + c] = avg;
+      if (avg < best) {
+        best = avg;
+        bt = t;
+      }
+    }
+    tiles[k] = bt;
+  }
+  hipEventDestroy(e0);
+  hipEventDestroy(e1);
+  HIP_TRY(hipGetLastError());
+  zero_wave(W, N, s);
+  zero_wave(A, N, s);
+  zero_psi(P, N, s);
+  launch_fill(Gr.glam, N, 0.0f, s);
+  launch_fill(Gr.gmu, N, 0.0f, s);
+  launch_fill(Gr.grho, N, 0.0f, s);
+  HIP_TRY(hipStreamSynchronize(s));
   return EFWI_OK;
 }
 
@@ -710,6 +772,29 @@ int efwi_profile_read(efwi_ctx* ctx, double* ms, long long* launches, int reset)
     }
   }
   return EFWI_OK;
+}
+
+int efwi_set_tile(efwi_ctx* ctx, int kernel, int rows, int zdepth) {
+  if (!ctx) return fail(EFWI_ERR_ARG, "NULL context");
+  if (kernel < 0 || kernel > efwi::K_STRESS_ADJ) return fail(EFWI_ERR_ARG, "kernel index out of range");
+  if (rows != 1 && rows != 2 && rows != 4) return fail(EFWI_ERR_ARG, "rows must be 1, 2 or 4");
+  if (zdepth < 1 || zdepth > 64) return fail(EFWI_ERR_ARG, "zdepth must be in [1, 64]");
+  reinterpret_cast<Ctx*>(ctx)->tiles[kernel] = efwi::Tile{rows, zdepth};
+  return EFWI_OK;
+}
+
+int efwi_get_tile(const efwi_ctx* ctx, int kernel, int* rows, int* zdepth) {
+  if (!ctx || !rows || !zdepth) return fail(EFWI_ERR_ARG, "NULL argument");
+  if (kernel < 0 || kernel >= efwi::K_COUNT) return fail(EFWI_ERR_ARG, "kernel index out of range");
+  const efwi::Tile t = reinterpret_cast<const Ctx*>(ctx)->tiles[kernel];
+  *rows = t.rows;
+  *zdepth = t.zdepth;
+  return EFWI_OK;
+}
+
+int efwi_autotune(efwi_ctx* ctx, int reps, double* report, void* stream) {
+  if (!ctx) return fail(EFWI_ERR_ARG, "NULL context");
+  return reinterpret_cast<Ctx*>(ctx)->autotune(reps, (hipStream_t)stream, report);
 }
 
 long long efwi_device_bytes(const efwi_ctx* ctx) {

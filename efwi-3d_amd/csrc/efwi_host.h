@@ -39,6 +39,7 @@ struct Profiler {
 struct Ctx {
   static constexpr int kMaxPartials = 4096;
   Profiler prof;
+  Tile tiles[K_COUNT] = {{1, 1}, {1, 1}, {1, 1}, {1, 1}, {1, 1}, {1, 1}, {1, 1}, {1, 1}};
   efwi_config cfg{};
   Geom G{};
   long N = 0;
@@ -97,6 +98,7 @@ struct Ctx {
   int run_shot(const efwi_shot* sh, int calc_id, const float* obs, float* syn, float* res,
                double* misfit, hipStream_t s);
   int get_gradient(float* gl, float* gm, float* gr, int on_host, hipStream_t s);
+  int autotune(int reps, hipStream_t s, double* report);
 };
 
 }  // namespace efwi

@@ -20,24 +20,29 @@ struct Fields9 {
   float* f[9];
 };
 
-// XCD-aware sweep of a cell box [z0..z0+nz-1] x [x0..] x [y0..y1]: one
-// 64-lane workgroup per y-segment of a row; workgroup b belongs to XCD b % 8,
-// which owns a contiguous strip of rows_base (+1 for the first rows_rem XCDs)
-// x rows and walks it plane by plane.
+// Launch shape of a stencil kernel: workgroups of 64 (y lanes) x `rows`
+// (x rows) threads, each thread updating `zdepth` consecutive z planes.
+struct Tile {
+  int rows, zdepth;
+};
+
+// XCD-aware sweep of a cell box [z0..z0+nz-1] x [x0..] x [y0..y1]: workgroup
+// b belongs to XCD b % 8, which owns a contiguous strip of rows_base (+1 for
+// the first rows_rem XCDs) x rows and walks it by chunks of Z planes.
 struct Sweep {
-  int z0, nz, x0, rows_base, rows_rem, rows_max, y0, y1, nyb;
+  int z0, nz, x0, rows_base, rows_rem, rg_max, y0, y1, nyb, R, Z, nzc;
   long blocks;
 };
-Sweep make_sweep(int z0, int z1, int x0, int x1, int y0, int y1);
+Sweep make_sweep(int z0, int z1, int x0, int x1, int y0, int y1, Tile t);
 
 // stencil kernels (efwi_stencil.hip)
-void launch_stress_fwd(const Geom& G, const Wave& W, const Psi& P, const Model& M, const Cpml& C,
+void launch_stress_fwd(Tile t, const Geom& G, const Wave& W, const Psi& P, const Model& M, const Cpml& C,
                        const Source& S, int it, hipStream_t s);
-void launch_velocity_fwd(int form, const Geom& G, const Wave& W, const Psi& P, const Model& M,
+void launch_velocity_fwd(Tile t, int form, const Geom& G, const Wave& W, const Psi& P, const Model& M,
                          const Cpml& C, hipStream_t s);
-void launch_velocity_bwd(int form, const Geom& G, const Wave& W, const Wave& A, const Psi& P, const Model& M,
-                         const Cpml& C, const Grad& Gr, hipStream_t s);
-void launch_stress_bwd(const Geom& G, const Wave& W, const Wave& A, const Psi& P, const Model& M,
+void launch_velocity_bwd(Tile t, int form, const Geom& G, const Wave& W, const Wave& A, const Psi& P,
+                         const Model& M, const Cpml& C, const Grad& Gr, hipStream_t s);
+void launch_stress_bwd(Tile t, const Geom& G, const Wave& W, const Wave& A, const Psi& P, const Model& M,
                        const Cpml& C, const Grad& Gr, const Source& S, int it, hipStream_t s);
 void launch_velocity_adj(int form, const Geom& G, const Wave& A, const Psi& P, const Model& M,
                          const Cpml& C, hipStream_t s);

@@ -84,22 +84,33 @@ struct Cell {
   long i;
 };
 
-__device__ __forceinline__ bool sweep_cell(const Sweep& S, const Geom& G, Cell& c) {
+// Workgroup b (64 x R threads: y lanes x rows) covers R rows of the strip of
+// XCD b % 8 and Z consecutive planes; the strip is walked plane chunk by
+// plane chunk (z slowest), so each XCD's L2 sees a compact, z-ordered region.
+template <typename Body>
+__device__ __forceinline__ void sweep(const Sweep& S, const Geom& G, Body&& body) {
   const int b = blockIdx.x;
   const int xcd = b & 7, j = b >> 3;
-  const int per_plane = S.rows_max * S.nyb;
-  const int zz = j / per_plane;
-  const int r = j - zz * per_plane;
-  const int row = r / S.nyb;
-  const int yb = r - row * S.nyb;
+  const int per_plane = S.rg_max * S.nyb;
+  const int zc = j / per_plane;
+  const int r = j - zc * per_plane;
+  const int rg = r / S.nyb;
+  const int yb = r - rg * S.nyb;
   const int rows = S.rows_base + (xcd < S.rows_rem ? 1 : 0);
-  if (row >= rows) return false;
+  const int row = rg * S.R + (int)threadIdx.y;
+  if (row >= rows) return;
+  Cell c;
   c.x = S.x0 + xcd * S.rows_base + min(xcd, S.rows_rem) + row;
-  c.z = S.z0 + zz;
   c.y = yb * 64 + (int)threadIdx.x;
-  if (c.y < S.y0 || c.y > S.y1) return false;
-  c.i = (long)c.z * G.pxy + (long)c.x * G.py + c.y;
-  return true;
+  if (c.y < S.y0 || c.y > S.y1) return;
+  const int za = S.z0 + zc * S.Z;
+  const int zb = min(za + S.Z, S.z0 + S.nz);
+  const long base = (long)c.x * G.py + c.y;
+  for (int z = za; z < zb; z++) {
+    c.z = z;
+    c.i = (long)z * G.pxy + base;
+    body(c);
+  }
 }
 
 __device__ __forceinline__ bool interior(const Geom& G, const Cell& c) {
@@ -394,56 +405,50 @@ __device__ __forceinline__ void stress_adj_cell(const Geom& G, const Wave& A, co
 // Kernels.
 // ---------------------------------------------------------------------------
 // Forward stress, then the source of this step at its cell (Src/libCUFD.cu:319-330).
-__global__ __launch_bounds__(64) void k_stress_fwd(Sweep Sw, Geom G, Wave W, Psi P, Model M, Cpml C, Source S,
-                                                   int it) {
-  Cell c;
-  if (!sweep_cell(Sw, G, c)) return;
-  stress_fwd_cell(G, W, P, M, C, c);
-  if (c.i == S.cell) {
-    const float a = 2250000.0f * S.stf[it] * G.dt * 1.0f;
-    W.szz[c.i] += a; W.sxx[c.i] += a; W.syy[c.i] += a;
-  }
+__global__ __launch_bounds__(256) void k_stress_fwd(Sweep Sw, Geom G, Wave W, Psi P, Model M, Cpml C, Source S,
+                                                    int it) {
+  sweep(Sw, G, [&](const Cell& c) {
+    stress_fwd_cell(G, W, P, M, C, c);
+    if (c.i == S.cell) {
+      const float a = 2250000.0f * S.stf[it] * G.dt * 1.0f;
+      W.szz[c.i] += a; W.sxx[c.i] += a; W.syy[c.i] += a;
+    }
+  });
 }
 
 template <int FORM>
-__global__ __launch_bounds__(64) void k_velocity_fwd(Sweep Sw, Geom G, Wave W, Psi P, Model M, Cpml C) {
-  Cell c;
-  if (!sweep_cell(Sw, G, c)) return;
-  velocity_fwd_cell<FORM>(G, W, P, M, C, c);
+__global__ __launch_bounds__(256) void k_velocity_fwd(Sweep Sw, Geom G, Wave W, Psi P, Model M, Cpml C) {
+  sweep(Sw, G, [&](const Cell& c) { velocity_fwd_cell<FORM>(G, W, P, M, C, c); });
 }
 
 // Backward velocity step: reverse velocity + density imaging (interior), then
 // the adjoint velocity ([2, n-3]).
 template <int FORM>
-__global__ __launch_bounds__(64) void k_velocity_bwd(Sweep Sw, Geom G, Wave W, Wave A, Psi P, Model M, Cpml C,
-                                                     Grad Gr) {
-  Cell c;
-  if (!sweep_cell(Sw, G, c)) return;
-  if (interior(G, c)) velocity_rev_cell<FORM>(G, W, A, M, Gr, c);
-  velocity_adj_cell<FORM>(G, A, P, M, C, c);
+__global__ __launch_bounds__(256) void k_velocity_bwd(Sweep Sw, Geom G, Wave W, Wave A, Psi P, Model M, Cpml C,
+                                                      Grad Gr) {
+  sweep(Sw, G, [&](const Cell& c) {
+    if (interior(G, c)) velocity_rev_cell<FORM>(G, W, A, M, Gr, c);
+    velocity_adj_cell<FORM>(G, A, P, M, C, c);
+  });
 }
 
 // Backward stress step: (source subtraction +) reverse stress + lambda/mu
 // imaging (interior), then the adjoint stress ([2, n-3]).
-__global__ __launch_bounds__(64) void k_stress_bwd(Sweep Sw, Geom G, Wave W, Wave A, Psi P, Model M, Cpml C,
-                                                   Grad Gr, Source S, int it) {
-  Cell c;
-  if (!sweep_cell(Sw, G, c)) return;
-  if (interior(G, c)) stress_rev_cell(G, W, A, M, Gr, S, it, c);
-  stress_adj_cell(G, A, P, M, C, c);
+__global__ __launch_bounds__(256) void k_stress_bwd(Sweep Sw, Geom G, Wave W, Wave A, Psi P, Model M, Cpml C,
+                                                    Grad Gr, Source S, int it) {
+  sweep(Sw, G, [&](const Cell& c) {
+    if (interior(G, c)) stress_rev_cell(G, W, A, M, Gr, S, it, c);
+    stress_adj_cell(G, A, P, M, C, c);
+  });
 }
 
 template <int FORM>
-__global__ __launch_bounds__(64) void k_velocity_adj(Sweep Sw, Geom G, Wave A, Psi P, Model M, Cpml C) {
-  Cell c;
-  if (!sweep_cell(Sw, G, c)) return;
-  velocity_adj_cell<FORM>(G, A, P, M, C, c);
+__global__ __launch_bounds__(256) void k_velocity_adj(Sweep Sw, Geom G, Wave A, Psi P, Model M, Cpml C) {
+  sweep(Sw, G, [&](const Cell& c) { velocity_adj_cell<FORM>(G, A, P, M, C, c); });
 }
 
-__global__ __launch_bounds__(64) void k_stress_adj(Sweep Sw, Geom G, Wave A, Psi P, Model M, Cpml C) {
-  Cell c;
-  if (!sweep_cell(Sw, G, c)) return;
-  stress_adj_cell(G, A, P, M, C, c);
+__global__ __launch_bounds__(256) void k_stress_adj(Sweep Sw, Geom G, Wave A, Psi P, Model M, Cpml C) {
+  sweep(Sw, G, [&](const Cell& c) { stress_adj_cell(G, A, P, M, C, c); });
 }
 
 }  // namespace
@@ -451,62 +456,67 @@ __global__ __launch_bounds__(64) void k_stress_adj(Sweep Sw, Geom G, Wave A, Psi
 // ---------------------------------------------------------------------------
 // Launchers.
 // ---------------------------------------------------------------------------
-Sweep make_sweep(int z0, int z1, int x0, int x1, int y0, int y1) {
+Sweep make_sweep(int z0, int z1, int x0, int x1, int y0, int y1, Tile t) {
   Sweep S;
+  S.R = t.rows < 1 ? 1 : (t.rows > 4 ? 4 : t.rows);
+  S.Z = t.zdepth < 1 ? 1 : t.zdepth;
   S.z0 = z0; S.nz = z1 - z0 + 1;
   S.x0 = x0;
   const int nxr = x1 - x0 + 1;
   S.rows_base = nxr / 8;
   S.rows_rem = nxr % 8;
-  S.rows_max = S.rows_base + (S.rows_rem ? 1 : 0);
+  const int rows_max = S.rows_base + (S.rows_rem ? 1 : 0);
+  S.rg_max = (rows_max + S.R - 1) / S.R;
   S.y0 = y0; S.y1 = y1;
   S.nyb = (y1 + 64) / 64;  // segments of 64 aligned at y = 0
-  S.blocks = 8L * S.rows_max * S.nyb * S.nz;
+  S.nzc = (S.nz + S.Z - 1) / S.Z;
+  S.blocks = 8L * S.rg_max * S.nyb * S.nzc;
   return S;
 }
 
-Sweep sweep_full(const Geom& G) { return make_sweep(2, G.nz - 3, 2, G.nx - 3, 2, G.ny - 3); }
+static inline Sweep sweep_full(const Geom& G, Tile t) { return make_sweep(2, G.nz - 3, 2, G.nx - 3, 2, G.ny - 3, t); }
 
 static inline dim3 grid_of(const Sweep& S) { return dim3((unsigned)S.blocks); }
+static inline dim3 block_of(const Sweep& S) { return dim3(64, S.R); }
 
-void launch_stress_fwd(const Geom& G, const Wave& W, const Psi& P, const Model& M, const Cpml& C,
+void launch_stress_fwd(Tile t, const Geom& G, const Wave& W, const Psi& P, const Model& M, const Cpml& C,
                        const Source& S, int it, hipStream_t s) {
-  const Sweep Sw = sweep_full(G);
-  hipLaunchKernelGGL(k_stress_fwd, grid_of(Sw), dim3(64), 0, s, Sw, G, W, P, M, C, S, it);
+  const Sweep Sw = sweep_full(G, t);
+  hipLaunchKernelGGL(k_stress_fwd, grid_of(Sw), block_of(Sw), 0, s, Sw, G, W, P, M, C, S, it);
 }
-void launch_velocity_fwd(int form, const Geom& G, const Wave& W, const Psi& P, const Model& M,
+void launch_velocity_fwd(Tile t, int form, const Geom& G, const Wave& W, const Psi& P, const Model& M,
                          const Cpml& C, hipStream_t s) {
-  const Sweep Sw = sweep_full(G);
+  const Sweep Sw = sweep_full(G, t);
   if (form == 1)
-    hipLaunchKernelGGL(k_velocity_fwd<1>, grid_of(Sw), dim3(64), 0, s, Sw, G, W, P, M, C);
+    hipLaunchKernelGGL(k_velocity_fwd<1>, grid_of(Sw), block_of(Sw), 0, s, Sw, G, W, P, M, C);
   else
-    hipLaunchKernelGGL(k_velocity_fwd<0>, grid_of(Sw), dim3(64), 0, s, Sw, G, W, P, M, C);
+    hipLaunchKernelGGL(k_velocity_fwd<0>, grid_of(Sw), block_of(Sw), 0, s, Sw, G, W, P, M, C);
 }
-void launch_velocity_bwd(int form, const Geom& G, const Wave& W, const Wave& A, const Psi& P, const Model& M,
-                         const Cpml& C, const Grad& Gr, hipStream_t s) {
-  const Sweep Sw = sweep_full(G);
+void launch_velocity_bwd(Tile t, int form, const Geom& G, const Wave& W, const Wave& A, const Psi& P,
+                         const Model& M, const Cpml& C, const Grad& Gr, hipStream_t s) {
+  const Sweep Sw = sweep_full(G, t);
   if (form == 1)
-    hipLaunchKernelGGL(k_velocity_bwd<1>, grid_of(Sw), dim3(64), 0, s, Sw, G, W, A, P, M, C, Gr);
+    hipLaunchKernelGGL(k_velocity_bwd<1>, grid_of(Sw), block_of(Sw), 0, s, Sw, G, W, A, P, M, C, Gr);
   else
-    hipLaunchKernelGGL(k_velocity_bwd<0>, grid_of(Sw), dim3(64), 0, s, Sw, G, W, A, P, M, C, Gr);
+    hipLaunchKernelGGL(k_velocity_bwd<0>, grid_of(Sw), block_of(Sw), 0, s, Sw, G, W, A, P, M, C, Gr);
 }
-void launch_stress_bwd(const Geom& G, const Wave& W, const Wave& A, const Psi& P, const Model& M,
+void launch_stress_bwd(Tile t, const Geom& G, const Wave& W, const Wave& A, const Psi& P, const Model& M,
                        const Cpml& C, const Grad& Gr, const Source& S, int it, hipStream_t s) {
-  const Sweep Sw = sweep_full(G);
-  hipLaunchKernelGGL(k_stress_bwd, grid_of(Sw), dim3(64), 0, s, Sw, G, W, A, P, M, C, Gr, S, it);
+  const Sweep Sw = sweep_full(G, t);
+  hipLaunchKernelGGL(k_stress_bwd, grid_of(Sw), block_of(Sw), 0, s, Sw, G, W, A, P, M, C, Gr, S, it);
 }
 void launch_velocity_adj(int form, const Geom& G, const Wave& A, const Psi& P, const Model& M,
                          const Cpml& C, hipStream_t s) {
-  const Sweep Sw = sweep_full(G);
+  const Sweep Sw = sweep_full(G, Tile{1, 1});
   if (form == 1)
-    hipLaunchKernelGGL(k_velocity_adj<1>, grid_of(Sw), dim3(64), 0, s, Sw, G, A, P, M, C);
+    hipLaunchKernelGGL(k_velocity_adj<1>, grid_of(Sw), block_of(Sw), 0, s, Sw, G, A, P, M, C);
   else
-    hipLaunchKernelGGL(k_velocity_adj<0>, grid_of(Sw), dim3(64), 0, s, Sw, G, A, P, M, C);
+    hipLaunchKernelGGL(k_velocity_adj<0>, grid_of(Sw), block_of(Sw), 0, s, Sw, G, A, P, M, C);
 }
 void launch_stress_adj(const Geom& G, const Wave& A, const Psi& P, const Model& M, const Cpml& C,
                        hipStream_t s) {
-  const Sweep Sw = sweep_full(G);
-  hipLaunchKernelGGL(k_stress_adj, grid_of(Sw), dim3(64), 0, s, Sw, G, A, P, M, C);
+  const Sweep Sw = sweep_full(G, Tile{1, 1});
+  hipLaunchKernelGGL(k_stress_adj, grid_of(Sw), block_of(Sw), 0, s, Sw, G, A, P, M, C);
 }
 
 }  // namespace efwi

@@ -21,6 +21,8 @@ OK = 0
 KERNELS = ("stress_fwd", "velocity_fwd", "stress_bwd", "velocity_bwd", "velocity_adj", "stress_adj",
            "ring", "other")
 CALC_MISFIT, CALC_GRADIENT, CALC_SYNTHETIC = 0, 1, 2
+TUNED_KERNELS = ("stress_fwd", "velocity_fwd", "stress_bwd", "velocity_bwd")
+TILE_CANDIDATES = tuple((r, z) for r in (1, 2, 4) for z in (1, 2, 4, 8))
 
 
 class EfwiError(RuntimeError):
@@ -53,6 +55,9 @@ SIGNATURES = {
     "efwi_run_shot": (C.c_int, [_vp, C.POINTER(Shot), C.c_int, _vp, _vp, _vp, _dp, _vp]),
     "efwi_get_gradient": (C.c_int, [_vp, _vp, _vp, _vp, C.c_int, _vp]),
     "efwi_device_bytes": (C.c_longlong, [_vp]),
+    "efwi_set_tile": (C.c_int, [_vp, C.c_int, C.c_int, C.c_int]),
+    "efwi_get_tile": (C.c_int, [_vp, C.c_int, _ip, _ip]),
+    "efwi_autotune": (C.c_int, [_vp, C.c_int, _dp, _vp]),
     "efwi_profile": (C.c_int, [_vp, C.c_int]),
     "efwi_profile_read": (C.c_int, [_vp, C.POINTER(C.c_double), C.POINTER(C.c_longlong), C.c_int]),
     "efwi_cpml_profile": (C.c_int, [C.c_int, C.c_int, C.c_float, C.c_float, C.c_float, _fp]),

@@ -133,6 +133,28 @@ class Propagator:
         N.check(N.lib().efwi_profile_read(self._h, ms, n, int(reset)), "efwi_profile_read")
         return {k: (ms[i], int(n[i])) for i, k in enumerate(N.KERNELS)}
 
+    def set_tile(self, kernel: str, rows: int, zdepth: int):
+        """Launch shape of one time-loop kernel (results do not depend on it)."""
+        N.check(N.lib().efwi_set_tile(self._h, N.KERNELS.index(kernel), int(rows), int(zdepth)), "efwi_set_tile")
+
+    def tiles(self) -> dict:
+        out = {}
+        for k in N.TUNED_KERNELS:
+            r, z = C.c_int(), C.c_int()
+            N.check(N.lib().efwi_get_tile(self._h, N.KERNELS.index(k), C.byref(r), C.byref(z)), "efwi_get_tile")
+            out[k] = (r.value, z.value)
+        return out
+
+    def autotune(self, reps: int = 5) -> dict:
+        """Time every candidate launch shape of the time-loop kernels on this
+        grid and keep the fastest; returns {kernel: {(rows, zdepth): ms}}."""
+        nc = len(N.TILE_CANDIDATES)
+        rep = (C.c_double * (4 * nc))()
+        with torch.cuda.device(self.device):
+            N.check(N.lib().efwi_autotune(self._h, int(reps), rep, self._stream()), "efwi_autotune")
+        kinds = ("stress_fwd", "velocity_fwd", "velocity_bwd", "stress_bwd")
+        return {k: {N.TILE_CANDIDATES[c]: rep[q * nc + c] for c in range(nc)} for q, k in enumerate(kinds)}
+
     def _stream(self):
         return N.stream_handle(self.device)
 

@@ -114,6 +114,19 @@ int efwi_get_gradient(efwi_ctx* ctx, float* g_lambda, float* g_mu, float* g_rho,
 int efwi_profile(efwi_ctx* ctx, int enable);
 int efwi_profile_read(efwi_ctx* ctx, double* ms, long long* launches, int reset);
 
+/* Launch shape of the four time-loop stencil kernels (kernel index as in
+ * efwi_profile_read: 0 stress_fwd, 1 velocity_fwd, 2 stress_bwd,
+ * 3 velocity_bwd): workgroups of 64 x rows threads (rows 1, 2, 4), each
+ * thread updating zdepth consecutive z planes. Results do not depend on it.
+ * efwi_autotune times the 12 shapes {1,2,4} x {1,2,4,8} of each kernel on this
+ * grid with HIP events (reps launches each) and keeps the fastest; report, if
+ * not NULL, receives the 4 x 12 average milliseconds. No reference
+ * counterpart (the reference launches fixed 32 x 1 x 32 blocks,
+ * Src/libCUFD.cu:93-97). */
+int efwi_set_tile(efwi_ctx* ctx, int kernel, int rows, int zdepth);
+int efwi_get_tile(const efwi_ctx* ctx, int kernel, int* rows, int* zdepth);
+int efwi_autotune(efwi_ctx* ctx, int reps, double* report, void* stream);
+
 /* Bytes of device memory held by the context (state + ring). */
 long long efwi_device_bytes(const efwi_ctx* ctx);
 

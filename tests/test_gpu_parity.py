@@ -18,11 +18,13 @@ pytestmark = pytest.mark.gpu
 TOL = 1e-5
 
 
-def _gpu_run(w, model, calc_id, obs=None):
+def _gpu_run(w, model, calc_id, obs=None, tiles=None):
     from efwi.propagator import Propagator
-    print(f"[gpu] {w.name} grid={w.grid.shape} nt={w.grid.nt} form={w.grid.formulation} calc={calc_id}",
-          flush=True)
+    print(f"[gpu] {w.name} grid={w.grid.shape} nt={w.grid.nt} form={w.grid.formulation} calc={calc_id} "
+          f"tiles={tiles}", flush=True)
     with Propagator(w.grid) as P:
+        for k, (r, z) in (tiles or {}).items():
+            P.set_tile(k, r, z)
         P.set_model(*(torch.from_numpy(a) for a in model))
         P.zero_gradient()
         total = 0.0
@@ -88,3 +90,34 @@ def test_checkpointed_ring_is_bit_identical(orc, small_case, every):
     j, _, _, g_ck = _gpu_run(wc, w.model, 1, obs)
     for a, b in zip(g_ck, g_ring):
         assert np.array_equal(a, b)
+
+
+@pytest.mark.parametrize("rows,zdepth", [(2, 3), (4, 8), (1, 64)])
+def test_tile_shapes_are_bit_identical(orc, small_case, rows, zdepth):
+    """Launch shapes only remap cells to threads: gradients and seismograms
+    are bit-identical to the default shape."""
+    from efwi import synth
+    w0, obs = small_case
+    w = synth.small(n=32, npml=6, nt=140)
+    ref = _gpu_run(w, w.model, 1, obs)
+    t = {k: (rows, zdepth) for k in ("stress_fwd", "velocity_fwd", "stress_bwd", "velocity_bwd")}
+    got = _gpu_run(w, w.model, 1, obs, tiles=t)
+    assert got[0] == ref[0]
+    assert np.array_equal(got[1][0], ref[1][0])
+    for a, b in zip(got[3], ref[3]):
+        assert np.array_equal(a, b)
+
+
+def test_autotuner_picks_a_candidate(small_case):
+    import torch
+    from efwi import synth
+    from efwi.propagator import Propagator
+    w = synth.small(n=32, npml=6, nt=20)
+    with Propagator(w.grid) as P:
+        P.set_model(*(torch.from_numpy(a) for a in w.model))
+        rep = P.autotune(reps=2)
+        tiles = P.tiles()
+    for k, times in rep.items():
+        assert len(times) == 12 and all(t > 0 for t in times.values())
+        best = min(times, key=times.get)
+        assert tiles[k] == best, (k, tiles[k], best)
