@@ -147,6 +147,20 @@ __global__ __launch_bounds__(256) void k_misfit(const float* __restrict__ obs_t,
   if (threadIdx.x == 0) partials[blockIdx.x] = (part[0] + part[1]) + (part[2] + part[3]);
 }
 
+// FP64 dot product of two float vectors: per-block partial sums in double.
+__global__ __launch_bounds__(256) void k_dot64(const float* __restrict__ a, const float* __restrict__ b, long n,
+                                               double* __restrict__ partials) {
+  double acc = 0.0;
+  for (long k = (long)blockIdx.x * blockDim.x + threadIdx.x; k < n; k += (long)gridDim.x * blockDim.x)
+    acc += (double)a[k] * (double)b[k];
+  for (int off = 32; off > 0; off >>= 1) acc += __shfl_down(acc, off, 64);
+  __shared__ double part[4];
+  const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
+  if (lane == 0) part[wid] = acc;
+  __syncthreads();
+  if (threadIdx.x == 0) partials[blockIdx.x] = (part[0] + part[1]) + (part[2] + part[3]);
+}
+
 __global__ __launch_bounds__(256) void k_sum_partials(const double* __restrict__ p, int n, double* out) {
   double acc = 0.0;
   for (int k = threadIdx.x; k < n; k += blockDim.x) acc += p[k];
@@ -233,6 +247,12 @@ int launch_misfit(const float* obs_t, const float* syn_t, float* res_t, const fl
                      partials);
   return nb;
 }
+int launch_dot64(const float* a, const float* b, long n, double* partials, int max_blocks, hipStream_t s) {
+  const int nb = blocks_for(n, 256, max_blocks);
+  hipLaunchKernelGGL(k_dot64, dim3(nb), dim3(256), 0, s, a, b, n, partials);
+  return nb;
+}
+
 void launch_sum_partials(const double* partials, int n, double* out, hipStream_t s) {
   hipLaunchKernelGGL(k_sum_partials, dim3(1), dim3(256), 0, s, partials, n, out);
 }

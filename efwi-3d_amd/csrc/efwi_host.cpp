@@ -797,6 +797,23 @@ int efwi_autotune(efwi_ctx* ctx, int reps, double* report, void* stream) {
   return reinterpret_cast<Ctx*>(ctx)->autotune(reps, (hipStream_t)stream, report);
 }
 
+int efwi_dot64(const float* a, const float* b, long long n, double* out, void* stream) {
+  if (!a || !b || !out || n < 0) return fail(EFWI_ERR_ARG, "bad argument");
+  hipStream_t s = (hipStream_t)stream;
+  double* buf = nullptr;
+  const int maxb = 2048;
+  hipError_t e = hipMallocAsync((void**)&buf, sizeof(double) * (maxb + 1), s);
+  if (e != hipSuccess) return fail(EFWI_ERR_NOMEM, std::string("dot workspace: ") + hipGetErrorString(e));
+  const int nb = efwi::launch_dot64(a, b, (long)n, buf + 1, maxb, s);
+  efwi::launch_sum_partials(buf + 1, nb, buf, s);
+  e = hipMemcpyAsync(out, buf, sizeof(double), hipMemcpyDeviceToHost, s);
+  hipFreeAsync(buf, s);
+  if (e != hipSuccess) return fail(EFWI_ERR_HIP, std::string("dot result copy: ") + hipGetErrorString(e));
+  e = hipStreamSynchronize(s);
+  if (e != hipSuccess) return fail(EFWI_ERR_HIP, std::string("dot synchronize: ") + hipGetErrorString(e));
+  return EFWI_OK;
+}
+
 long long efwi_device_bytes(const efwi_ctx* ctx) {
   return ctx ? reinterpret_cast<const Ctx*>(ctx)->bytes_held : 0;
 }

@@ -66,6 +66,8 @@ void launch_ring(const Band& B, const Geom& G, const Fields9& F, float* ring, lo
 int launch_misfit(const float* obs_t, const float* syn_t, float* res_t, const float* fac,
                   int per_trace, int nt, int nrec, double* partials, int max_blocks, hipStream_t s);
 void launch_sum_partials(const double* partials, int n, double* out, hipStream_t s);
+// FP64 dot product of float vectors; returns the number of partials written.
+int launch_dot64(const float* a, const float* b, long n, double* partials, int max_blocks, hipStream_t s);
 // [3][nrec][nt] <-> [nt][3][nrec]
 void launch_to_time_major(const float* src, float* dst, int nrec, int nt, hipStream_t s);
 void launch_from_time_major(const float* src, float* dst, int nrec, int nt, hipStream_t s);

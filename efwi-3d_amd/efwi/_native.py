@@ -55,6 +55,7 @@ SIGNATURES = {
     "efwi_run_shot": (C.c_int, [_vp, C.POINTER(Shot), C.c_int, _vp, _vp, _vp, _dp, _vp]),
     "efwi_get_gradient": (C.c_int, [_vp, _vp, _vp, _vp, C.c_int, _vp]),
     "efwi_device_bytes": (C.c_longlong, [_vp]),
+    "efwi_dot64": (C.c_int, [_vp, _vp, C.c_longlong, _dp, _vp]),
     "efwi_set_tile": (C.c_int, [_vp, C.c_int, C.c_int, C.c_int]),
     "efwi_get_tile": (C.c_int, [_vp, C.c_int, _ip, _ip]),
     "efwi_autotune": (C.c_int, [_vp, C.c_int, _dp, _vp]),
@@ -102,3 +103,15 @@ def vptr(t):
 def stream_handle(device=None):
     """The current torch stream of `device` as a hipStream_t (void*)."""
     return C.c_void_p(torch.cuda.current_stream(device).cuda_stream)
+
+
+def dot64(a, b) -> float:
+    """FP64 dot product of two float32 CUDA tensors (libefwi.so k_dot64)."""
+    assert a.is_cuda and b.is_cuda and a.dtype == torch.float32 and b.dtype == torch.float32
+    a = a.contiguous()
+    b = b.contiguous()
+    assert a.numel() == b.numel()
+    out = C.c_double(0.0)
+    with torch.cuda.device(a.device):
+        check(lib().efwi_dot64(vptr(a), vptr(b), a.numel(), C.byref(out), stream_handle(a.device)), "efwi_dot64")
+    return out.value

@@ -127,6 +127,12 @@ int efwi_set_tile(efwi_ctx* ctx, int kernel, int rows, int zdepth);
 int efwi_get_tile(const efwi_ctx* ctx, int kernel, int* rows, int* zdepth);
 int efwi_autotune(efwi_ctx* ctx, int reps, double* report, void* stream);
 
+/* FP64 dot product of two float device vectors on `stream` (per-block FP64
+ * partial sums, then one FP64 reduction); *out is a host double. Used by the
+ * L-BFGS two-loop recursion and gradient norms. No reference counterpart (the
+ * reference's optimiser runs in scipy on the host). */
+int efwi_dot64(const float* a, const float* b, long long n, double* out, void* stream);
+
 /* Bytes of device memory held by the context (state + ring). */
 long long efwi_device_bytes(const efwi_ctx* ctx);
 

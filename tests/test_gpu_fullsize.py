@@ -1,0 +1,78 @@
+"""Full-size plain-method workload (64^3 physical + 10 PML cells, 500 steps,
+4096 receivers) on the GPU against the oracle's summary of the same run
+(tests/golden/cfg1_summary.npz, written by make_cfg1_summary.py), plus
+size-independent properties at that size, and the FP64 dot product that the
+L-BFGS driver uses."""
+import os
+
+import numpy as np
+import pytest
+import torch
+
+pytestmark = pytest.mark.gpu
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+TOL = 1e-5
+
+
+def _rel(a, b):
+    a = np.asarray(a, np.float64); b = np.asarray(b, np.float64)
+    n = np.linalg.norm(b.ravel())
+    return np.linalg.norm((a - b).ravel()) / n if n > 0 else np.linalg.norm(a.ravel())
+
+
+@pytest.fixture(scope="module")
+def cfg1_gpu():
+    from efwi import synth
+    from efwi.propagator import Propagator
+    import sys
+    sys.path.insert(0, os.path.join(HERE, "golden"))
+    from make_cfg1_summary import summarize
+    w = synth.cfg1()
+    print(f"[gpu] cfg1 grid={w.grid.shape} nt={w.grid.nt} nrec={len(w.shots[0].rec)}", flush=True)
+    with Propagator(w.grid) as P:
+        P.set_model(*(torch.from_numpy(a) for a in w.true))
+        _, obs, _ = P.run_shot(w.shots[0], 2, None, want_syn=True)
+        obs = obs.cpu().numpy()
+        P.set_model(*(torch.from_numpy(a) for a in w.model))
+        P.zero_gradient()
+        j, syn, res = P.run_shot(w.shots[0], 1, torch.from_numpy(obs), want_syn=True, want_res=True)
+        g = [x.cpu().numpy() for x in P.gradient()]
+        torch.cuda.synchronize()
+    out = summarize(syn.cpu().numpy(), res.cpu().numpy(), j, *g)
+    out["obs_trace_energy"] = np.sum(np.asarray(obs, np.float64) ** 2, axis=2)
+    return out, g
+
+
+def test_cfg1_matches_oracle_summary(cfg1_gpu):
+    got, _ = cfg1_gpu
+    ref = np.load(os.path.join(HERE, "golden", "cfg1_summary.npz"))
+    assert abs(got["misfit64"] - ref["misfit64"]) <= 1e-9 * abs(ref["misfit64"])
+    for k in ref.files:
+        e = _rel(got[k], ref[k])
+        print(f"{k}: rel-L2 {e:.3e}")
+        assert e <= TOL, k
+    assert np.array_equal(got["traces"], ref["traces"])
+
+
+def test_cfg1_gradient_is_finite_and_localised(cfg1_gpu):
+    """The anomaly sits at the grid centre: the lambda gradient is finite and
+    zero in the PML (reverse kernels cover the interior only)."""
+    _, (glam, gmu, grho) = cfg1_gpu
+    for g in (glam, gmu, grho):
+        assert np.all(np.isfinite(g))
+    p = 10
+    assert np.all(glam[:p - 0] == 0) and np.all(glam[-p:] == 0)
+    assert np.abs(glam[p:-p, p:-p, p:-p]).max() > 0
+
+
+@pytest.mark.parametrize("n", [1, 255, 256 * 1024 + 7, 5 * 10 ** 6])
+def test_dot64_matches_numpy(n):
+    from efwi._native import dot64
+    rng = np.random.default_rng(n)
+    a = rng.standard_normal(n).astype(np.float32)
+    b = rng.standard_normal(n).astype(np.float32)
+    got = dot64(torch.from_numpy(a).cuda(), torch.from_numpy(b).cuda())
+    ref = float(np.dot(a.astype(np.float64), b.astype(np.float64)))
+    # each product of two floats is exact in FP64; only the summation order differs
+    assert abs(got - ref) <= 1e-12 * max(1.0, float(np.sum(np.abs(a.astype(np.float64) * b))))
