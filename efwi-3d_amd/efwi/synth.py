@@ -72,6 +72,8 @@ class Workload:
     true: tuple           # (lam, mu, rho) padded float32, the model that makes the observed data
     model: tuple          # (lam, mu, rho) padded float32, the model whose misfit/gradient is computed
     vp_model: np.ndarray  # padded vp of `model` (for stability checks)
+    velocities_true: tuple = None   # padded float32 (vp, vs, rho[kg/m^3]) of `true`
+    velocities_model: tuple = None  # padded float32 (vp, vs, rho[kg/m^3]) of `model`
 
     def courant(self):
         g = self.grid
@@ -87,7 +89,9 @@ def _finish(name, nzp, nxp, nyp, p, h, dt, nt, f0, vp_t, vs_t, rho_t, vp_m, vs_m
     model = lame(pads(vp_m), pads(vs_m), pads(rho_m))
     stf = ricker_integrated(f0, nt, dt).astype(np.float32)
     shots = [ShotSpec((s[0] + p, s[1] + p, s[2] + p), stf, np.asarray(rec_zxy, np.int32) + p) for s in src]
-    return Workload(name, grid, shots, true, model, pads(vp_m))
+    f32 = lambda a: pads(a).astype(np.float32)
+    return Workload(name, grid, shots, true, model, pads(vp_m), (f32(vp_t), f32(vs_t), f32(rho_t)),
+                    (f32(vp_m), f32(vs_m), f32(rho_m)))
 
 
 def cfg1(nt=500, n=64, npml=10):
