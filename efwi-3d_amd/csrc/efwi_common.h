@@ -17,6 +17,33 @@ struct Geom {
   int py;        // row pitch in floats
   long pxy;      // plane pitch in floats
   float dz, dx, dy, dt;
+  unsigned fs;  // bytes between consecutive fields of a field group
+};
+
+// Field groups: each group is one allocation holding its fields back to back,
+// field k at byte k * Geom::fs (fs is a multiple of 256 and every group stays
+// below 4 GiB, so a cell's byte offset in a field fits 32 bits). The stencil
+// kernels address them through one buffer resource per group.
+enum WaveField { F_VZ, F_VX, F_VY, F_SZZ, F_SXX, F_SXZ, F_SYY, F_SXY, F_SYZ, F_WAVE_COUNT };
+enum PsiField {
+  P_VZ_Z, P_VZ_X, P_VZ_Y, P_VX_Z, P_VX_X, P_VX_Y, P_VY_Z, P_VY_X, P_VY_Y,
+  P_SZZ_Z, P_SXZ_X, P_SXZ_Z, P_SXX_X, P_SXY_X, P_SXY_Y, P_SYY_Y, P_SYZ_Y, P_SYZ_Z, P_PSI_COUNT
+};
+enum ModelField { M_LAM, M_MU, M_BYC, M_GLAM, M_GMU, M_GRHO, M_COUNT };
+struct Groups {
+  float* w;  // F_WAVE_COUNT fields: forward (or replayed) wavefield
+  float* a;  // F_WAVE_COUNT fields: adjoint wavefield
+  float* p;  // P_PSI_COUNT fields: C-PML memory variables
+  float* m;  // M_COUNT fields: lambda, mu, buoyancy and the three gradients
+};
+
+// C-PML profiles of the three axes (0 = z, 1 = x, 2 = y) in one array: axis
+// `ax` holds kProfRows rows of n[ax] values starting at off[ax]: K, a, b on
+// the integer grid, then Kh, ah, bh on the half grid.
+enum ProfRow { R_K, R_A, R_B, R_KH, R_AH, R_BH, kProfRows };
+struct Prof {
+  const float* p;
+  int off[3], n[3];
 };
 
 // The nine wavefield components (velocity + stress). Used for the forward
@@ -41,12 +68,6 @@ struct Model {
 };
 struct Grad {
   float *glam, *gmu, *grho;
-};
-
-// 1-D C-PML profiles per axis (0 = z, 1 = x, 2 = y): K, a, b on the integer
-// grid and on the half grid.
-struct Cpml {
-  const float *K[3], *a[3], *b[3], *Kh[3], *ah[3], *bh[3];
 };
 
 // Source injected at one cell with amplitude stf_tapered[it] (device array).

@@ -196,6 +196,16 @@ static int grow(Ctx* c, void** p, size_t* cap, size_t need) {
   return rc;
 }
 
+// Field pointers of a wave group / psi group (fields fsf floats apart).
+static void wave_of(float* base, size_t fsf, Wave& W) {
+  float** f[9] = {&W.vz, &W.vx, &W.vy, &W.szz, &W.sxx, &W.sxz, &W.syy, &W.sxy, &W.syz};
+  for (int k = 0; k < F_WAVE_COUNT; k++) *f[k] = base + k * fsf;
+}
+static void psi_of(float* base, size_t fsf, Psi& P) {
+  float** f = reinterpret_cast<float**>(&P);
+  for (int k = 0; k < P_PSI_COUNT; k++) f[k] = base + k * fsf;
+}
+
 int Ctx::init() {
   const efwi_config& c = cfg;
   if (c.nz < 8 || c.nx < 8 || c.ny < 8 || c.npml < 2 || c.nt < 2)
@@ -214,43 +224,47 @@ int Ctx::init() {
   G.pxy = (long)c.nx * c.ny;
   G.dz = c.dz; G.dx = c.dx; G.dy = c.dy; G.dt = c.dt;
   N = (long)c.nz * G.pxy;
-  const size_t fb = (size_t)N * sizeof(float);
-  float** wf[9] = {&W.vz, &W.vx, &W.vy, &W.szz, &W.sxx, &W.sxz, &W.syy, &W.sxy, &W.syz};
-  float** af[9] = {&A.vz, &A.vx, &A.vy, &A.szz, &A.sxx, &A.sxz, &A.syy, &A.sxy, &A.syz};
-  for (int k = 0; k < 9; k++) {
-    int rc = alloc((void**)wf[k], fb);
-    if (rc) return rc;
-    rc = alloc((void**)af[k], fb);
-    if (rc) return rc;
-  }
-  float** pf[18] = {&P.vz_z, &P.vz_x, &P.vz_y, &P.vx_z, &P.vx_x, &P.vx_y, &P.vy_z, &P.vy_x, &P.vy_y,
-                    &P.szz_z, &P.sxz_x, &P.sxz_z, &P.sxx_x, &P.sxy_x, &P.sxy_y, &P.syy_y, &P.syz_y,
-                    &P.syz_z};
-  for (int k = 0; k < 18; k++) {
-    int rc = alloc((void**)pf[k], fb);
-    if (rc) return rc;
-  }
+  // field groups: fields fs bytes apart (256-byte aligned), each group below 4 GiB
+  const size_t fs = ((size_t)N * sizeof(float) + 255) / 256 * 256;
+  if (fs * P_PSI_COUNT > 0xFFFFFFFFull)
+    return fail(EFWI_ERR_ARG, "grid of " + std::to_string(N) + " cells exceeds the 32-bit field groups (at most " +
+                                  std::to_string(0xFFFFFFFFull / P_PSI_COUNT / 4) + " cells)");
+  G.fs = (unsigned)fs;
+  const size_t fsf = fs / sizeof(float);
   int rc;
-  if ((rc = alloc((void**)&lam, fb)) || (rc = alloc((void**)&mu, fb)) || (rc = alloc((void**)&byc, fb)) ||
-      (rc = alloc((void**)&Gr.glam, fb)) || (rc = alloc((void**)&Gr.gmu, fb)) ||
-      (rc = alloc((void**)&Gr.grho, fb)))
+  if ((rc = alloc((void**)&F.w, fs * F_WAVE_COUNT)) || (rc = alloc((void**)&F.a, fs * F_WAVE_COUNT)) ||
+      (rc = alloc((void**)&F.p, fs * P_PSI_COUNT)) || (rc = alloc((void**)&F.m, fs * M_COUNT)))
     return rc;
+  wave_of(F.w, fsf, W);
+  wave_of(F.a, fsf, A);
+  psi_of(F.p, fsf, P);
+  lam = F.m + M_LAM * fsf; mu = F.m + M_MU * fsf; byc = F.m + M_BYC * fsf;
+  Gr.glam = F.m + M_GLAM * fsf; Gr.gmu = F.m + M_GMU * fsf; Gr.grho = F.m + M_GRHO * fsf;
   M.lam = lam; M.mu = mu; M.byc = byc;
-  // C-PML profiles: 3 axes x 6 rows.
+  // C-PML profiles: 3 axes x kProfRows rows (K, a, b, Kh, ah, bh)
   const int n3[3] = {c.nz, c.nx, c.ny};
   const float h3[3] = {c.dz, c.dx, c.dy};
-  const int nmax = std::max(c.nz, std::max(c.nx, c.ny));
-  float* prof = nullptr;
-  if ((rc = alloc((void**)&prof, sizeof(float) * 18 * nmax))) return rc;
-  std::vector<float> host(18 * (size_t)nmax, 0.0f);
+  std::vector<float> host;
   for (int ax = 0; ax < 3; ax++) {
-    float* base = host.data() + (size_t)ax * 6 * nmax;
-    cpml_profile_1d(n3[ax], c.npml, h3[ax], c.f0, c.dt, base, base + nmax, base + 2 * nmax,
-                    base + 3 * nmax, base + 4 * nmax, base + 5 * nmax);
-    const float* d = prof + (size_t)ax * 6 * nmax;
-    C.K[ax] = d; C.a[ax] = d + nmax; C.b[ax] = d + 2 * nmax;
-    C.Kh[ax] = d + 3 * nmax; C.ah[ax] = d + 4 * nmax; C.bh[ax] = d + 5 * nmax;
+    const int n = n3[ax];
+    C.off[ax] = (int)host.size();
+    C.n[ax] = n;
+    host.resize(host.size() + (size_t)kProfRows * n);
+    float* r = host.data() + C.off[ax];
+    cpml_profile_1d(n, c.npml, h3[ax], c.f0, c.dt, r + R_K * n, r + R_A * n, r + R_B * n, r + R_KH * n,
+                    r + R_AH * n, r + R_BH * n);
+    for (int i = 0; i < n; i++) {
+      const float K = r[R_K * n + i], Kh = r[R_KH * n + i];
+      // the backward kernels drop the C-PML terms of rows interior along x and z
+      // (and of wavefronts clear of the y slabs): there a = 0 and K = 1 exactly
+      const bool interior = i >= c.npml && i <= n - c.npml - 1;
+      if (interior && (r[R_A * n + i] != 0.0f || r[R_AH * n + i] != 0.0f || K != 1.0f || Kh != 1.0f))
+        return fail(EFWI_ERR_ARG, "C-PML profile is not trivial inside the PML-free interior");
+    }
   }
+  float* prof = nullptr;
+  if ((rc = alloc((void**)&prof, sizeof(float) * host.size()))) return rc;
+  C.p = prof;
   HIP_TRY(hipMemcpy(prof, host.data(), sizeof(float) * host.size(), hipMemcpyHostToDevice));
   // boundary band of the reverse reconstruction (Src/Boundary.cu:14-20)
   for (int ax = 0; ax < 3; ax++) {
@@ -362,13 +376,13 @@ int Ctx::ensure_checkpoints() {
   ckpt_floats = per;
   const size_t need = per * (size_t)nseg * sizeof(float);
   int rc;
-  if (!W2.vz) {
-    float** wf[9] = {&W2.vz, &W2.vx, &W2.vy, &W2.szz, &W2.sxx, &W2.sxz, &W2.syy, &W2.sxy, &W2.syz};
-    for (int k = 0; k < 9; k++)
-      if ((rc = alloc((void**)wf[k], (size_t)N * sizeof(float)))) return rc;
-    float** pf = reinterpret_cast<float**>(&P2);
-    for (int k = 0; k < 18; k++)
-      if ((rc = alloc((void**)&pf[k], (size_t)N * sizeof(float)))) return rc;
+  if (!F2.w) {
+    const size_t fs = G.fs, fsf = fs / sizeof(float);
+    if ((rc = alloc((void**)&F2.w, fs * F_WAVE_COUNT)) || (rc = alloc((void**)&F2.p, fs * P_PSI_COUNT))) return rc;
+    F2.a = F.a;
+    F2.m = F.m;
+    wave_of(F2.w, fsf, W2);
+    psi_of(F2.p, fsf, P2);
     // psi outside its slabs is never written: zero it once
     zero_psi(P2, N, nullptr);
     HIP_TRY(hipDeviceSynchronize());
@@ -510,13 +524,13 @@ int Ctx::prepare_shot(const efwi_shot* sh, int calc_id) {
 
 // One forward time step on (Wf, Pf): stress + source, velocity, and (record)
 // the receiver gather of step it + 1.
-void Ctx::forward_step(const Wave& Wf, const Psi& Pf, const Source& Sf, int it, bool record, hipStream_t s) {
+void Ctx::forward_step(const Groups& Ff, const Wave& Wf, const Source& Sf, int it, bool record, hipStream_t s) {
   prof.begin(K_STRESS_FWD, s);
-  launch_stress_fwd(tiles[K_STRESS_FWD], G, Wf, Pf, M, C, Sf, it, s);
+  launch_stress_fwd(tiles[K_STRESS_FWD], G, Ff, C, Sf, it, s);
   prof.end(s);
   if (!src_stencil) launch_add_source(Wf.szz, Wf.sxx, Wf.syy, src_cell, stf_dev, it, G.dt, +1, s);
   prof.begin(K_VELOCITY_FWD, s);
-  launch_velocity_fwd(tiles[K_VELOCITY_FWD], cfg.formulation, G, Wf, Pf, M, C, s);
+  launch_velocity_fwd(tiles[K_VELOCITY_FWD], cfg.formulation, G, Ff, C, s);
   prof.end(s);
   if (record) launch_record(Wf, rec_dev, nrec, it + 1, syn_t, s);
 }
@@ -534,8 +548,8 @@ int Ctx::run_shot(const efwi_shot* sh, int calc_id, const float* obs, float* syn
   const bool grad = calc_id == EFWI_CALC_GRADIENT;
   if (grad && (rc = ensure_ring())) return rc;
   const int nt = cfg.nt, form = cfg.formulation;
-  Fields9 F;
-  fields9(W, F);
+  Fields9 F9;
+  fields9(W, F9);
 
   zero_wave(W, N, s);
   zero_wave(A, N, s);
@@ -554,10 +568,10 @@ int Ctx::run_shot(const efwi_shot* sh, int calc_id, const float* obs, float* syn
     // the ring slots of the last segment are filled by this pass directly
     if (grad && seg == last_seg) {
       prof.begin(K_RING, s);
-      launch_ring(B, G, F, ring, it - seg * seg_len, 0, 9, 0, s);
+      launch_ring(B, G, F9, ring, it - seg * seg_len, 0, 9, 0, s);
       prof.end(s);
     }
-    forward_step(W, P, Sf, it, true, s);
+    forward_step(F, W, Sf, it, true, s);
   }
   HIP_TRY(hipGetLastError());
   if (syn && nrec > 0) launch_from_time_major(syn_t, syn, nrec, nt, s);
@@ -588,11 +602,11 @@ int Ctx::run_shot(const efwi_shot* sh, int calc_id, const float* obs, float* syn
     Source S;
     S.cell = src_interior ? src_cell : -1;
     S.stf = stf_dev;
-    launch_velocity_adj(form, G, A, P, M, C, s);
+    launch_velocity_adj(form, G, F, C, s);
     launch_inject(A, rec_dev, nrec, nt - 1, res_t, s);
-    launch_stress_adj(G, A, P, M, C, s);
-    Fields9 F2;
-    fields9(W2, F2);
+    launch_stress_adj(G, F, C, s);
+    Fields9 F9r;
+    fields9(W2, F9r);
     for (int seg = last_seg; seg >= 0; seg--) {
       const int a = seg * seg_len, b = std::min(nt - 2, a + seg_len - 1);
       if (seg < last_seg) {
@@ -600,23 +614,23 @@ int Ctx::run_shot(const efwi_shot* sh, int calc_id, const float* obs, float* syn
         if ((rc = checkpoint(seg, W2, P2, false, s))) return rc;
         for (int it = a; it <= b; it++) {
           prof.begin(K_RING, s);
-          launch_ring(B, G, F2, ring, it - a, 0, 9, 0, s);
+          launch_ring(B, G, F9r, ring, it - a, 0, 9, 0, s);
           prof.end(s);
-          forward_step(W2, P2, Sf, it, false, s);
+          forward_step(F2, W2, Sf, it, false, s);
         }
       }
       for (int it = b; it >= a; it--) {
         const long slot = it - a;
         prof.begin(K_VELOCITY_BWD, s);
-        launch_velocity_bwd(tiles[K_VELOCITY_BWD], form, G, W, A, P, M, C, Gr, s);
+        launch_velocity_bwd(tiles[K_VELOCITY_BWD], form, G, F, C, s);
         prof.end(s);
-        launch_ring(B, G, F, ring, slot, 0, 3, 1, s);
+        launch_ring(B, G, F9, ring, slot, 0, 3, 1, s);
         if (!src_interior) launch_add_source(W.szz, W.sxx, W.syy, src_cell, stf_dev, it, G.dt, -1, s);
         launch_inject(A, rec_dev, nrec, it, res_t, s);
         prof.begin(K_STRESS_BWD, s);
-        launch_stress_bwd(tiles[K_STRESS_BWD], G, W, A, P, M, C, Gr, S, it, s);
+        launch_stress_bwd(tiles[K_STRESS_BWD], G, F, C, S, it, s);
         prof.end(s);
-        launch_ring(B, G, F, ring, slot, 3, 9, 1, s);
+        launch_ring(B, G, F9, ring, slot, 3, 9, 1, s);
       }
     }
     HIP_TRY(hipGetLastError());
@@ -653,10 +667,10 @@ int Ctx::autotune(int reps, hipStream_t s, double* report) {
     for (int c = 0; c < ncand; c++) {
       const Tile t = cands[c];
       auto launch = [&]() {
-        if (k == K_STRESS_FWD) launch_stress_fwd(t, G, W, P, M, C, none, 0, s);
-        else if (k == K_VELOCITY_FWD) launch_velocity_fwd(t, cfg.formulation, G, W, P, M, C, s);
-        else if (k == K_VELOCITY_BWD) launch_velocity_bwd(t, cfg.formulation, G, W, A, P, M, C, Gr, s);
-        else launch_stress_bwd(t, G, W, A, P, M, C, Gr, none, 0, s);
+        if (k == K_STRESS_FWD) launch_stress_fwd(t, G, F, C, none, 0, s);
+        else if (k == K_VELOCITY_FWD) launch_velocity_fwd(t, cfg.formulation, G, F, C, s);
+        else if (k == K_VELOCITY_BWD) launch_velocity_bwd(t, cfg.formulation, G, F, C, s);
+        else launch_stress_bwd(t, G, F, C, none, 0, s);
       };
       launch();  // warm-up
       HIP_TRY(hipEventRecord(e0, s));

@@ -47,7 +47,9 @@ struct Ctx {
   Psi P{};
   Model M{};
   Grad Gr{};
-  Cpml C{};
+  Prof C{};
+  Groups F{};   // field groups of the state; F2: replay copies of w and p (memory_mode 1)
+  Groups F2{};
   Band B{};
   float *lam = nullptr, *mu = nullptr, *byc = nullptr;
   bool model_set = false;
@@ -91,7 +93,7 @@ struct Ctx {
   int ensure_checkpoints();
   size_t slab_floats(int axis) const;
   int checkpoint(int k, const Wave& Wf, const Psi& Pf, bool save, hipStream_t s);
-  void forward_step(const Wave& Wf, const Psi& Pf, const Source& Sf, int it, bool record, hipStream_t s);
+  void forward_step(const Groups& Ff, const Wave& Wf, const Source& Sf, int it, bool record, hipStream_t s);
   int set_model(const float* l, const float* m, const float* rho, int on_host, hipStream_t s);
   int zero_gradient(hipStream_t s);
   int prepare_shot(const efwi_shot* sh, int calc_id);

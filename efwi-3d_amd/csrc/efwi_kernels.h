@@ -35,19 +35,18 @@ struct Sweep {
 };
 Sweep make_sweep(int z0, int z1, int x0, int x1, int y0, int y1, Tile t);
 
-// stencil kernels (efwi_stencil.hip)
-void launch_stress_fwd(Tile t, const Geom& G, const Wave& W, const Psi& P, const Model& M, const Cpml& C,
-                       const Source& S, int it, hipStream_t s);
-void launch_velocity_fwd(Tile t, int form, const Geom& G, const Wave& W, const Psi& P, const Model& M,
-                         const Cpml& C, hipStream_t s);
-void launch_velocity_bwd(Tile t, int form, const Geom& G, const Wave& W, const Wave& A, const Psi& P,
-                         const Model& M, const Cpml& C, const Grad& Gr, hipStream_t s);
-void launch_stress_bwd(Tile t, const Geom& G, const Wave& W, const Wave& A, const Psi& P, const Model& M,
-                       const Cpml& C, const Grad& Gr, const Source& S, int it, hipStream_t s);
-void launch_velocity_adj(int form, const Geom& G, const Wave& A, const Psi& P, const Model& M,
-                         const Cpml& C, hipStream_t s);
-void launch_stress_adj(const Geom& G, const Wave& A, const Psi& P, const Model& M, const Cpml& C,
+// stencil kernels (efwi_stencil.hip). F.w is the wavefield a forward kernel
+// advances (the forward state or its replay copy); the backward kernels
+// reverse F.w and advance the adjoint F.a.
+void launch_stress_fwd(Tile t, const Geom& G, const Groups& F, const Prof& C, const Source& S, int it,
                        hipStream_t s);
+void launch_velocity_fwd(Tile t, int form, const Geom& G, const Groups& F, const Prof& C, hipStream_t s);
+void launch_velocity_bwd(Tile t, int form, const Geom& G, const Groups& F, const Prof& C, hipStream_t s);
+void launch_stress_bwd(Tile t, const Geom& G, const Groups& F, const Prof& C, const Source& S, int it,
+                       hipStream_t s);
+// adjoint-only steps (no reverse reconstruction): the first backward step
+void launch_velocity_adj(int form, const Geom& G, const Groups& F, const Prof& C, hipStream_t s);
+void launch_stress_adj(const Geom& G, const Groups& F, const Prof& C, hipStream_t s);
 
 // auxiliary kernels (efwi_aux.hip)
 void launch_fill(float* p, long n, float v, hipStream_t s);

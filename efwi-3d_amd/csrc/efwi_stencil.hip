@@ -3,25 +3,44 @@
 //
 // Arithmetic: every cell evaluates the same expression tree as the reference
 // kernel it replaces, with IEEE division and no fused multiply-add (this file
-// is compiled with -ffp-contract=off), so the float results are the correctly
-// rounded results of the reference's float expressions. The only liberty
-// taken is to skip a C-PML term a*d(psi) where the profile coefficient a is
-// exactly zero (the reference multiplies by zero there), which can change the
-// sign of an exact zero and nothing else.
+// is compiled with -ffp-contract=off), so every float result is the
+// correctly rounded result of the reference's float expression. The only
+// liberties taken: a C-PML term a * d(psi) is skipped where the profile
+// coefficient a is exactly zero, and a division by K where K is exactly one
+// (the reference multiplies by zero or divides by one there); both can
+// change the sign of an exact zero and nothing else.
 //
-// Work distribution ("sweep"): a 1-D grid of one-wavefront workgroups, each a
-// 64-cell segment of one y-row. Workgroup b runs on XCD b % 8 (round-robin
-// dispatch); the kernels map b so that each XCD owns a contiguous strip of x
-// rows and walks it plane by plane in z. The x and z neighbours a cell reads
-// were then fetched into that XCD's L2 by the segments just before it, so
-// every field is read from HBM about once per launch.
+// Addressing: the fields live in four groups (forward wavefield, adjoint
+// wavefield, the 18 psi arrays, and lambda/mu/buoyancy/gradients), each one
+// allocation with its fields fs bytes apart. Every group is one buffer
+// resource: a field access is a buffer_load/store_dword with the field's
+// byte offset in the scalar offset, the cell's (or a neighbour's) byte offset
+// in a VGPR, and y neighbours folded into the instruction offset (the
+// compiler merges a row's y neighbours into wider loads). No 64-bit address
+// arithmetic is left per load. The x and z C-PML profile values are uniform
+// across a wavefront (a wavefront is one 64-cell segment of one x row) and
+// come through scalar loads.
+//
+// Work distribution ("sweep"): workgroups of 64 x R threads (R rows of 64
+// y-consecutive cells), numbered b = 0, 1, ...; workgroup b runs on XCD
+// b % 8 (round-robin dispatch), and the kernels map b so that each XCD owns a
+// contiguous strip of x rows and walks it plane chunk by plane chunk in z.
+// The x and z neighbours a cell reads were then fetched into that XCD's L2 by
+// the workgroups just before it.
+//
+// Per-wavefront specialisation of the backward kernels: a wavefront whose
+// row lies in the x or z C-PML runs only the adjoint part (the reverse part
+// covers the interior only), and one whose row is interior along x and z
+// drops every x/z C-PML term (there a = 0 and K = 1 exactly, on both grids);
+// the y C-PML terms are compiled in only for wavefronts that touch a y slab.
 //
 // Fusion: the source injection is folded into the forward stress kernel; the
 // backward pass runs two kernels per step, {reverse velocity + density
 // imaging + adjoint velocity} and {source subtraction + reverse stress +
 // lambda/mu imaging + adjoint stress}. Each cell reads its own adjoint field
 // before updating it, as the reference's separate launches do, and no kernel
-// reads a neighbour value that it writes.
+// reads a neighbour value that it writes. Each cell function issues all its
+// loads before its first store.
 //
 // FORM = 0: reference formulation (typo in dsxy/dx, Src/velocity.cu:79;
 //           off-by-one x/y C-PML predicate, Src/velocity.cu:46-90; swapped
@@ -33,35 +52,104 @@
 namespace efwi {
 namespace {
 
+// Optional occupancy targets (waves per SIMD) of the forward and backward
+// kernels, which bound their VGPR budgets: a build-time tuning knob.
+#ifdef EFWI_FWD_WAVES
+#define EFWI_FWD_ATTR __attribute__((amdgpu_waves_per_eu(EFWI_FWD_WAVES)))
+#else
+#define EFWI_FWD_ATTR
+#endif
+#ifdef EFWI_BWD_WAVES
+#define EFWI_BWD_ATTR __attribute__((amdgpu_waves_per_eu(EFWI_BWD_WAVES)))
+#else
+#define EFWI_BWD_ATTR
+#endif
+
 constexpr float C1 = (float)(9.0 / 8.0);   // Src/stress.cu:33
 constexpr float C2 = (float)(1.0 / 24.0);  // Src/stress.cu:34
 
-// Backward / forward 4th-order staggered differences (Src/stress.cu:40-68).
-__device__ __forceinline__ float Db(const float* __restrict__ f, long i, long s, float h) {
-  return (C1 * (f[i] - f[i - s]) - C2 * (f[i + s] - f[i - 2 * s])) / h;
+// Word 3 of a gfx9 buffer resource for raw dword access (32-bit data format).
+constexpr int kRsrcWord3 = 0x00020000;
+
+struct Grp {
+  __amdgpu_buffer_rsrc_t r;
+  unsigned fs;
+};
+
+__device__ __forceinline__ Grp group(float* base, int nfields, unsigned fs) {
+  return Grp{__builtin_amdgcn_make_buffer_rsrc(base, (short)0, (int)((unsigned)nfields * fs), kRsrcWord3), fs};
 }
-__device__ __forceinline__ float Df(const float* __restrict__ f, long i, long s, float h) {
-  return (C1 * (f[i + s] - f[i]) - C2 * (f[i + 2 * s] - f[i - s])) / h;
+__device__ __forceinline__ float ld(const Grp& g, int k, unsigned o) {
+  return __builtin_bit_cast(float, __builtin_amdgcn_raw_buffer_load_b32(g.r, (int)o, (int)(k * g.fs), 0));
+}
+__device__ __forceinline__ void st(const Grp& g, int k, unsigned o, float v) {
+  __builtin_amdgcn_raw_buffer_store_b32(__builtin_bit_cast(unsigned, v), g.r, (int)o, (int)(k * g.fs), 0);
+}
+
+// Byte offsets of the neighbours at -2, -1, +1, +2 along one axis.
+struct Nb {
+  unsigned m2, m1, p1, p2;
+};
+
+struct Cell {
+  int z, x, y;  // z and x are wavefront-uniform
+  unsigned o;   // byte offset of the cell in a field
+  Nb zn, xn, yn;
+};
+
+// Difference numerators of the 4th-order staggered operators; the caller
+// divides by the grid step.
+// Backward / forward, Src/stress.cu:40-68.
+__device__ __forceinline__ float n_b(const Grp& g, int k, unsigned o, const Nb& n) {
+  return C1 * (ld(g, k, o) - ld(g, k, n.m1)) - C2 * (ld(g, k, n.p1) - ld(g, k, n.m2));
+}
+__device__ __forceinline__ float n_f(const Grp& g, int k, unsigned o, const Nb& n) {
+  return C1 * (ld(g, k, n.p1) - ld(g, k, o)) - C2 * (ld(g, k, n.p2) - ld(g, k, n.m1));
 }
 // Adjoint forms as written in Src/stress_adj.cu:54-59.
-__device__ __forceinline__ float Af(const float* __restrict__ f, long i, long s, float h) {
-  return (-C1 * (f[i + s] - f[i]) + C2 * (f[i + 2 * s] - f[i - s])) / h;
+__device__ __forceinline__ float n_af(const Grp& g, int k, unsigned o, const Nb& n) {
+  return -C1 * (ld(g, k, n.p1) - ld(g, k, o)) + C2 * (ld(g, k, n.p2) - ld(g, k, n.m1));
 }
-__device__ __forceinline__ float Ab(const float* __restrict__ f, long i, long s, float h) {
-  return (-C1 * (f[i] - f[i - s]) + C2 * (f[i + s] - f[i - 2 * s])) / h;
+__device__ __forceinline__ float n_ab(const Grp& g, int k, unsigned o, const Nb& n) {
+  return -C1 * (ld(g, k, o) - ld(g, k, n.m1)) + C2 * (ld(g, k, n.p1) - ld(g, k, n.m2));
 }
 // dsxy/dx of the vy update (Src/velocity.cu:79 and :122).
 template <int FORM>
-__device__ __forceinline__ float Dsxy_x(const float* __restrict__ f, long i, long s, float h) {
-  if (FORM == 1) return Db(f, i, s, h);
-  return (C1 * (f[i] - f[i - s]) - C2 * (f[i - s] - f[i - 2 * s])) / h;
+__device__ __forceinline__ float n_sxy_x(const Grp& g, int k, unsigned o, const Nb& n) {
+  if (FORM == 1) return n_b(g, k, o, n);
+  return C1 * (ld(g, k, o) - ld(g, k, n.m1)) - C2 * (ld(g, k, n.m1) - ld(g, k, n.m2));
 }
-// a * d(psi), skipped where a == 0 (outside the PML along that axis).
-__device__ __forceinline__ float apsi_f(float a, const float* __restrict__ f, long i, long s, float h) {
-  return a != 0.0f ? a * Af(f, i, s, h) : 0.0f;
+
+// Quotients by the grid steps (IEEE division, as the reference's).
+struct Dv {
+  const Geom& G;
+  __device__ __forceinline__ explicit Dv(const Geom& g) : G(g) {}
+  __device__ __forceinline__ float z(float n) const { return n / G.dz; }
+  __device__ __forceinline__ float x(float n) const { return n / G.dx; }
+  __device__ __forceinline__ float y(float n) const { return n / G.dy; }
+};
+
+// C-PML profile value of axis `ax`, row `row`, index i.
+__device__ __forceinline__ float pr(const Prof& C, int ax, int row, int i) {
+  return C.p[C.off[ax] + row * C.n[ax] + i];
 }
-__device__ __forceinline__ float apsi_b(float a, const float* __restrict__ f, long i, long s, float h) {
-  return a != 0.0f ? a * Ab(f, i, s, h) : 0.0f;
+
+// a * d(psi)/dh of the adjoint kernels, skipped where a == 0 (the reference
+// multiplies by zero there); ON = false where a is known to be zero.
+template <bool ON>
+__device__ __forceinline__ float apsi_f(float a, const Grp& P, int k, const Cell& c, const Nb& n, float h) {
+  if (!ON) return 0.0f;
+  return a != 0.0f ? a * (n_af(P, k, c.o, n) / h) : 0.0f;
+}
+template <bool ON>
+__device__ __forceinline__ float apsi_b(float a, const Grp& P, int k, const Cell& c, const Nb& n, float h) {
+  if (!ON) return 0.0f;
+  return a != 0.0f ? a * (n_ab(P, k, c.o, n) / h) : 0.0f;
+}
+// v / K; ON = false where K is known to be one.
+template <bool ON>
+__device__ __forceinline__ float kdiv(float v, float K) {
+  return ON ? v / K : v;
 }
 
 __device__ __forceinline__ bool in_pml(int i, int n, int p) { return i < p || i > n - p - 1; }
@@ -72,22 +160,25 @@ __device__ __forceinline__ bool in_pml_v(int i, int n, int p) {
   return i < p || i > n - p;
 }
 
-// psi <- b psi + a d ; d <- d / K + psi
-__device__ __forceinline__ float cpml(float* __restrict__ m, long i, float b, float a, float K, float d) {
-  float ps = b * m[i] + a * d;
-  m[i] = ps;
-  return d / K + ps;
+// psi <- b psi + a d (in place) ; returns d / K + psi  (Src/stress.cu:71-99)
+__device__ __forceinline__ float cpml(float b, float a, float K, float& psi, float d) {
+  psi = b * psi + a * d;
+  return d / K + psi;
 }
 
-struct Cell {
-  int z, x, y;
-  long i;
+// Axis profile values used by one C-PML correction.
+struct Pv {
+  float b, a, K;
 };
+__device__ __forceinline__ Pv pv(const Prof& C, int ax, bool half, int i) {
+  return half ? Pv{pr(C, ax, R_BH, i), pr(C, ax, R_AH, i), pr(C, ax, R_KH, i)}
+              : Pv{pr(C, ax, R_B, i), pr(C, ax, R_A, i), pr(C, ax, R_K, i)};
+}
 
 // Workgroup b (64 x R threads: y lanes x rows) covers R rows of the strip of
 // XCD b % 8 and Z consecutive planes; the strip is walked plane chunk by
 // plane chunk (z slowest), so each XCD's L2 sees a compact, z-ordered region.
-template <int Z, typename Body>
+template <typename Body>
 __device__ __forceinline__ void sweep(const Sweep& S, const Geom& G, Body&& body) {
   const int b = blockIdx.x;
   const int xcd = b & 7, j = b >> 3;
@@ -97,123 +188,147 @@ __device__ __forceinline__ void sweep(const Sweep& S, const Geom& G, Body&& body
   const int rg = r / S.nyb;
   const int yb = r - rg * S.nyb;
   const int rows = S.rows_base + (xcd < S.rows_rem ? 1 : 0);
-  const int row = rg * S.R + (int)threadIdx.y;
+  // blockDim.x == 64: threadIdx.y is uniform across a wavefront
+  const int row = __builtin_amdgcn_readfirstlane(rg * S.R + (int)threadIdx.y);
   if (row >= rows) return;
   Cell c;
   c.x = S.x0 + xcd * S.rows_base + min(xcd, S.rows_rem) + row;
   c.y = yb * 64 + (int)threadIdx.x;
   if (c.y < S.y0 || c.y > S.y1) return;
-  const int za = S.z0 + zc * Z;
-  const long base = (long)c.x * G.py + c.y;
-  if (Z == 1) {
-    c.z = za;
-    c.i = (long)za * G.pxy + base;
+  const unsigned sx = 4u * (unsigned)G.py, sz = 4u * (unsigned)G.pxy;
+  const int za = S.z0 + zc * S.Z;
+  const int zb = min(za + S.Z, S.z0 + S.nz);
+  // byte offset of the cell's y-2 neighbour (y >= 2): the cell and its y
+  // neighbours are this plus 0..16, which the compiler folds into the
+  // instruction offsets (and merges into one dwordx4 + one dword per field)
+  unsigned oy = 4u * ((unsigned)za * (unsigned)G.pxy + (unsigned)c.x * (unsigned)G.py + (unsigned)c.y) - 8u;
+#pragma unroll 1
+  for (int z = za; z < zb; z++) {
+    const unsigned o = oy + 8u;
+    c.z = z;
+    c.o = o;
+    c.zn = Nb{o - 2 * sz, o - sz, o + sz, o + 2 * sz};
+    c.xn = Nb{o - 2 * sx, o - sx, o + sx, o + 2 * sx};
+    c.yn = Nb{oy, oy + 4u, oy + 12u, oy + 16u};
     body(c);
-  } else {
-    const int zb = min(za + Z, S.z0 + S.nz);
-    for (int z = za; z < zb; z++) {
-      c.z = z;
-      c.i = (long)z * G.pxy + base;
-      body(c);
-    }
+    oy += sz;
   }
 }
 
-__device__ __forceinline__ bool interior(const Geom& G, const Cell& c) {
-  const int p = G.npml;
-  return c.z >= p && c.z <= G.nz - 1 - p && c.x >= p && c.x <= G.nx - 1 - p && c.y >= p &&
-         c.y <= G.ny - 1 - p;
-}
-
-// The cell functions below are written in three phases: every global load of
-// the cell (neighbours and own values), then the arithmetic, then every store.
-// Loads never follow a store in program order, so a wavefront issues all of
-// its loads at once and waits once. Each stored value is still the same
-// expression tree as the reference's, so the phase order changes no bit.
-
 // ---------------------------------------------------------------------------
-// Forward stress of one cell, Src/stress.cu:36-101.
+// Forward stress of one cell, Src/stress.cu:36-101, plus the source of this
+// step at its cell (Src/libCUFD.cu:319-330).
 // ---------------------------------------------------------------------------
-__device__ __forceinline__ void stress_fwd_cell(const Geom& G, const Wave& W, const Psi& P, const Model& M,
-                                                const Cpml& C, const Cell& c) {
-  const long i = c.i, SZ = G.pxy, SX = G.py;
+__device__ __forceinline__ void stress_fwd_cell(const Geom& G, const Grp& W, const Grp& P, const Grp& M,
+                                                const Prof& C, const Cell& c, unsigned src_o, float amp) {
+  const unsigned o = c.o;
   const int z = c.z, x = c.x, y = c.y, p = G.npml;
-  float dvz_dz = Db(W.vz, i, SZ, G.dz);
-  float dvx_dx = Db(W.vx, i, SX, G.dx);
-  float dvy_dy = Db(W.vy, i, 1, G.dy);
-  float dvx_dz = Df(W.vx, i, SZ, G.dz);
-  float dvx_dy = Df(W.vx, i, 1, G.dy);
-  float dvz_dx = Df(W.vz, i, SX, G.dx);
-  float dvz_dy = Df(W.vz, i, 1, G.dy);
-  float dvy_dx = Df(W.vy, i, SX, G.dx);
-  float dvy_dz = Df(W.vy, i, SZ, G.dz);
-  const float lam = M.lam[i], mu = M.mu[i], dt = G.dt;
-  const float szz = W.szz[i], sxx = W.sxx[i], syy = W.syy[i];
-  const float sxz = W.sxz[i], sxy = W.sxy[i], syz = W.syz[i];
+  const Dv D(G);
+  float dvz_dz = D.z(n_b(W, F_VZ, o, c.zn));
+  float dvx_dx = D.x(n_b(W, F_VX, o, c.xn));
+  float dvy_dy = D.y(n_b(W, F_VY, o, c.yn));
+  float dvx_dz = D.z(n_f(W, F_VX, o, c.zn));
+  float dvx_dy = D.y(n_f(W, F_VX, o, c.yn));
+  float dvz_dx = D.x(n_f(W, F_VZ, o, c.xn));
+  float dvz_dy = D.y(n_f(W, F_VZ, o, c.yn));
+  float dvy_dx = D.x(n_f(W, F_VY, o, c.xn));
+  float dvy_dz = D.z(n_f(W, F_VY, o, c.zn));
+  const float lam = ld(M, M_LAM, o), mu = ld(M, M_MU, o), dt = G.dt;
+  const float szz = ld(W, F_SZZ, o), sxx = ld(W, F_SXX, o), syy = ld(W, F_SYY, o);
+  const float sxz = ld(W, F_SXZ, o), sxy = ld(W, F_SXY, o), syz = ld(W, F_SYZ, o);
   const bool pz = in_pml(z, G.nz, p), px = in_pml(x, G.nx, p), py = in_pml(y, G.ny, p);
+  float qz0 = 0, qz1 = 0, qz2 = 0, qx0 = 0, qx1 = 0, qx2 = 0, qy0 = 0, qy1 = 0, qy2 = 0;
+  if (pz) { qz0 = ld(P, P_VZ_Z, o); qz1 = ld(P, P_VX_Z, o); qz2 = ld(P, P_VY_Z, o); }
+  if (px) { qx0 = ld(P, P_VX_X, o); qx1 = ld(P, P_VZ_X, o); qx2 = ld(P, P_VY_X, o); }
+  if (py) { qy0 = ld(P, P_VY_Y, o); qy1 = ld(P, P_VZ_Y, o); qy2 = ld(P, P_VX_Y, o); }
   if (pz) {
-    dvz_dz = cpml(P.vz_z, i, C.b[0][z], C.a[0][z], C.K[0][z], dvz_dz);
-    dvx_dz = cpml(P.vx_z, i, C.bh[0][z], C.ah[0][z], C.Kh[0][z], dvx_dz);
-    dvy_dz = cpml(P.vy_z, i, C.bh[0][z], C.ah[0][z], C.Kh[0][z], dvy_dz);
+    const Pv i = pv(C, 0, false, z), h = pv(C, 0, true, z);
+    dvz_dz = cpml(i.b, i.a, i.K, qz0, dvz_dz);
+    dvx_dz = cpml(h.b, h.a, h.K, qz1, dvx_dz);
+    dvy_dz = cpml(h.b, h.a, h.K, qz2, dvy_dz);
   }
   if (px) {
-    dvx_dx = cpml(P.vx_x, i, C.b[1][x], C.a[1][x], C.K[1][x], dvx_dx);
-    dvz_dx = cpml(P.vz_x, i, C.bh[1][x], C.ah[1][x], C.Kh[1][x], dvz_dx);
-    dvy_dx = cpml(P.vy_x, i, C.bh[1][x], C.ah[1][x], C.Kh[1][x], dvy_dx);
+    const Pv i = pv(C, 1, false, x), h = pv(C, 1, true, x);
+    dvx_dx = cpml(i.b, i.a, i.K, qx0, dvx_dx);
+    dvz_dx = cpml(h.b, h.a, h.K, qx1, dvz_dx);
+    dvy_dx = cpml(h.b, h.a, h.K, qx2, dvy_dx);
   }
   if (py) {
-    dvy_dy = cpml(P.vy_y, i, C.b[2][y], C.a[2][y], C.K[2][y], dvy_dy);
-    dvz_dy = cpml(P.vz_y, i, C.bh[2][y], C.ah[2][y], C.Kh[2][y], dvz_dy);
-    dvx_dy = cpml(P.vx_y, i, C.bh[2][y], C.ah[2][y], C.Kh[2][y], dvx_dy);
+    const Pv i = pv(C, 2, false, y), h = pv(C, 2, true, y);
+    dvy_dy = cpml(i.b, i.a, i.K, qy0, dvy_dy);
+    dvz_dy = cpml(h.b, h.a, h.K, qy1, dvz_dy);
+    dvx_dy = cpml(h.b, h.a, h.K, qy2, dvx_dy);
   }
   const float l2m = lam + 2.0f * mu;
-  W.szz[i] = szz + (l2m * dvz_dz + lam * dvx_dx + lam * dvy_dy) * dt;
-  W.sxx[i] = sxx + (lam * dvz_dz + l2m * dvx_dx + lam * dvy_dy) * dt;
-  W.syy[i] = syy + (lam * dvz_dz + lam * dvx_dx + l2m * dvy_dy) * dt;
-  W.sxz[i] = sxz + mu * (dvx_dz + dvz_dx) * dt;
-  W.sxy[i] = sxy + mu * (dvx_dy + dvy_dx) * dt;
-  W.syz[i] = syz + mu * (dvy_dz + dvz_dy) * dt;
+  float nszz = szz + (l2m * dvz_dz + lam * dvx_dx + lam * dvy_dy) * dt;
+  float nsxx = sxx + (lam * dvz_dz + l2m * dvx_dx + lam * dvy_dy) * dt;
+  float nsyy = syy + (lam * dvz_dz + lam * dvx_dx + l2m * dvy_dy) * dt;
+  if (o == src_o) {
+    nszz = nszz + amp;
+    nsxx = nsxx + amp;
+    nsyy = nsyy + amp;
+  }
+  st(W, F_SZZ, o, nszz);
+  st(W, F_SXX, o, nsxx);
+  st(W, F_SYY, o, nsyy);
+  st(W, F_SXZ, o, sxz + mu * (dvx_dz + dvz_dx) * dt);
+  st(W, F_SXY, o, sxy + mu * (dvx_dy + dvy_dx) * dt);
+  st(W, F_SYZ, o, syz + mu * (dvy_dz + dvz_dy) * dt);
+  if (pz) { st(P, P_VZ_Z, o, qz0); st(P, P_VX_Z, o, qz1); st(P, P_VY_Z, o, qz2); }
+  if (px) { st(P, P_VX_X, o, qx0); st(P, P_VZ_X, o, qx1); st(P, P_VY_X, o, qx2); }
+  if (py) { st(P, P_VY_Y, o, qy0); st(P, P_VZ_Y, o, qy1); st(P, P_VX_Y, o, qy2); }
 }
 
 // ---------------------------------------------------------------------------
 // Forward velocity of one cell, Src/velocity.cu:175-240.
 // ---------------------------------------------------------------------------
 template <int FORM>
-__device__ __forceinline__ void velocity_fwd_cell(const Geom& G, const Wave& W, const Psi& P, const Model& M,
-                                                  const Cpml& C, const Cell& c) {
-  const long i = c.i, SZ = G.pxy, SX = G.py;
+__device__ __forceinline__ void velocity_fwd_cell(const Geom& G, const Grp& W, const Grp& P, const Grp& M,
+                                                  const Prof& C, const Cell& c) {
+  const unsigned o = c.o;
   const int z = c.z, x = c.x, y = c.y, p = G.npml;
-  float dszz_dz = Df(W.szz, i, SZ, G.dz);
-  float dsxz_dx = Db(W.sxz, i, SX, G.dx);
-  float dsyz_dy = Db(W.syz, i, 1, G.dy);
-  float dsxz_dz = Db(W.sxz, i, SZ, G.dz);
-  float dsxx_dx = Df(W.sxx, i, SX, G.dx);
-  float dsxy_dy = Db(W.sxy, i, 1, G.dy);
-  float dsyz_dz = Db(W.syz, i, SZ, G.dz);
-  float dsxy_dx = Dsxy_x<FORM>(W.sxy, i, SX, G.dx);
-  float dsyy_dy = Df(W.syy, i, 1, G.dy);
-  const float byc = M.byc[i], dt = G.dt;
-  const float vz = W.vz[i], vx = W.vx[i], vy = W.vy[i];
+  const Dv D(G);
+  float dszz_dz = D.z(n_f(W, F_SZZ, o, c.zn));
+  float dsxz_dx = D.x(n_b(W, F_SXZ, o, c.xn));
+  float dsyz_dy = D.y(n_b(W, F_SYZ, o, c.yn));
+  float dsxz_dz = D.z(n_b(W, F_SXZ, o, c.zn));
+  float dsxx_dx = D.x(n_f(W, F_SXX, o, c.xn));
+  float dsxy_dy = D.y(n_b(W, F_SXY, o, c.yn));
+  float dsyz_dz = D.z(n_b(W, F_SYZ, o, c.zn));
+  float dsxy_dx = D.x(n_sxy_x<FORM>(W, F_SXY, o, c.xn));
+  float dsyy_dy = D.y(n_f(W, F_SYY, o, c.yn));
+  const float byc = ld(M, M_BYC, o), dt = G.dt;
+  const float vz = ld(W, F_VZ, o), vx = ld(W, F_VX, o), vy = ld(W, F_VY, o);
   const bool pz = in_pml(z, G.nz, p);
   const bool px = in_pml_v<FORM>(x, G.nx, p), py = in_pml_v<FORM>(y, G.ny, p);
+  float qz0 = 0, qz1 = 0, qz2 = 0, qx0 = 0, qx1 = 0, qx2 = 0, qy0 = 0, qy1 = 0, qy2 = 0;
+  if (pz) { qz0 = ld(P, P_SZZ_Z, o); qz1 = ld(P, P_SXZ_Z, o); qz2 = ld(P, P_SYZ_Z, o); }
+  if (px) { qx0 = ld(P, P_SXZ_X, o); qx1 = ld(P, P_SXX_X, o); qx2 = ld(P, P_SXY_X, o); }
+  if (py) { qy0 = ld(P, P_SYZ_Y, o); qy1 = ld(P, P_SXY_Y, o); qy2 = ld(P, P_SYY_Y, o); }
   if (pz) {
-    dszz_dz = cpml(P.szz_z, i, C.bh[0][z], C.ah[0][z], C.Kh[0][z], dszz_dz);
-    dsxz_dz = cpml(P.sxz_z, i, C.b[0][z], C.a[0][z], C.K[0][z], dsxz_dz);
-    dsyz_dz = cpml(P.syz_z, i, C.b[0][z], C.a[0][z], C.K[0][z], dsyz_dz);
+    const Pv i = pv(C, 0, false, z), h = pv(C, 0, true, z);
+    dszz_dz = cpml(h.b, h.a, h.K, qz0, dszz_dz);
+    dsxz_dz = cpml(i.b, i.a, i.K, qz1, dsxz_dz);
+    dsyz_dz = cpml(i.b, i.a, i.K, qz2, dsyz_dz);
   }
   if (px) {
-    dsxz_dx = cpml(P.sxz_x, i, C.b[1][x], C.a[1][x], C.K[1][x], dsxz_dx);
-    dsxx_dx = cpml(P.sxx_x, i, C.bh[1][x], C.ah[1][x], C.Kh[1][x], dsxx_dx);
-    dsxy_dx = cpml(P.sxy_x, i, C.b[1][x], C.a[1][x], C.K[1][x], dsxy_dx);
+    const Pv i = pv(C, 1, false, x), h = pv(C, 1, true, x);
+    dsxz_dx = cpml(i.b, i.a, i.K, qx0, dsxz_dx);
+    dsxx_dx = cpml(h.b, h.a, h.K, qx1, dsxx_dx);
+    dsxy_dx = cpml(i.b, i.a, i.K, qx2, dsxy_dx);
   }
   if (py) {
-    dsyz_dy = cpml(P.syz_y, i, C.b[2][y], C.a[2][y], C.K[2][y], dsyz_dy);
-    dsxy_dy = cpml(P.sxy_y, i, C.b[2][y], C.a[2][y], C.K[2][y], dsxy_dy);
-    dsyy_dy = cpml(P.syy_y, i, C.bh[2][y], C.ah[2][y], C.Kh[2][y], dsyy_dy);
+    const Pv i = pv(C, 2, false, y), h = pv(C, 2, true, y);
+    dsyz_dy = cpml(i.b, i.a, i.K, qy0, dsyz_dy);
+    dsxy_dy = cpml(i.b, i.a, i.K, qy1, dsxy_dy);
+    dsyy_dy = cpml(h.b, h.a, h.K, qy2, dsyy_dy);
   }
-  W.vz[i] = vz + (dszz_dz + dsxz_dx + dsyz_dy) * byc * dt;
-  W.vx[i] = vx + (dsxz_dz + dsxx_dx + dsxy_dy) * byc * dt;
-  W.vy[i] = vy + (dsyz_dz + dsxy_dx + dsyy_dy) * byc * dt;
+  st(W, F_VZ, o, vz + (dszz_dz + dsxz_dx + dsyz_dy) * byc * dt);
+  st(W, F_VX, o, vx + (dsxz_dz + dsxx_dx + dsxy_dy) * byc * dt);
+  st(W, F_VY, o, vy + (dsyz_dz + dsxy_dx + dsyy_dy) * byc * dt);
+  if (pz) { st(P, P_SZZ_Z, o, qz0); st(P, P_SXZ_Z, o, qz1); st(P, P_SYZ_Z, o, qz2); }
+  if (px) { st(P, P_SXZ_X, o, qx0); st(P, P_SXX_X, o, qx1); st(P, P_SXY_X, o, qx2); }
+  if (py) { st(P, P_SYZ_Y, o, qy0); st(P, P_SXY_Y, o, qy1); st(P, P_SYY_Y, o, qy2); }
 }
 
 // ---------------------------------------------------------------------------
@@ -224,91 +339,113 @@ __device__ __forceinline__ void velocity_fwd_cell(const Geom& G, const Wave& W, 
 // launch order does. The adjoint part reads the adjoint stress and the psi
 // arrays written by the adjoint stress step, and writes v-dagger and the psi
 // arrays the adjoint stress step reads.
+// RV: reverse part allowed; XZ: the row lies in the x or z C-PML; Y: the
+// wavefront touches a y C-PML slab.
 // ---------------------------------------------------------------------------
-template <int FORM>
-__device__ __forceinline__ void velocity_bwd_cell(const Geom& G, const Wave& W, const Wave& A, const Psi& P,
-                                                  const Model& M, const Cpml& C, const Grad& Gr, bool rev,
-                                                  const Cell& c) {
-  const long i = c.i, SZ = G.pxy, SX = G.py;
+template <int FORM, bool RV, bool XZ, bool Y>
+__device__ __forceinline__ void velocity_bwd_cell(const Geom& G, const Grp& W, const Grp& A, const Grp& P,
+                                                  const Grp& M, const Prof& C, const Cell& c) {
+  const unsigned o = c.o;
   const int z = c.z, x = c.x, y = c.y, p = G.npml;
   const float dt = G.dt;
+  const Dv D(G);
+  const bool py = Y && in_pml(y, G.ny, p);
+  const bool px = XZ && in_pml(x, G.nx, p), pz = XZ && in_pml(z, G.nz, p);
+  const bool rev = RV && !XZ && !py;
   // ---- loads
   float r_sz = 0.0f, r_sx = 0.0f, r_sy = 0.0f, vz = 0.0f, vx = 0.0f, vy = 0.0f, grho = 0.0f;
   if (rev) {
-    r_sz = Df(W.szz, i, SZ, G.dz) + Db(W.sxz, i, SX, G.dx) + Db(W.syz, i, 1, G.dy);
-    r_sx = Db(W.sxz, i, SZ, G.dz) + Df(W.sxx, i, SX, G.dx) + Db(W.sxy, i, 1, G.dy);
-    r_sy = Db(W.syz, i, SZ, G.dz) + Dsxy_x<FORM>(W.sxy, i, SX, G.dx) + Df(W.syy, i, 1, G.dy);
-    vz = W.vz[i]; vx = W.vx[i]; vy = W.vy[i];
-    grho = Gr.grho[i];
+    r_sz = D.z(n_f(W, F_SZZ, o, c.zn)) + D.x(n_b(W, F_SXZ, o, c.xn)) + D.y(n_b(W, F_SYZ, o, c.yn));
+    r_sx = D.z(n_b(W, F_SXZ, o, c.zn)) + D.x(n_f(W, F_SXX, o, c.xn)) + D.y(n_b(W, F_SXY, o, c.yn));
+    r_sy = D.z(n_b(W, F_SYZ, o, c.zn)) + D.x(n_sxy_x<FORM>(W, F_SXY, o, c.xn)) +
+           D.y(n_f(W, F_SYY, o, c.yn));
+    vz = ld(W, F_VZ, o); vx = ld(W, F_VX, o); vy = ld(W, F_VY, o);
+    grho = ld(M, M_GRHO, o);
   }
-  const float lambda = M.lam[i], mu = M.mu[i], byc = M.byc[i];
-  const float avz = A.vz[i], avx = A.vx[i], avy = A.vy[i];
-  const float ax = C.a[1][x], az = C.a[0][z], ay = C.a[2][y];
-  const float axh = C.ah[1][x], azh = C.ah[0][z], ayh = C.ah[2][y];
-  const float Kx = C.K[1][x], Kz = C.K[0][z], Ky = C.K[2][y];
-  const float Kxh = C.Kh[1][x], Kzh = C.Kh[0][z], Kyh = C.Kh[2][y];
+  const float lambda = ld(M, M_LAM, o), mu = ld(M, M_MU, o), byc = ld(M, M_BYC, o);
+  const float avz = ld(A, F_VZ, o), avx = ld(A, F_VX, o), avy = ld(A, F_VY, o);
+  const float ax = XZ ? pr(C, 1, R_A, x) : 0.0f, az = XZ ? pr(C, 0, R_A, z) : 0.0f;
+  const float axh = XZ ? pr(C, 1, R_AH, x) : 0.0f, azh = XZ ? pr(C, 0, R_AH, z) : 0.0f;
+  const float ay = Y ? pr(C, 2, R_A, y) : 0.0f, ayh = Y ? pr(C, 2, R_AH, y) : 0.0f;
+  const float Kx = XZ ? pr(C, 1, R_K, x) : 1.0f;
+  const float Kz = XZ ? pr(C, 0, R_K, z) : 1.0f;
+  const float Kxh = XZ ? pr(C, 1, R_KH, x) : 1.0f;
+  const float Kzh = XZ ? pr(C, 0, R_KH, z) : 1.0f;
+  const float Ky = Y ? pr(C, 2, R_K, y) : 1.0f;
+  const float Kyh = Y ? pr(C, 2, R_KH, y) : 1.0f;
   // vx-dagger terms
-  const float x1 = apsi_f(ax, P.vx_x, i, SX, G.dx);
-  const float x_szz = Af(A.szz, i, SX, G.dx);
-  const float x_sxx = Af(A.sxx, i, SX, G.dx);
-  const float x_syy = Af(A.syy, i, SX, G.dx);
-  const float x6 = apsi_b(azh, P.vx_z, i, SZ, G.dz);
-  const float x_sxz = Ab(A.sxz, i, SZ, G.dz);
-  const float x8 = apsi_b(ayh, P.vx_y, i, 1, G.dy);
-  const float x_sxy = Ab(A.sxy, i, 1, G.dy);
+  const float x1 = apsi_f<XZ>(ax, P, P_VX_X, c, c.xn, G.dx);
+  const float x_szz = D.x(n_af(A, F_SZZ, o, c.xn));
+  const float x_sxx = D.x(n_af(A, F_SXX, o, c.xn));
+  const float x_syy = D.x(n_af(A, F_SYY, o, c.xn));
+  const float x6 = apsi_b<XZ>(azh, P, P_VX_Z, c, c.zn, G.dz);
+  const float x_sxz = D.z(n_ab(A, F_SXZ, o, c.zn));
+  const float x8 = apsi_b<Y>(ayh, P, P_VX_Y, c, c.yn, G.dy);
+  const float x_sxy = D.y(n_ab(A, F_SXY, o, c.yn));
   // vz-dagger terms
-  const float z1 = apsi_f(az, P.vz_z, i, SZ, G.dz);
-  const float z_szz = Af(A.szz, i, SZ, G.dz);
-  const float z_sxx = Af(A.sxx, i, SZ, G.dz);
-  const float z_syy = Af(A.syy, i, SZ, G.dz);
-  const float z8 = apsi_b(axh, P.vz_x, i, SX, G.dx);
-  const float z_sxz = Ab(A.sxz, i, SX, G.dx);
-  const float z6 = apsi_b(ayh, P.vz_y, i, 1, G.dy);
-  const float z_syz = Ab(A.syz, i, 1, G.dy);
+  const float z1 = apsi_f<XZ>(az, P, P_VZ_Z, c, c.zn, G.dz);
+  const float z_szz = D.z(n_af(A, F_SZZ, o, c.zn));
+  const float z_sxx = D.z(n_af(A, F_SXX, o, c.zn));
+  const float z_syy = D.z(n_af(A, F_SYY, o, c.zn));
+  const float z8 = apsi_b<XZ>(axh, P, P_VZ_X, c, c.xn, G.dx);
+  const float z_sxz = D.x(n_ab(A, F_SXZ, o, c.xn));
+  const float z6 = apsi_b<Y>(ayh, P, P_VZ_Y, c, c.yn, G.dy);
+  const float z_syz = D.y(n_ab(A, F_SYZ, o, c.yn));
   // vy-dagger terms
-  const float y1 = apsi_f(ay, P.vy_y, i, 1, G.dy);
-  const float y_syy = Af(A.syy, i, 1, G.dy);
-  const float y_sxx = Af(A.sxx, i, 1, G.dy);
-  const float y_szz = Af(A.szz, i, 1, G.dy);
-  const float y6 = apsi_b(axh, P.vy_x, i, SX, G.dx);
-  const float y_sxy = Ab(A.sxy, i, SX, G.dx);
-  const float y8 = apsi_b(azh, P.vy_z, i, SZ, G.dz);
-  const float y_syz = Ab(A.syz, i, SZ, G.dz);
-  const bool pz = in_pml(z, G.nz, p), px = in_pml(x, G.nx, p), py = in_pml(y, G.ny, p);
-  // ---- arithmetic and stores
-  if (rev) {
-    W.vz[i] = vz - r_sz * byc * dt;
-    W.vx[i] = vx - r_sx * byc * dt;
-    W.vy[i] = vy - r_sy * byc * dt;
-    const float nb2 = -(byc * byc);
-    Gr.grho[i] = grho + -(avz * r_sz * dt * nb2 + avx * r_sx * dt * nb2 + avy * r_sy * dt * nb2);
-  }
+  const float y1 = apsi_f<Y>(ay, P, P_VY_Y, c, c.yn, G.dy);
+  const float y_syy = D.y(n_af(A, F_SYY, o, c.yn));
+  const float y_sxx = D.y(n_af(A, F_SXX, o, c.yn));
+  const float y_szz = D.y(n_af(A, F_SZZ, o, c.yn));
+  const float y6 = apsi_b<XZ>(axh, P, P_VY_X, c, c.xn, G.dx);
+  const float y_sxy = D.x(n_ab(A, F_SXY, o, c.xn));
+  const float y8 = apsi_b<XZ>(azh, P, P_VY_Z, c, c.zn, G.dz);
+  const float y_syz = D.z(n_ab(A, F_SYZ, o, c.zn));
+  // psi values updated by this step
+  float qx0 = 0, qx1 = 0, qx2 = 0, qz0 = 0, qz1 = 0, qz2 = 0, qy0 = 0, qy1 = 0, qy2 = 0;
+  if (px) { qx0 = ld(P, P_SXX_X, o); qx1 = ld(P, P_SXZ_X, o); qx2 = ld(P, P_SXY_X, o); }
+  if (pz) { qz0 = ld(P, P_SXZ_Z, o); qz1 = ld(P, P_SZZ_Z, o); qz2 = ld(P, P_SYZ_Z, o); }
+  if (py) { qy0 = ld(P, P_SXY_Y, o); qy1 = ld(P, P_SYZ_Y, o); qy2 = ld(P, P_SYY_Y, o); }
+  // ---- arithmetic
   const float l2m = lambda + 2.0f * mu;
-  const float nvx = avx + (x1 + l2m * x_sxx / Kx * dt + lambda * x_szz / Kx * dt + lambda * x_syy / Kx * dt +
-                           mu / Kzh * x_sxz * dt + x6 + mu / Kyh * x_sxy * dt + x8);
-  const float nvz = avz + (z1 + l2m * z_szz / Kz * dt + lambda * z_sxx / Kz * dt + lambda * z_syy / Kz * dt +
-                           mu / Kyh * z_syz * dt + z6 + mu / Kxh * z_sxz * dt + z8);
-  const float nvy = avy + (y1 + l2m * y_syy / Ky * dt + lambda * y_sxx / Ky * dt + lambda * y_szz / Ky * dt +
-                           mu / Kxh * y_sxy * dt + y6 + mu / Kzh * y_syz * dt + y8);
-  A.vx[i] = nvx;
-  A.vz[i] = nvz;
-  A.vy[i] = nvy;
+  const float nvx = avx + (x1 + kdiv<XZ>(l2m * x_sxx, Kx) * dt + kdiv<XZ>(lambda * x_szz, Kx) * dt +
+                           kdiv<XZ>(lambda * x_syy, Kx) * dt + kdiv<XZ>(mu, Kzh) * x_sxz * dt + x6 +
+                           kdiv<Y>(mu, Kyh) * x_sxy * dt + x8);
+  const float nvz = avz + (z1 + kdiv<XZ>(l2m * z_szz, Kz) * dt + kdiv<XZ>(lambda * z_sxx, Kz) * dt +
+                           kdiv<XZ>(lambda * z_syy, Kz) * dt + kdiv<Y>(mu, Kyh) * z_syz * dt + z6 +
+                           kdiv<XZ>(mu, Kxh) * z_sxz * dt + z8);
+  const float nvy = avy + (y1 + kdiv<Y>(l2m * y_syy, Ky) * dt + kdiv<Y>(lambda * y_sxx, Ky) * dt +
+                           kdiv<Y>(lambda * y_szz, Ky) * dt + kdiv<XZ>(mu, Kxh) * y_sxy * dt + y6 +
+                           kdiv<XZ>(mu, Kzh) * y_syz * dt + y8);
+  // ---- stores
+  if (rev) {
+    st(W, F_VZ, o, vz - r_sz * byc * dt);
+    st(W, F_VX, o, vx - r_sx * byc * dt);
+    st(W, F_VY, o, vy - r_sy * byc * dt);
+    const float nb2 = -(byc * byc);
+    st(M, M_GRHO, o, grho + -(avz * r_sz * dt * nb2 + avx * r_sx * dt * nb2 + avy * r_sy * dt * nb2));
+  }
+  st(A, F_VX, o, nvx);
+  st(A, F_VZ, o, nvz);
+  st(A, F_VY, o, nvy);
   if (px) {
-    P.sxx_x[i] = C.bh[1][x] * P.sxx_x[i] + byc * nvx * dt;
-    P.sxz_x[i] = C.b[1][x] * P.sxz_x[i] + byc * nvz * dt;
-    P.sxy_x[i] = C.b[1][x] * P.sxy_x[i] + byc * nvy * dt;
+    const float b = pr(C, 1, R_B, x), bh = pr(C, 1, R_BH, x);
+    st(P, P_SXX_X, o, bh * qx0 + byc * nvx * dt);
+    st(P, P_SXZ_X, o, b * qx1 + byc * nvz * dt);
+    st(P, P_SXY_X, o, b * qx2 + byc * nvy * dt);
   }
   if (pz) {
-    P.sxz_z[i] = C.b[0][z] * P.sxz_z[i] + byc * nvx * dt;
-    P.szz_z[i] = C.bh[0][z] * P.szz_z[i] + byc * nvz * dt;
+    const float b = pr(C, 0, R_B, z), bh = pr(C, 0, R_BH, z);
+    st(P, P_SXZ_Z, o, b * qz0 + byc * nvx * dt);
+    st(P, P_SZZ_Z, o, bh * qz1 + byc * nvz * dt);
     // Src/velocity_adj.cu:285 decays psi_dsyz_dz with b_z_half (the forward uses b_z)
-    P.syz_z[i] = (FORM == 1 ? C.b[0][z] : C.bh[0][z]) * P.syz_z[i] + byc * nvy * dt;
+    st(P, P_SYZ_Z, o, (FORM == 1 ? b : bh) * qz2 + byc * nvy * dt);
   }
   if (py) {
-    P.sxy_y[i] = C.b[2][y] * P.sxy_y[i] + byc * nvx * dt;
-    P.syz_y[i] = C.b[2][y] * P.syz_y[i] + byc * nvz * dt;
+    const float b = pr(C, 2, R_B, y), bh = pr(C, 2, R_BH, y);
+    st(P, P_SXY_Y, o, b * qy0 + byc * nvx * dt);
+    st(P, P_SYZ_Y, o, b * qy1 + byc * nvz * dt);
     // Src/velocity_adj.cu:288 decays psi_dsyy_dy with b_y (the forward uses b_y_half)
-    P.syy_y[i] = (FORM == 1 ? C.bh[2][y] : C.b[2][y]) * P.syy_y[i] + byc * nvy * dt;
+    st(P, P_SYY_Y, o, (FORM == 1 ? bh : b) * qy2 + byc * nvy * dt);
   }
 }
 
@@ -319,149 +456,194 @@ __device__ __forceinline__ void velocity_bwd_cell(const Geom& G, const Wave& W, 
 // [2, n-3] the adjoint stress (Src/stress_adj.cu:1-146). The imaging terms
 // read the adjoint stress before the adjoint part updates it.
 // ---------------------------------------------------------------------------
-__device__ __forceinline__ void stress_bwd_cell(const Geom& G, const Wave& W, const Wave& A, const Psi& P,
-                                                const Model& M, const Cpml& C, const Grad& Gr, const Source& S,
-                                                int it, bool rev, const Cell& c) {
-  const long i = c.i, SZ = G.pxy, SX = G.py;
+template <bool RV, bool XZ, bool Y>
+__device__ __forceinline__ void stress_bwd_cell(const Geom& G, const Grp& W, const Grp& A, const Grp& P,
+                                                const Grp& M, const Prof& C, const Cell& c, unsigned src_o,
+                                                float amp) {
+  const unsigned o = c.o;
   const int z = c.z, x = c.x, y = c.y, p = G.npml;
   const float dt = G.dt;
-  const float lambda = M.lam[i], mu = M.mu[i], byc = M.byc[i];
+  const Dv D(G);
+  const bool py = Y && in_pml(y, G.ny, p);
+  const bool px = XZ && in_pml(x, G.nx, p), pz = XZ && in_pml(z, G.nz, p);
+  const bool rev = RV && !XZ && !py;
+  const float lambda = ld(M, M_LAM, o), mu = ld(M, M_MU, o), byc = ld(M, M_BYC, o);
   // ---- loads of the reverse part
   float dvz_dz = 0, dvx_dx = 0, dvy_dy = 0, dvx_dz = 0, dvx_dy = 0, dvz_dx = 0, dvz_dy = 0, dvy_dx = 0, dvy_dz = 0;
-  float szz = 0, sxx = 0, syy = 0, sxz = 0, sxy = 0, syz = 0, glam = 0, gmu = 0, amp = 0;
+  float szz = 0, sxx = 0, syy = 0, sxz = 0, sxy = 0, syz = 0, glam = 0, gmu = 0;
   if (rev) {
-    dvz_dz = Db(W.vz, i, SZ, G.dz);
-    dvx_dx = Db(W.vx, i, SX, G.dx);
-    dvy_dy = Db(W.vy, i, 1, G.dy);
-    dvx_dz = Df(W.vx, i, SZ, G.dz);
-    dvx_dy = Df(W.vx, i, 1, G.dy);
-    dvz_dx = Df(W.vz, i, SX, G.dx);
-    dvz_dy = Df(W.vz, i, 1, G.dy);
-    dvy_dx = Df(W.vy, i, SX, G.dx);
-    dvy_dz = Df(W.vy, i, SZ, G.dz);
-    szz = W.szz[i]; sxx = W.sxx[i]; syy = W.syy[i];
-    sxz = W.sxz[i]; sxy = W.sxy[i]; syz = W.syz[i];
-    glam = Gr.glam[i]; gmu = Gr.gmu[i];
-    if (i == S.cell) amp = 2250000.0f * S.stf[it] * dt * 1.0f;
+    dvz_dz = D.z(n_b(W, F_VZ, o, c.zn));
+    dvx_dx = D.x(n_b(W, F_VX, o, c.xn));
+    dvy_dy = D.y(n_b(W, F_VY, o, c.yn));
+    dvx_dz = D.z(n_f(W, F_VX, o, c.zn));
+    dvx_dy = D.y(n_f(W, F_VX, o, c.yn));
+    dvz_dx = D.x(n_f(W, F_VZ, o, c.xn));
+    dvz_dy = D.y(n_f(W, F_VZ, o, c.yn));
+    dvy_dx = D.x(n_f(W, F_VY, o, c.xn));
+    dvy_dz = D.z(n_f(W, F_VY, o, c.zn));
+    szz = ld(W, F_SZZ, o); sxx = ld(W, F_SXX, o); syy = ld(W, F_SYY, o);
+    sxz = ld(W, F_SXZ, o); sxy = ld(W, F_SXY, o); syz = ld(W, F_SYZ, o);
+    glam = ld(M, M_GLAM, o); gmu = ld(M, M_GMU, o);
   }
   // ---- loads of the adjoint part
-  const float aszz = A.szz[i], asxx = A.sxx[i], asyy = A.syy[i];
-  const float asxz = A.sxz[i], asxy = A.sxy[i], asyz = A.syz[i];
-  const float ax = C.a[1][x], az = C.a[0][z], ay = C.a[2][y];
-  const float Kx = C.K[1][x], Kz = C.K[0][z], Ky = C.K[2][y];
-  const float xz1 = apsi_f(ax, P.sxz_x, i, SX, G.dx);
-  const float xz_vz = Af(A.vz, i, SX, G.dx);
-  const float xz3 = apsi_f(az, P.sxz_z, i, SZ, G.dz);
-  const float xz_vx = Af(A.vx, i, SZ, G.dz);
-  const float xy1 = apsi_f(ax, P.sxy_x, i, SX, G.dx);
-  const float xy_vy = Af(A.vy, i, SX, G.dx);
-  const float xy3 = apsi_f(ay, P.sxy_y, i, 1, G.dy);
-  const float xy_vx = Af(A.vx, i, 1, G.dy);
-  const float yz1 = apsi_f(ay, P.syz_y, i, 1, G.dy);
-  const float yz_vz = Af(A.vz, i, 1, G.dy);
-  const float yz3 = apsi_f(az, P.syz_z, i, SZ, G.dz);
-  const float yz_vy = Af(A.vy, i, SZ, G.dz);
-  const float txx = apsi_b(C.ah[1][x], P.sxx_x, i, SX, G.dx);
-  const float xx_vx = Ab(A.vx, i, SX, G.dx);
-  const float tzz = apsi_b(C.ah[0][z], P.szz_z, i, SZ, G.dz);
-  const float zz_vz = Ab(A.vz, i, SZ, G.dz);
-  const float tyy = apsi_b(C.ah[2][y], P.syy_y, i, 1, G.dy);
-  const float yy_vy = Ab(A.vy, i, 1, G.dy);
-  const bool pz = in_pml(z, G.nz, p), px = in_pml(x, G.nx, p), py = in_pml(y, G.ny, p);
-  // ---- arithmetic and stores
+  const float aszz = ld(A, F_SZZ, o), asxx = ld(A, F_SXX, o), asyy = ld(A, F_SYY, o);
+  const float asxz = ld(A, F_SXZ, o), asxy = ld(A, F_SXY, o), asyz = ld(A, F_SYZ, o);
+  const float ax = XZ ? pr(C, 1, R_A, x) : 0.0f, az = XZ ? pr(C, 0, R_A, z) : 0.0f;
+  const float axh = XZ ? pr(C, 1, R_AH, x) : 0.0f, azh = XZ ? pr(C, 0, R_AH, z) : 0.0f;
+  const float ay = Y ? pr(C, 2, R_A, y) : 0.0f, ayh = Y ? pr(C, 2, R_AH, y) : 0.0f;
+  const float Kx = XZ ? pr(C, 1, R_K, x) : 1.0f;
+  const float Kz = XZ ? pr(C, 0, R_K, z) : 1.0f;
+  const float Ky = Y ? pr(C, 2, R_K, y) : 1.0f;
+  const float Kxh = XZ ? pr(C, 1, R_KH, x) : 1.0f;
+  const float Kzh = XZ ? pr(C, 0, R_KH, z) : 1.0f;
+  const float Kyh = Y ? pr(C, 2, R_KH, y) : 1.0f;
+  const float xz1 = apsi_f<XZ>(ax, P, P_SXZ_X, c, c.xn, G.dx);
+  const float xz_vz = D.x(n_af(A, F_VZ, o, c.xn));
+  const float xz3 = apsi_f<XZ>(az, P, P_SXZ_Z, c, c.zn, G.dz);
+  const float xz_vx = D.z(n_af(A, F_VX, o, c.zn));
+  const float xy1 = apsi_f<XZ>(ax, P, P_SXY_X, c, c.xn, G.dx);
+  const float xy_vy = D.x(n_af(A, F_VY, o, c.xn));
+  const float xy3 = apsi_f<Y>(ay, P, P_SXY_Y, c, c.yn, G.dy);
+  const float xy_vx = D.y(n_af(A, F_VX, o, c.yn));
+  const float yz1 = apsi_f<Y>(ay, P, P_SYZ_Y, c, c.yn, G.dy);
+  const float yz_vz = D.y(n_af(A, F_VZ, o, c.yn));
+  const float yz3 = apsi_f<XZ>(az, P, P_SYZ_Z, c, c.zn, G.dz);
+  const float yz_vy = D.z(n_af(A, F_VY, o, c.zn));
+  const float txx = apsi_b<XZ>(axh, P, P_SXX_X, c, c.xn, G.dx);
+  const float xx_vx = D.x(n_ab(A, F_VX, o, c.xn));
+  const float tzz = apsi_b<XZ>(azh, P, P_SZZ_Z, c, c.zn, G.dz);
+  const float zz_vz = D.z(n_ab(A, F_VZ, o, c.zn));
+  const float tyy = apsi_b<Y>(ayh, P, P_SYY_Y, c, c.yn, G.dy);
+  const float yy_vy = D.y(n_ab(A, F_VY, o, c.yn));
+  // psi values updated by this step
+  float qx0 = 0, qx1 = 0, qx2 = 0, qz0 = 0, qz1 = 0, qz2 = 0, qy0 = 0, qy1 = 0, qy2 = 0;
+  if (px) { qx0 = ld(P, P_VZ_X, o); qx1 = ld(P, P_VY_X, o); qx2 = ld(P, P_VX_X, o); }
+  if (pz) { qz0 = ld(P, P_VX_Z, o); qz1 = ld(P, P_VY_Z, o); qz2 = ld(P, P_VZ_Z, o); }
+  if (py) { qy0 = ld(P, P_VX_Y, o); qy1 = ld(P, P_VZ_Y, o); qy2 = ld(P, P_VY_Y, o); }
+  // ---- arithmetic
   const float l2m = lambda + 2.0f * mu;
+  const float nsxz =
+      asxz + (xz1 + kdiv<XZ>(xz_vz, Kx) * byc * dt + xz3 + kdiv<XZ>(xz_vx, Kz) * byc * dt);
+  const float nsxy =
+      asxy + (xy1 + kdiv<XZ>(xy_vy, Kx) * byc * dt + xy3 + kdiv<Y>(xy_vx, Ky) * byc * dt);
+  const float nsyz =
+      asyz + (yz1 + kdiv<Y>(yz_vz, Ky) * byc * dt + yz3 + kdiv<XZ>(yz_vy, Kz) * byc * dt);
+  const float nsxx = asxx + (txx + kdiv<XZ>(byc * xx_vx, Kxh) * dt);
+  const float nszz = aszz + (tzz + kdiv<XZ>(byc * zz_vz, Kzh) * dt);
+  const float nsyy = asyy + (tyy + kdiv<Y>(byc * yy_vy, Kyh) * dt);
+  // ---- stores
   if (rev) {
-    if (i == S.cell) {
+    if (o == src_o) {
       szz -= amp; sxx -= amp; syy -= amp;
     }
-    W.szz[i] = szz - (l2m * dvz_dz + lambda * dvx_dx + lambda * dvy_dy) * dt;
-    W.sxx[i] = sxx - (lambda * dvz_dz + l2m * dvx_dx + lambda * dvy_dy) * dt;
-    W.syy[i] = syy - (lambda * dvz_dz + lambda * dvx_dx + l2m * dvy_dy) * dt;
-    W.sxz[i] = sxz - mu * (dvx_dz + dvz_dx) * dt;
-    W.sxy[i] = sxy - mu * (dvx_dy + dvy_dx) * dt;
-    W.syz[i] = syz - mu * (dvy_dz + dvz_dy) * dt;
-    Gr.glam[i] = glam + -(aszz + asxx + asyy) * (dvz_dz + dvx_dx + dvy_dy) * dt;
-    Gr.gmu[i] = gmu + -(2.0f * aszz * dvz_dz + 2.0f * asxx * dvx_dx + 2.0f * asyy * dvy_dy +
-                        asxz * (dvx_dz + dvz_dx) + asyz * (dvy_dz + dvz_dy) + asxy * (dvy_dx + dvx_dy)) * dt;
+    st(W, F_SZZ, o, szz - (l2m * dvz_dz + lambda * dvx_dx + lambda * dvy_dy) * dt);
+    st(W, F_SXX, o, sxx - (lambda * dvz_dz + l2m * dvx_dx + lambda * dvy_dy) * dt);
+    st(W, F_SYY, o, syy - (lambda * dvz_dz + lambda * dvx_dx + l2m * dvy_dy) * dt);
+    st(W, F_SXZ, o, sxz - mu * (dvx_dz + dvz_dx) * dt);
+    st(W, F_SXY, o, sxy - mu * (dvx_dy + dvy_dx) * dt);
+    st(W, F_SYZ, o, syz - mu * (dvy_dz + dvz_dy) * dt);
+    st(M, M_GLAM, o, glam + -(aszz + asxx + asyy) * (dvz_dz + dvx_dx + dvy_dy) * dt);
+    st(M, M_GMU, o, gmu + -(2.0f * aszz * dvz_dz + 2.0f * asxx * dvx_dx + 2.0f * asyy * dvy_dy +
+                            asxz * (dvx_dz + dvz_dx) + asyz * (dvy_dz + dvz_dy) + asxy * (dvy_dx + dvx_dy)) * dt);
   }
-  const float nsxz = asxz + (xz1 + xz_vz / Kx * byc * dt + xz3 + xz_vx / Kz * byc * dt);
-  const float nsxy = asxy + (xy1 + xy_vy / Kx * byc * dt + xy3 + xy_vx / Ky * byc * dt);
-  const float nsyz = asyz + (yz1 + yz_vz / Ky * byc * dt + yz3 + yz_vy / Kz * byc * dt);
-  const float nsxx = asxx + (txx + byc * xx_vx / C.Kh[1][x] * dt);
-  const float nszz = aszz + (tzz + byc * zz_vz / C.Kh[0][z] * dt);
-  const float nsyy = asyy + (tyy + byc * yy_vy / C.Kh[2][y] * dt);
-  A.sxz[i] = nsxz;
-  A.sxy[i] = nsxy;
-  A.syz[i] = nsyz;
-  A.sxx[i] = nsxx;
-  A.szz[i] = nszz;
-  A.syy[i] = nsyy;
+  st(A, F_SXZ, o, nsxz);
+  st(A, F_SXY, o, nsxy);
+  st(A, F_SYZ, o, nsyz);
+  st(A, F_SXX, o, nsxx);
+  st(A, F_SZZ, o, nszz);
+  st(A, F_SYY, o, nsyy);
   if (px) {
-    P.vz_x[i] = C.bh[1][x] * P.vz_x[i] + nsxz * mu * dt;
-    P.vy_x[i] = C.bh[1][x] * P.vy_x[i] + nsxy * mu * dt;
-    P.vx_x[i] = C.b[1][x] * P.vx_x[i] + lambda * nszz * dt + l2m * nsxx * dt + lambda * nsyy * dt;
+    const float b = pr(C, 1, R_B, x), bh = pr(C, 1, R_BH, x);
+    st(P, P_VZ_X, o, bh * qx0 + nsxz * mu * dt);
+    st(P, P_VY_X, o, bh * qx1 + nsxy * mu * dt);
+    st(P, P_VX_X, o, b * qx2 + lambda * nszz * dt + l2m * nsxx * dt + lambda * nsyy * dt);
   }
   if (pz) {
-    P.vx_z[i] = C.bh[0][z] * P.vx_z[i] + nsxz * mu * dt;
-    P.vy_z[i] = C.bh[0][z] * P.vy_z[i] + nsyz * mu * dt;
-    P.vz_z[i] = C.b[0][z] * P.vz_z[i] + l2m * nszz * dt + lambda * nsxx * dt + lambda * nsyy * dt;
+    const float b = pr(C, 0, R_B, z), bh = pr(C, 0, R_BH, z);
+    st(P, P_VX_Z, o, bh * qz0 + nsxz * mu * dt);
+    st(P, P_VY_Z, o, bh * qz1 + nsyz * mu * dt);
+    st(P, P_VZ_Z, o, b * qz2 + l2m * nszz * dt + lambda * nsxx * dt + lambda * nsyy * dt);
   }
   if (py) {
-    P.vx_y[i] = C.bh[2][y] * P.vx_y[i] + nsxy * mu * dt;
-    P.vz_y[i] = C.bh[2][y] * P.vz_y[i] + nsyz * mu * dt;
-    P.vy_y[i] = C.b[2][y] * P.vy_y[i] + lambda * nszz * dt + lambda * nsxx * dt + l2m * nsyy * dt;
+    const float b = pr(C, 2, R_B, y), bh = pr(C, 2, R_BH, y);
+    st(P, P_VX_Y, o, bh * qy0 + nsxy * mu * dt);
+    st(P, P_VZ_Y, o, bh * qy1 + nsyz * mu * dt);
+    st(P, P_VY_Y, o, b * qy2 + lambda * nszz * dt + lambda * nsxx * dt + l2m * nsyy * dt);
   }
+}
+
+// Per-wavefront dispatch of the backward cells on (XZ, Y).
+struct Flags {
+  bool xz, y;
+};
+__device__ __forceinline__ Flags flags_of(const Geom& G, const Cell& c) {
+  const int p = G.npml;
+  return Flags{in_pml(c.x, G.nx, p) || in_pml(c.z, G.nz, p), __any(in_pml(c.y, G.ny, p)) != 0};
+}
+
+// Source amplitude of step it at the source cell, 1500^2 * stf[it] * dt
+// (Src/utilities.cu:442-479; the 9^3 amplitude cube is exactly 1 at its
+// centre and 0 elsewhere), and the cell's byte offset (a value no cell has
+// when there is no source).
+__device__ __forceinline__ float source_amp(const Source& S, const Geom& G, int it) {
+  return S.cell >= 0 ? 2250000.0f * S.stf[it] * G.dt * 1.0f : 0.0f;
+}
+__device__ __forceinline__ unsigned source_off(const Source& S) {
+  return S.cell >= 0 ? 4u * (unsigned)S.cell : 0xFFFFFFFFu;
 }
 
 // ---------------------------------------------------------------------------
 // Kernels.
 // ---------------------------------------------------------------------------
-// Forward stress, then the source of this step at its cell (Src/libCUFD.cu:319-330).
-template <int Z>
-__global__ __launch_bounds__(256) void k_stress_fwd(Sweep Sw, Geom G, Wave W, Psi P, Model M, Cpml C, Source S,
-                                                    int it) {
-  sweep<Z>(Sw, G, [&](const Cell& c) {
-    stress_fwd_cell(G, W, P, M, C, c);
-    if (c.i == S.cell) {
-      const float a = 2250000.0f * S.stf[it] * G.dt * 1.0f;
-      W.szz[c.i] += a; W.sxx[c.i] += a; W.syy[c.i] += a;
+__global__ __launch_bounds__(256) EFWI_FWD_ATTR void k_stress_fwd(Sweep Sw, Geom G, Groups F, Prof C, Source S, int it) {
+  const Grp W = group(F.w, F_WAVE_COUNT, G.fs), P = group(F.p, P_PSI_COUNT, G.fs), M = group(F.m, M_COUNT, G.fs);
+  const unsigned so = source_off(S);
+  const float amp = source_amp(S, G, it);
+  sweep(Sw, G, [&](const Cell& c) { stress_fwd_cell(G, W, P, M, C, c, so, amp); });
+}
+
+template <int FORM>
+__global__ __launch_bounds__(256) EFWI_FWD_ATTR void k_velocity_fwd(Sweep Sw, Geom G, Groups F, Prof C) {
+  const Grp W = group(F.w, F_WAVE_COUNT, G.fs), P = group(F.p, P_PSI_COUNT, G.fs), M = group(F.m, M_COUNT, G.fs);
+  sweep(Sw, G, [&](const Cell& c) { velocity_fwd_cell<FORM>(G, W, P, M, C, c); });
+}
+
+// Backward velocity step (RV = true): reverse velocity + density imaging
+// (interior), then the adjoint velocity ([2, n-3]); RV = false: adjoint only.
+template <int FORM, bool RV>
+__global__ __launch_bounds__(256) EFWI_BWD_ATTR void k_velocity_bwd(Sweep Sw, Geom G, Groups F, Prof C) {
+  const Grp W = group(F.w, F_WAVE_COUNT, G.fs), A = group(F.a, F_WAVE_COUNT, G.fs);
+  const Grp P = group(F.p, P_PSI_COUNT, G.fs), M = group(F.m, M_COUNT, G.fs);
+  sweep(Sw, G, [&](const Cell& c) {
+    const Flags f = flags_of(G, c);
+    if (f.xz) {
+      if (f.y) velocity_bwd_cell<FORM, RV, true, true>(G, W, A, P, M, C, c);
+      else velocity_bwd_cell<FORM, RV, true, false>(G, W, A, P, M, C, c);
+    } else {
+      if (f.y) velocity_bwd_cell<FORM, RV, false, true>(G, W, A, P, M, C, c);
+      else velocity_bwd_cell<FORM, RV, false, false>(G, W, A, P, M, C, c);
     }
   });
 }
 
-template <int FORM, int Z>
-__global__ __launch_bounds__(256) void k_velocity_fwd(Sweep Sw, Geom G, Wave W, Psi P, Model M, Cpml C) {
-  sweep<Z>(Sw, G, [&](const Cell& c) { velocity_fwd_cell<FORM>(G, W, P, M, C, c); });
-}
-
-// Backward velocity step: reverse velocity + density imaging (interior), then
-// the adjoint velocity ([2, n-3]).
-template <int FORM, int Z>
-__global__ __launch_bounds__(256) void k_velocity_bwd(Sweep Sw, Geom G, Wave W, Wave A, Psi P, Model M, Cpml C,
-                                                      Grad Gr) {
-  sweep<Z>(Sw, G, [&](const Cell& c) { velocity_bwd_cell<FORM>(G, W, A, P, M, C, Gr, interior(G, c), c); });
-}
-
-// Backward stress step: (source subtraction +) reverse stress + lambda/mu
-// imaging (interior), then the adjoint stress ([2, n-3]).
-template <int Z>
-__global__ __launch_bounds__(256) void k_stress_bwd(Sweep Sw, Geom G, Wave W, Wave A, Psi P, Model M, Cpml C,
-                                                    Grad Gr, Source S, int it) {
-  sweep<Z>(Sw, G, [&](const Cell& c) { stress_bwd_cell(G, W, A, P, M, C, Gr, S, it, interior(G, c), c); });
-}
-
-template <int FORM, int Z>
-__global__ __launch_bounds__(256) void k_velocity_adj(Sweep Sw, Geom G, Wave W, Wave A, Psi P, Model M, Cpml C) {
-  sweep<Z>(Sw, G, [&](const Cell& c) { velocity_bwd_cell<FORM>(G, W, A, P, M, C, Grad{}, false, c); });
-}
-
-template <int Z>
-__global__ __launch_bounds__(256) void k_stress_adj(Sweep Sw, Geom G, Wave W, Wave A, Psi P, Model M, Cpml C) {
-  sweep<Z>(Sw, G, [&](const Cell& c) {
-    Source none;
-    none.cell = -1;
-    none.stf = nullptr;
-    stress_bwd_cell(G, W, A, P, M, C, Grad{}, none, 0, false, c);
+// Backward stress step (RV = true): (source subtraction +) reverse stress +
+// lambda/mu imaging (interior), then the adjoint stress ([2, n-3]);
+// RV = false: adjoint only.
+template <bool RV>
+__global__ __launch_bounds__(256) EFWI_BWD_ATTR void k_stress_bwd(Sweep Sw, Geom G, Groups F, Prof C, Source S, int it) {
+  const Grp W = group(F.w, F_WAVE_COUNT, G.fs), A = group(F.a, F_WAVE_COUNT, G.fs);
+  const Grp P = group(F.p, P_PSI_COUNT, G.fs), M = group(F.m, M_COUNT, G.fs);
+  const unsigned so = RV ? source_off(S) : 0xFFFFFFFFu;
+  const float amp = RV ? source_amp(S, G, it) : 0.0f;
+  sweep(Sw, G, [&](const Cell& c) {
+    const Flags f = flags_of(G, c);
+    if (f.xz) {
+      if (f.y) stress_bwd_cell<RV, true, true>(G, W, A, P, M, C, c, so, amp);
+      else stress_bwd_cell<RV, true, false>(G, W, A, P, M, C, c, so, amp);
+    } else {
+      if (f.y) stress_bwd_cell<RV, false, true>(G, W, A, P, M, C, c, so, amp);
+      else stress_bwd_cell<RV, false, false>(G, W, A, P, M, C, c, so, amp);
+    }
   });
 }
 
@@ -493,71 +675,37 @@ static inline Sweep sweep_full(const Geom& G, Tile t) { return make_sweep(2, G.n
 static inline dim3 grid_of(const Sweep& S) { return dim3((unsigned)S.blocks); }
 static inline dim3 block_of(const Sweep& S) { return dim3(64, S.R); }
 
-// Dispatch on the compile-time plane depth of a tile (1, 2 or 4).
-#define EFWI_DISPATCH_Z(t, LAUNCH) \
-  do {                             \
-    if ((t).zdepth >= 4) {         \
-      constexpr int ZD = 4;        \
-      LAUNCH;                      \
-    } else if ((t).zdepth == 2) {  \
-      constexpr int ZD = 2;        \
-      LAUNCH;                      \
-    } else {                       \
-      constexpr int ZD = 1;        \
-      LAUNCH;                      \
-    }                              \
-  } while (0)
-
-static inline Tile norm_tile(Tile t) {
-  t.zdepth = t.zdepth >= 4 ? 4 : (t.zdepth == 2 ? 2 : 1);
-  return t;
-}
-
-void launch_stress_fwd(Tile t, const Geom& G, const Wave& W, const Psi& P, const Model& M, const Cpml& C,
-                       const Source& S, int it, hipStream_t s) {
-  t = norm_tile(t);
-  const Sweep Sw = sweep_full(G, t);
-  EFWI_DISPATCH_Z(t, hipLaunchKernelGGL(k_stress_fwd<ZD>, grid_of(Sw), block_of(Sw), 0, s, Sw, G, W, P, M, C, S, it));
-}
-void launch_velocity_fwd(Tile t, int form, const Geom& G, const Wave& W, const Psi& P, const Model& M,
-                         const Cpml& C, hipStream_t s) {
-  t = norm_tile(t);
-  const Sweep Sw = sweep_full(G, t);
-  if (form == 1)
-    EFWI_DISPATCH_Z(t, hipLaunchKernelGGL((k_velocity_fwd<1, ZD>), grid_of(Sw), block_of(Sw), 0, s, Sw, G, W, P, M, C));
-  else
-    EFWI_DISPATCH_Z(t, hipLaunchKernelGGL((k_velocity_fwd<0, ZD>), grid_of(Sw), block_of(Sw), 0, s, Sw, G, W, P, M, C));
-}
-void launch_velocity_bwd(Tile t, int form, const Geom& G, const Wave& W, const Wave& A, const Psi& P,
-                         const Model& M, const Cpml& C, const Grad& Gr, hipStream_t s) {
-  t = norm_tile(t);
-  const Sweep Sw = sweep_full(G, t);
-  if (form == 1)
-    EFWI_DISPATCH_Z(t, hipLaunchKernelGGL((k_velocity_bwd<1, ZD>), grid_of(Sw), block_of(Sw), 0, s, Sw, G, W, A, P,
-                                          M, C, Gr));
-  else
-    EFWI_DISPATCH_Z(t, hipLaunchKernelGGL((k_velocity_bwd<0, ZD>), grid_of(Sw), block_of(Sw), 0, s, Sw, G, W, A, P,
-                                          M, C, Gr));
-}
-void launch_stress_bwd(Tile t, const Geom& G, const Wave& W, const Wave& A, const Psi& P, const Model& M,
-                       const Cpml& C, const Grad& Gr, const Source& S, int it, hipStream_t s) {
-  t = norm_tile(t);
-  const Sweep Sw = sweep_full(G, t);
-  EFWI_DISPATCH_Z(t, hipLaunchKernelGGL(k_stress_bwd<ZD>, grid_of(Sw), block_of(Sw), 0, s, Sw, G, W, A, P, M, C, Gr,
-                                        S, it));
-}
-void launch_velocity_adj(int form, const Geom& G, const Wave& A, const Psi& P, const Model& M,
-                         const Cpml& C, hipStream_t s) {
-  const Sweep Sw = sweep_full(G, Tile{1, 1});
-  if (form == 1)
-    hipLaunchKernelGGL((k_velocity_adj<1, 1>), grid_of(Sw), block_of(Sw), 0, s, Sw, G, Wave{}, A, P, M, C);
-  else
-    hipLaunchKernelGGL((k_velocity_adj<0, 1>), grid_of(Sw), block_of(Sw), 0, s, Sw, G, Wave{}, A, P, M, C);
-}
-void launch_stress_adj(const Geom& G, const Wave& A, const Psi& P, const Model& M, const Cpml& C,
+void launch_stress_fwd(Tile t, const Geom& G, const Groups& F, const Prof& C, const Source& S, int it,
                        hipStream_t s) {
+  const Sweep Sw = sweep_full(G, t);
+  hipLaunchKernelGGL(k_stress_fwd, grid_of(Sw), block_of(Sw), 0, s, Sw, G, F, C, S, it);
+}
+void launch_velocity_fwd(Tile t, int form, const Geom& G, const Groups& F, const Prof& C, hipStream_t s) {
+  const Sweep Sw = sweep_full(G, t);
+  if (form == 1) hipLaunchKernelGGL(k_velocity_fwd<1>, grid_of(Sw), block_of(Sw), 0, s, Sw, G, F, C);
+  else hipLaunchKernelGGL(k_velocity_fwd<0>, grid_of(Sw), block_of(Sw), 0, s, Sw, G, F, C);
+}
+void launch_velocity_bwd(Tile t, int form, const Geom& G, const Groups& F, const Prof& C, hipStream_t s) {
+  const Sweep Sw = sweep_full(G, t);
+  if (form == 1) hipLaunchKernelGGL((k_velocity_bwd<1, true>), grid_of(Sw), block_of(Sw), 0, s, Sw, G, F, C);
+  else hipLaunchKernelGGL((k_velocity_bwd<0, true>), grid_of(Sw), block_of(Sw), 0, s, Sw, G, F, C);
+}
+void launch_stress_bwd(Tile t, const Geom& G, const Groups& F, const Prof& C, const Source& S, int it,
+                       hipStream_t s) {
+  const Sweep Sw = sweep_full(G, t);
+  hipLaunchKernelGGL(k_stress_bwd<true>, grid_of(Sw), block_of(Sw), 0, s, Sw, G, F, C, S, it);
+}
+void launch_velocity_adj(int form, const Geom& G, const Groups& F, const Prof& C, hipStream_t s) {
   const Sweep Sw = sweep_full(G, Tile{1, 1});
-  hipLaunchKernelGGL(k_stress_adj<1>, grid_of(Sw), block_of(Sw), 0, s, Sw, G, Wave{}, A, P, M, C);
+  if (form == 1) hipLaunchKernelGGL((k_velocity_bwd<1, false>), grid_of(Sw), block_of(Sw), 0, s, Sw, G, F, C);
+  else hipLaunchKernelGGL((k_velocity_bwd<0, false>), grid_of(Sw), block_of(Sw), 0, s, Sw, G, F, C);
+}
+void launch_stress_adj(const Geom& G, const Groups& F, const Prof& C, hipStream_t s) {
+  const Sweep Sw = sweep_full(G, Tile{1, 1});
+  Source none;
+  none.cell = -1;
+  none.stf = nullptr;
+  hipLaunchKernelGGL(k_stress_bwd<false>, grid_of(Sw), block_of(Sw), 0, s, Sw, G, F, C, none, 0);
 }
 
 }  // namespace efwi

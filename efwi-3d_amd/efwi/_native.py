@@ -15,7 +15,8 @@ import os
 import torch  # noqa: F401  (must precede loading libefwi.so)
 
 _HERE = os.path.dirname(os.path.abspath(__file__))
-LIB_PATH = os.path.join(_HERE, "libefwi.so")
+# EFWI_LIBRARY selects another build of the same library (e.g. a tuning variant)
+LIB_PATH = os.environ.get("EFWI_LIBRARY") or os.path.join(_HERE, "libefwi.so")
 
 OK = 0
 KERNELS = ("stress_fwd", "velocity_fwd", "stress_bwd", "velocity_bwd", "velocity_adj", "stress_adj",

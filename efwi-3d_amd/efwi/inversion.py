@@ -80,7 +80,7 @@ class Problem:
         f = shot_misfit(lam, mu, rho, self.prop, self.shots, self.obs)
         f.backward()
         self.nfev += 1
-        return float(f), (x.grad * self.mask * self.grad_scale).detach()
+        return float(f.detach()), (x.grad * self.mask * self.grad_scale).detach()
 
     def project(self, vp):
         return vp.clamp(self.vmin, self.vmax)

@@ -66,7 +66,7 @@ class LBFGS:
 
 
 def armijo(f0: float, slope: float, evaluate: Callable, step: float, c1: float = 1e-4, shrink: float = 0.5,
-           max_tries: int = 8):
+           max_tries: int = 12):
     """Backtracking line search on f(step) <= f0 + c1 step slope.
 
     `evaluate(step)` returns (f, payload). Returns (step, f, payload, tries) of
