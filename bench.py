@@ -40,6 +40,24 @@ def alg_bytes_pipeline(grid):
     return 120.0 * npad * (nt - 1) + 180.0 * nint * (nt - 1) + 132.0 * npad * nt
 
 
+def hbm_traffic(kernel, grid, workload):
+    """HBM bytes per launch of `kernel` from the committed rocprofv3 counter
+    summary (profiles/hbm_traffic.json: FETCH_SIZE and WRITE_SIZE in KiB per
+    dispatch, each scaled by the factor the same counter reads for a plain
+    copy of known size), or None when there is none for this workload."""
+    path = os.path.join(ROOT, "profiles", "hbm_traffic.json")
+    if not os.path.exists(path):
+        return None
+    with open(path) as f:
+        t = json.load(f)
+    if t.get("workload") != workload or tuple(t.get("grid", ())) != tuple(grid.shape):
+        return None
+    k = t.get("kernels", {}).get(kernel)
+    if not k:
+        return None
+    return 1024.0 * (k["fetch_kib"] * t["fetch_scale"] + k["write_kib"] * t["write_scale"])
+
+
 def cpu_baseline(w, nsteps):
     """The CPU oracle (fp32, OpenMP over all host threads) on the first
     `nsteps` time steps of the same workload (fwd + adjoint + gradient)."""
@@ -153,8 +171,11 @@ def main():
     if kernels:
         dom = max(kernels, key=lambda k: prof[k][0])
         ach = kernels[dom]["alg_GBps"]
+        traffic = hbm_traffic(dom, g, w.name)
         roof = {"bound": "hbm", "kernel": dom, "achieved": ach, "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                "frac": round(ach / HBM_PEAK_GBS, 4), "traffic": None}
+                "frac": round(ach / HBM_PEAK_GBS, 4),
+                "traffic": None if traffic is None else round(traffic),
+                "alg_bytes": round(alg_bytes_per_launch(dom, g))}
 
     cpu = None
     if not args.no_cpu_baseline:
