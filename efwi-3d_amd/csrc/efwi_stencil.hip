@@ -97,27 +97,50 @@ struct Cell {
   Nb zn, xn, yn;
 };
 
-// Difference numerators of the 4th-order staggered operators; the caller
-// divides by the grid step.
-// Backward / forward, Src/stress.cu:40-68.
-__device__ __forceinline__ float n_b(const Grp& g, int k, unsigned o, const Nb& n) {
-  return C1 * (ld(g, k, o) - ld(g, k, n.m1)) - C2 * (ld(g, k, n.p1) - ld(g, k, n.m2));
+// Field groups and axes of an accessor call.
+enum GroupId { GW, GA, GP, GM };
+enum Axis { AZ, AX, AY };
+
+// Accessor of the flat sweep: every value of the cell or of a neighbour is a
+// buffer load; offsets of the four neighbours per axis precomputed.
+struct FlatAcc {
+  Grp w, a, p, m;
+  unsigned o;
+  Nb n[3];
+  __device__ __forceinline__ const Grp& g(int gi) const { return gi == GW ? w : gi == GA ? a : gi == GP ? p : m; }
+  __device__ __forceinline__ float own(int gi, int k) const { return ld(g(gi), k, o); }
+  // value at offset d (-2..2) along axis ax
+  __device__ __forceinline__ float at(int gi, int k, int ax, int d) const {
+    const Nb& b = n[ax];
+    return ld(g(gi), k, d == 0 ? o : d == -2 ? b.m2 : d == -1 ? b.m1 : d == 1 ? b.p1 : b.p2);
+  }
+  __device__ __forceinline__ void st(int gi, int k, float v) const { efwi::st(g(gi), k, o, v); }
+};
+
+// Difference numerators of the 4th-order staggered operators along axis ax;
+// the caller divides by the grid step. Backward / forward, Src/stress.cu:40-68.
+template <class Acc>
+__device__ __forceinline__ float n_b(const Acc& f, int g, int k, int ax) {
+  return C1 * (f.at(g, k, ax, 0) - f.at(g, k, ax, -1)) - C2 * (f.at(g, k, ax, 1) - f.at(g, k, ax, -2));
 }
-__device__ __forceinline__ float n_f(const Grp& g, int k, unsigned o, const Nb& n) {
-  return C1 * (ld(g, k, n.p1) - ld(g, k, o)) - C2 * (ld(g, k, n.p2) - ld(g, k, n.m1));
+template <class Acc>
+__device__ __forceinline__ float n_f(const Acc& f, int g, int k, int ax) {
+  return C1 * (f.at(g, k, ax, 1) - f.at(g, k, ax, 0)) - C2 * (f.at(g, k, ax, 2) - f.at(g, k, ax, -1));
 }
 // Adjoint forms as written in Src/stress_adj.cu:54-59.
-__device__ __forceinline__ float n_af(const Grp& g, int k, unsigned o, const Nb& n) {
-  return -C1 * (ld(g, k, n.p1) - ld(g, k, o)) + C2 * (ld(g, k, n.p2) - ld(g, k, n.m1));
+template <class Acc>
+__device__ __forceinline__ float n_af(const Acc& f, int g, int k, int ax) {
+  return -C1 * (f.at(g, k, ax, 1) - f.at(g, k, ax, 0)) + C2 * (f.at(g, k, ax, 2) - f.at(g, k, ax, -1));
 }
-__device__ __forceinline__ float n_ab(const Grp& g, int k, unsigned o, const Nb& n) {
-  return -C1 * (ld(g, k, o) - ld(g, k, n.m1)) + C2 * (ld(g, k, n.p1) - ld(g, k, n.m2));
+template <class Acc>
+__device__ __forceinline__ float n_ab(const Acc& f, int g, int k, int ax) {
+  return -C1 * (f.at(g, k, ax, 0) - f.at(g, k, ax, -1)) + C2 * (f.at(g, k, ax, 1) - f.at(g, k, ax, -2));
 }
 // dsxy/dx of the vy update (Src/velocity.cu:79 and :122).
-template <int FORM>
-__device__ __forceinline__ float n_sxy_x(const Grp& g, int k, unsigned o, const Nb& n) {
-  if (FORM == 1) return n_b(g, k, o, n);
-  return C1 * (ld(g, k, o) - ld(g, k, n.m1)) - C2 * (ld(g, k, n.m1) - ld(g, k, n.m2));
+template <int FORM, class Acc>
+__device__ __forceinline__ float n_sxy_x(const Acc& f, int g, int k, int ax) {
+  if (FORM == 1) return n_b(f, g, k, ax);
+  return C1 * (f.at(g, k, ax, 0) - f.at(g, k, ax, -1)) - C2 * (f.at(g, k, ax, -1) - f.at(g, k, ax, -2));
 }
 
 // Quotients by the grid steps (IEEE division, as the reference's).
@@ -136,15 +159,15 @@ __device__ __forceinline__ float pr(const Prof& C, int ax, int row, int i) {
 
 // a * d(psi)/dh of the adjoint kernels, skipped where a == 0 (the reference
 // multiplies by zero there); ON = false where a is known to be zero.
-template <bool ON>
-__device__ __forceinline__ float apsi_f(float a, const Grp& P, int k, const Cell& c, const Nb& n, float h) {
+template <bool ON, class Acc>
+__device__ __forceinline__ float apsi_f(const Acc& f, float a, int k, int ax, float h) {
   if (!ON) return 0.0f;
-  return a != 0.0f ? a * (n_af(P, k, c.o, n) / h) : 0.0f;
+  return a != 0.0f ? a * (n_af(f, GP, k, ax) / h) : 0.0f;
 }
-template <bool ON>
-__device__ __forceinline__ float apsi_b(float a, const Grp& P, int k, const Cell& c, const Nb& n, float h) {
+template <bool ON, class Acc>
+__device__ __forceinline__ float apsi_b(const Acc& f, float a, int k, int ax, float h) {
   if (!ON) return 0.0f;
-  return a != 0.0f ? a * (n_ab(P, k, c.o, n) / h) : 0.0f;
+  return a != 0.0f ? a * (n_ab(f, GP, k, ax) / h) : 0.0f;
 }
 // v / K; ON = false where K is known to be one.
 template <bool ON>
@@ -219,28 +242,29 @@ __device__ __forceinline__ void sweep(const Sweep& S, const Geom& G, Body&& body
 // Forward stress of one cell, Src/stress.cu:36-101, plus the source of this
 // step at its cell (Src/libCUFD.cu:319-330).
 // ---------------------------------------------------------------------------
-__device__ __forceinline__ void stress_fwd_cell(const Geom& G, const Grp& W, const Grp& P, const Grp& M,
-                                                const Prof& C, const Cell& c, unsigned src_o, float amp) {
+template <class Acc>
+__device__ __forceinline__ void stress_fwd_cell(const Geom& G, const Acc& acc, const Prof& C, const Cell& c,
+                                                unsigned src_o, float amp) {
   const unsigned o = c.o;
   const int z = c.z, x = c.x, y = c.y, p = G.npml;
   const Dv D(G);
-  float dvz_dz = D.z(n_b(W, F_VZ, o, c.zn));
-  float dvx_dx = D.x(n_b(W, F_VX, o, c.xn));
-  float dvy_dy = D.y(n_b(W, F_VY, o, c.yn));
-  float dvx_dz = D.z(n_f(W, F_VX, o, c.zn));
-  float dvx_dy = D.y(n_f(W, F_VX, o, c.yn));
-  float dvz_dx = D.x(n_f(W, F_VZ, o, c.xn));
-  float dvz_dy = D.y(n_f(W, F_VZ, o, c.yn));
-  float dvy_dx = D.x(n_f(W, F_VY, o, c.xn));
-  float dvy_dz = D.z(n_f(W, F_VY, o, c.zn));
-  const float lam = ld(M, M_LAM, o), mu = ld(M, M_MU, o), dt = G.dt;
-  const float szz = ld(W, F_SZZ, o), sxx = ld(W, F_SXX, o), syy = ld(W, F_SYY, o);
-  const float sxz = ld(W, F_SXZ, o), sxy = ld(W, F_SXY, o), syz = ld(W, F_SYZ, o);
+  float dvz_dz = D.z(n_b(acc, GW, F_VZ, AZ));
+  float dvx_dx = D.x(n_b(acc, GW, F_VX, AX));
+  float dvy_dy = D.y(n_b(acc, GW, F_VY, AY));
+  float dvx_dz = D.z(n_f(acc, GW, F_VX, AZ));
+  float dvx_dy = D.y(n_f(acc, GW, F_VX, AY));
+  float dvz_dx = D.x(n_f(acc, GW, F_VZ, AX));
+  float dvz_dy = D.y(n_f(acc, GW, F_VZ, AY));
+  float dvy_dx = D.x(n_f(acc, GW, F_VY, AX));
+  float dvy_dz = D.z(n_f(acc, GW, F_VY, AZ));
+  const float lam = acc.own(GM, M_LAM), mu = acc.own(GM, M_MU), dt = G.dt;
+  const float szz = acc.own(GW, F_SZZ), sxx = acc.own(GW, F_SXX), syy = acc.own(GW, F_SYY);
+  const float sxz = acc.own(GW, F_SXZ), sxy = acc.own(GW, F_SXY), syz = acc.own(GW, F_SYZ);
   const bool pz = in_pml(z, G.nz, p), px = in_pml(x, G.nx, p), py = in_pml(y, G.ny, p);
   float qz0 = 0, qz1 = 0, qz2 = 0, qx0 = 0, qx1 = 0, qx2 = 0, qy0 = 0, qy1 = 0, qy2 = 0;
-  if (pz) { qz0 = ld(P, P_VZ_Z, o); qz1 = ld(P, P_VX_Z, o); qz2 = ld(P, P_VY_Z, o); }
-  if (px) { qx0 = ld(P, P_VX_X, o); qx1 = ld(P, P_VZ_X, o); qx2 = ld(P, P_VY_X, o); }
-  if (py) { qy0 = ld(P, P_VY_Y, o); qy1 = ld(P, P_VZ_Y, o); qy2 = ld(P, P_VX_Y, o); }
+  if (pz) { qz0 = acc.own(GP, P_VZ_Z); qz1 = acc.own(GP, P_VX_Z); qz2 = acc.own(GP, P_VY_Z); }
+  if (px) { qx0 = acc.own(GP, P_VX_X); qx1 = acc.own(GP, P_VZ_X); qx2 = acc.own(GP, P_VY_X); }
+  if (py) { qy0 = acc.own(GP, P_VY_Y); qy1 = acc.own(GP, P_VZ_Y); qy2 = acc.own(GP, P_VX_Y); }
   if (pz) {
     const Pv i = pv(C, 0, false, z), h = pv(C, 0, true, z);
     dvz_dz = cpml(i.b, i.a, i.K, qz0, dvz_dz);
@@ -268,43 +292,41 @@ __device__ __forceinline__ void stress_fwd_cell(const Geom& G, const Grp& W, con
     nsxx = nsxx + amp;
     nsyy = nsyy + amp;
   }
-  st(W, F_SZZ, o, nszz);
-  st(W, F_SXX, o, nsxx);
-  st(W, F_SYY, o, nsyy);
-  st(W, F_SXZ, o, sxz + mu * (dvx_dz + dvz_dx) * dt);
-  st(W, F_SXY, o, sxy + mu * (dvx_dy + dvy_dx) * dt);
-  st(W, F_SYZ, o, syz + mu * (dvy_dz + dvz_dy) * dt);
-  if (pz) { st(P, P_VZ_Z, o, qz0); st(P, P_VX_Z, o, qz1); st(P, P_VY_Z, o, qz2); }
-  if (px) { st(P, P_VX_X, o, qx0); st(P, P_VZ_X, o, qx1); st(P, P_VY_X, o, qx2); }
-  if (py) { st(P, P_VY_Y, o, qy0); st(P, P_VZ_Y, o, qy1); st(P, P_VX_Y, o, qy2); }
+  acc.st(GW, F_SZZ, nszz);
+  acc.st(GW, F_SXX, nsxx);
+  acc.st(GW, F_SYY, nsyy);
+  acc.st(GW, F_SXZ, sxz + mu * (dvx_dz + dvz_dx) * dt);
+  acc.st(GW, F_SXY, sxy + mu * (dvx_dy + dvy_dx) * dt);
+  acc.st(GW, F_SYZ, syz + mu * (dvy_dz + dvz_dy) * dt);
+  if (pz) { acc.st(GP, P_VZ_Z, qz0); acc.st(GP, P_VX_Z, qz1); acc.st(GP, P_VY_Z, qz2); }
+  if (px) { acc.st(GP, P_VX_X, qx0); acc.st(GP, P_VZ_X, qx1); acc.st(GP, P_VY_X, qx2); }
+  if (py) { acc.st(GP, P_VY_Y, qy0); acc.st(GP, P_VZ_Y, qy1); acc.st(GP, P_VX_Y, qy2); }
 }
 
 // ---------------------------------------------------------------------------
 // Forward velocity of one cell, Src/velocity.cu:175-240.
 // ---------------------------------------------------------------------------
-template <int FORM>
-__device__ __forceinline__ void velocity_fwd_cell(const Geom& G, const Grp& W, const Grp& P, const Grp& M,
-                                                  const Prof& C, const Cell& c) {
-  const unsigned o = c.o;
+template <int FORM, class Acc>
+__device__ __forceinline__ void velocity_fwd_cell(const Geom& G, const Acc& acc, const Prof& C, const Cell& c) {
   const int z = c.z, x = c.x, y = c.y, p = G.npml;
   const Dv D(G);
-  float dszz_dz = D.z(n_f(W, F_SZZ, o, c.zn));
-  float dsxz_dx = D.x(n_b(W, F_SXZ, o, c.xn));
-  float dsyz_dy = D.y(n_b(W, F_SYZ, o, c.yn));
-  float dsxz_dz = D.z(n_b(W, F_SXZ, o, c.zn));
-  float dsxx_dx = D.x(n_f(W, F_SXX, o, c.xn));
-  float dsxy_dy = D.y(n_b(W, F_SXY, o, c.yn));
-  float dsyz_dz = D.z(n_b(W, F_SYZ, o, c.zn));
-  float dsxy_dx = D.x(n_sxy_x<FORM>(W, F_SXY, o, c.xn));
-  float dsyy_dy = D.y(n_f(W, F_SYY, o, c.yn));
-  const float byc = ld(M, M_BYC, o), dt = G.dt;
-  const float vz = ld(W, F_VZ, o), vx = ld(W, F_VX, o), vy = ld(W, F_VY, o);
+  float dszz_dz = D.z(n_f(acc, GW, F_SZZ, AZ));
+  float dsxz_dx = D.x(n_b(acc, GW, F_SXZ, AX));
+  float dsyz_dy = D.y(n_b(acc, GW, F_SYZ, AY));
+  float dsxz_dz = D.z(n_b(acc, GW, F_SXZ, AZ));
+  float dsxx_dx = D.x(n_f(acc, GW, F_SXX, AX));
+  float dsxy_dy = D.y(n_b(acc, GW, F_SXY, AY));
+  float dsyz_dz = D.z(n_b(acc, GW, F_SYZ, AZ));
+  float dsxy_dx = D.x(n_sxy_x<FORM>(acc, GW, F_SXY, AX));
+  float dsyy_dy = D.y(n_f(acc, GW, F_SYY, AY));
+  const float byc = acc.own(GM, M_BYC), dt = G.dt;
+  const float vz = acc.own(GW, F_VZ), vx = acc.own(GW, F_VX), vy = acc.own(GW, F_VY);
   const bool pz = in_pml(z, G.nz, p);
   const bool px = in_pml_v<FORM>(x, G.nx, p), py = in_pml_v<FORM>(y, G.ny, p);
   float qz0 = 0, qz1 = 0, qz2 = 0, qx0 = 0, qx1 = 0, qx2 = 0, qy0 = 0, qy1 = 0, qy2 = 0;
-  if (pz) { qz0 = ld(P, P_SZZ_Z, o); qz1 = ld(P, P_SXZ_Z, o); qz2 = ld(P, P_SYZ_Z, o); }
-  if (px) { qx0 = ld(P, P_SXZ_X, o); qx1 = ld(P, P_SXX_X, o); qx2 = ld(P, P_SXY_X, o); }
-  if (py) { qy0 = ld(P, P_SYZ_Y, o); qy1 = ld(P, P_SXY_Y, o); qy2 = ld(P, P_SYY_Y, o); }
+  if (pz) { qz0 = acc.own(GP, P_SZZ_Z); qz1 = acc.own(GP, P_SXZ_Z); qz2 = acc.own(GP, P_SYZ_Z); }
+  if (px) { qx0 = acc.own(GP, P_SXZ_X); qx1 = acc.own(GP, P_SXX_X); qx2 = acc.own(GP, P_SXY_X); }
+  if (py) { qy0 = acc.own(GP, P_SYZ_Y); qy1 = acc.own(GP, P_SXY_Y); qy2 = acc.own(GP, P_SYY_Y); }
   if (pz) {
     const Pv i = pv(C, 0, false, z), h = pv(C, 0, true, z);
     dszz_dz = cpml(h.b, h.a, h.K, qz0, dszz_dz);
@@ -323,12 +345,12 @@ __device__ __forceinline__ void velocity_fwd_cell(const Geom& G, const Grp& W, c
     dsxy_dy = cpml(i.b, i.a, i.K, qy1, dsxy_dy);
     dsyy_dy = cpml(h.b, h.a, h.K, qy2, dsyy_dy);
   }
-  st(W, F_VZ, o, vz + (dszz_dz + dsxz_dx + dsyz_dy) * byc * dt);
-  st(W, F_VX, o, vx + (dsxz_dz + dsxx_dx + dsxy_dy) * byc * dt);
-  st(W, F_VY, o, vy + (dsyz_dz + dsxy_dx + dsyy_dy) * byc * dt);
-  if (pz) { st(P, P_SZZ_Z, o, qz0); st(P, P_SXZ_Z, o, qz1); st(P, P_SYZ_Z, o, qz2); }
-  if (px) { st(P, P_SXZ_X, o, qx0); st(P, P_SXX_X, o, qx1); st(P, P_SXY_X, o, qx2); }
-  if (py) { st(P, P_SYZ_Y, o, qy0); st(P, P_SXY_Y, o, qy1); st(P, P_SYY_Y, o, qy2); }
+  acc.st(GW, F_VZ, vz + (dszz_dz + dsxz_dx + dsyz_dy) * byc * dt);
+  acc.st(GW, F_VX, vx + (dsxz_dz + dsxx_dx + dsxy_dy) * byc * dt);
+  acc.st(GW, F_VY, vy + (dsyz_dz + dsxy_dx + dsyy_dy) * byc * dt);
+  if (pz) { acc.st(GP, P_SZZ_Z, qz0); acc.st(GP, P_SXZ_Z, qz1); acc.st(GP, P_SYZ_Z, qz2); }
+  if (px) { acc.st(GP, P_SXZ_X, qx0); acc.st(GP, P_SXX_X, qx1); acc.st(GP, P_SXY_X, qx2); }
+  if (py) { acc.st(GP, P_SYZ_Y, qy0); acc.st(GP, P_SXY_Y, qy1); acc.st(GP, P_SYY_Y, qy2); }
 }
 
 // ---------------------------------------------------------------------------
@@ -342,10 +364,8 @@ __device__ __forceinline__ void velocity_fwd_cell(const Geom& G, const Grp& W, c
 // RV: reverse part allowed; XZ: the row lies in the x or z C-PML; Y: the
 // wavefront touches a y C-PML slab.
 // ---------------------------------------------------------------------------
-template <int FORM, bool RV, bool XZ, bool Y>
-__device__ __forceinline__ void velocity_bwd_cell(const Geom& G, const Grp& W, const Grp& A, const Grp& P,
-                                                  const Grp& M, const Prof& C, const Cell& c) {
-  const unsigned o = c.o;
+template <int FORM, bool RV, bool XZ, bool Y, class Acc>
+__device__ __forceinline__ void velocity_bwd_cell(const Geom& G, const Acc& acc, const Prof& C, const Cell& c) {
   const int z = c.z, x = c.x, y = c.y, p = G.npml;
   const float dt = G.dt;
   const Dv D(G);
@@ -355,15 +375,15 @@ __device__ __forceinline__ void velocity_bwd_cell(const Geom& G, const Grp& W, c
   // ---- loads
   float r_sz = 0.0f, r_sx = 0.0f, r_sy = 0.0f, vz = 0.0f, vx = 0.0f, vy = 0.0f, grho = 0.0f;
   if (rev) {
-    r_sz = D.z(n_f(W, F_SZZ, o, c.zn)) + D.x(n_b(W, F_SXZ, o, c.xn)) + D.y(n_b(W, F_SYZ, o, c.yn));
-    r_sx = D.z(n_b(W, F_SXZ, o, c.zn)) + D.x(n_f(W, F_SXX, o, c.xn)) + D.y(n_b(W, F_SXY, o, c.yn));
-    r_sy = D.z(n_b(W, F_SYZ, o, c.zn)) + D.x(n_sxy_x<FORM>(W, F_SXY, o, c.xn)) +
-           D.y(n_f(W, F_SYY, o, c.yn));
-    vz = ld(W, F_VZ, o); vx = ld(W, F_VX, o); vy = ld(W, F_VY, o);
-    grho = ld(M, M_GRHO, o);
+    r_sz = D.z(n_f(acc, GW, F_SZZ, AZ)) + D.x(n_b(acc, GW, F_SXZ, AX)) + D.y(n_b(acc, GW, F_SYZ, AY));
+    r_sx = D.z(n_b(acc, GW, F_SXZ, AZ)) + D.x(n_f(acc, GW, F_SXX, AX)) + D.y(n_b(acc, GW, F_SXY, AY));
+    r_sy = D.z(n_b(acc, GW, F_SYZ, AZ)) + D.x(n_sxy_x<FORM>(acc, GW, F_SXY, AX)) +
+           D.y(n_f(acc, GW, F_SYY, AY));
+    vz = acc.own(GW, F_VZ); vx = acc.own(GW, F_VX); vy = acc.own(GW, F_VY);
+    grho = acc.own(GM, M_GRHO);
   }
-  const float lambda = ld(M, M_LAM, o), mu = ld(M, M_MU, o), byc = ld(M, M_BYC, o);
-  const float avz = ld(A, F_VZ, o), avx = ld(A, F_VX, o), avy = ld(A, F_VY, o);
+  const float lambda = acc.own(GM, M_LAM), mu = acc.own(GM, M_MU), byc = acc.own(GM, M_BYC);
+  const float avz = acc.own(GA, F_VZ), avx = acc.own(GA, F_VX), avy = acc.own(GA, F_VY);
   const float ax = XZ ? pr(C, 1, R_A, x) : 0.0f, az = XZ ? pr(C, 0, R_A, z) : 0.0f;
   const float axh = XZ ? pr(C, 1, R_AH, x) : 0.0f, azh = XZ ? pr(C, 0, R_AH, z) : 0.0f;
   const float ay = Y ? pr(C, 2, R_A, y) : 0.0f, ayh = Y ? pr(C, 2, R_AH, y) : 0.0f;
@@ -374,37 +394,37 @@ __device__ __forceinline__ void velocity_bwd_cell(const Geom& G, const Grp& W, c
   const float Ky = Y ? pr(C, 2, R_K, y) : 1.0f;
   const float Kyh = Y ? pr(C, 2, R_KH, y) : 1.0f;
   // vx-dagger terms
-  const float x1 = apsi_f<XZ>(ax, P, P_VX_X, c, c.xn, G.dx);
-  const float x_szz = D.x(n_af(A, F_SZZ, o, c.xn));
-  const float x_sxx = D.x(n_af(A, F_SXX, o, c.xn));
-  const float x_syy = D.x(n_af(A, F_SYY, o, c.xn));
-  const float x6 = apsi_b<XZ>(azh, P, P_VX_Z, c, c.zn, G.dz);
-  const float x_sxz = D.z(n_ab(A, F_SXZ, o, c.zn));
-  const float x8 = apsi_b<Y>(ayh, P, P_VX_Y, c, c.yn, G.dy);
-  const float x_sxy = D.y(n_ab(A, F_SXY, o, c.yn));
+  const float x1 = apsi_f<XZ>(acc, ax, P_VX_X, AX, G.dx);
+  const float x_szz = D.x(n_af(acc, GA, F_SZZ, AX));
+  const float x_sxx = D.x(n_af(acc, GA, F_SXX, AX));
+  const float x_syy = D.x(n_af(acc, GA, F_SYY, AX));
+  const float x6 = apsi_b<XZ>(acc, azh, P_VX_Z, AZ, G.dz);
+  const float x_sxz = D.z(n_ab(acc, GA, F_SXZ, AZ));
+  const float x8 = apsi_b<Y>(acc, ayh, P_VX_Y, AY, G.dy);
+  const float x_sxy = D.y(n_ab(acc, GA, F_SXY, AY));
   // vz-dagger terms
-  const float z1 = apsi_f<XZ>(az, P, P_VZ_Z, c, c.zn, G.dz);
-  const float z_szz = D.z(n_af(A, F_SZZ, o, c.zn));
-  const float z_sxx = D.z(n_af(A, F_SXX, o, c.zn));
-  const float z_syy = D.z(n_af(A, F_SYY, o, c.zn));
-  const float z8 = apsi_b<XZ>(axh, P, P_VZ_X, c, c.xn, G.dx);
-  const float z_sxz = D.x(n_ab(A, F_SXZ, o, c.xn));
-  const float z6 = apsi_b<Y>(ayh, P, P_VZ_Y, c, c.yn, G.dy);
-  const float z_syz = D.y(n_ab(A, F_SYZ, o, c.yn));
+  const float z1 = apsi_f<XZ>(acc, az, P_VZ_Z, AZ, G.dz);
+  const float z_szz = D.z(n_af(acc, GA, F_SZZ, AZ));
+  const float z_sxx = D.z(n_af(acc, GA, F_SXX, AZ));
+  const float z_syy = D.z(n_af(acc, GA, F_SYY, AZ));
+  const float z8 = apsi_b<XZ>(acc, axh, P_VZ_X, AX, G.dx);
+  const float z_sxz = D.x(n_ab(acc, GA, F_SXZ, AX));
+  const float z6 = apsi_b<Y>(acc, ayh, P_VZ_Y, AY, G.dy);
+  const float z_syz = D.y(n_ab(acc, GA, F_SYZ, AY));
   // vy-dagger terms
-  const float y1 = apsi_f<Y>(ay, P, P_VY_Y, c, c.yn, G.dy);
-  const float y_syy = D.y(n_af(A, F_SYY, o, c.yn));
-  const float y_sxx = D.y(n_af(A, F_SXX, o, c.yn));
-  const float y_szz = D.y(n_af(A, F_SZZ, o, c.yn));
-  const float y6 = apsi_b<XZ>(axh, P, P_VY_X, c, c.xn, G.dx);
-  const float y_sxy = D.x(n_ab(A, F_SXY, o, c.xn));
-  const float y8 = apsi_b<XZ>(azh, P, P_VY_Z, c, c.zn, G.dz);
-  const float y_syz = D.z(n_ab(A, F_SYZ, o, c.zn));
+  const float y1 = apsi_f<Y>(acc, ay, P_VY_Y, AY, G.dy);
+  const float y_syy = D.y(n_af(acc, GA, F_SYY, AY));
+  const float y_sxx = D.y(n_af(acc, GA, F_SXX, AY));
+  const float y_szz = D.y(n_af(acc, GA, F_SZZ, AY));
+  const float y6 = apsi_b<XZ>(acc, axh, P_VY_X, AX, G.dx);
+  const float y_sxy = D.x(n_ab(acc, GA, F_SXY, AX));
+  const float y8 = apsi_b<XZ>(acc, azh, P_VY_Z, AZ, G.dz);
+  const float y_syz = D.z(n_ab(acc, GA, F_SYZ, AZ));
   // psi values updated by this step
   float qx0 = 0, qx1 = 0, qx2 = 0, qz0 = 0, qz1 = 0, qz2 = 0, qy0 = 0, qy1 = 0, qy2 = 0;
-  if (px) { qx0 = ld(P, P_SXX_X, o); qx1 = ld(P, P_SXZ_X, o); qx2 = ld(P, P_SXY_X, o); }
-  if (pz) { qz0 = ld(P, P_SXZ_Z, o); qz1 = ld(P, P_SZZ_Z, o); qz2 = ld(P, P_SYZ_Z, o); }
-  if (py) { qy0 = ld(P, P_SXY_Y, o); qy1 = ld(P, P_SYZ_Y, o); qy2 = ld(P, P_SYY_Y, o); }
+  if (px) { qx0 = acc.own(GP, P_SXX_X); qx1 = acc.own(GP, P_SXZ_X); qx2 = acc.own(GP, P_SXY_X); }
+  if (pz) { qz0 = acc.own(GP, P_SXZ_Z); qz1 = acc.own(GP, P_SZZ_Z); qz2 = acc.own(GP, P_SYZ_Z); }
+  if (py) { qy0 = acc.own(GP, P_SXY_Y); qy1 = acc.own(GP, P_SYZ_Y); qy2 = acc.own(GP, P_SYY_Y); }
   // ---- arithmetic
   const float l2m = lambda + 2.0f * mu;
   const float nvx = avx + (x1 + kdiv<XZ>(l2m * x_sxx, Kx) * dt + kdiv<XZ>(lambda * x_szz, Kx) * dt +
@@ -418,34 +438,34 @@ __device__ __forceinline__ void velocity_bwd_cell(const Geom& G, const Grp& W, c
                            kdiv<XZ>(mu, Kzh) * y_syz * dt + y8);
   // ---- stores
   if (rev) {
-    st(W, F_VZ, o, vz - r_sz * byc * dt);
-    st(W, F_VX, o, vx - r_sx * byc * dt);
-    st(W, F_VY, o, vy - r_sy * byc * dt);
+    acc.st(GW, F_VZ, vz - r_sz * byc * dt);
+    acc.st(GW, F_VX, vx - r_sx * byc * dt);
+    acc.st(GW, F_VY, vy - r_sy * byc * dt);
     const float nb2 = -(byc * byc);
-    st(M, M_GRHO, o, grho + -(avz * r_sz * dt * nb2 + avx * r_sx * dt * nb2 + avy * r_sy * dt * nb2));
+    acc.st(GM, M_GRHO, grho + -(avz * r_sz * dt * nb2 + avx * r_sx * dt * nb2 + avy * r_sy * dt * nb2));
   }
-  st(A, F_VX, o, nvx);
-  st(A, F_VZ, o, nvz);
-  st(A, F_VY, o, nvy);
+  acc.st(GA, F_VX, nvx);
+  acc.st(GA, F_VZ, nvz);
+  acc.st(GA, F_VY, nvy);
   if (px) {
     const float b = pr(C, 1, R_B, x), bh = pr(C, 1, R_BH, x);
-    st(P, P_SXX_X, o, bh * qx0 + byc * nvx * dt);
-    st(P, P_SXZ_X, o, b * qx1 + byc * nvz * dt);
-    st(P, P_SXY_X, o, b * qx2 + byc * nvy * dt);
+    acc.st(GP, P_SXX_X, bh * qx0 + byc * nvx * dt);
+    acc.st(GP, P_SXZ_X, b * qx1 + byc * nvz * dt);
+    acc.st(GP, P_SXY_X, b * qx2 + byc * nvy * dt);
   }
   if (pz) {
     const float b = pr(C, 0, R_B, z), bh = pr(C, 0, R_BH, z);
-    st(P, P_SXZ_Z, o, b * qz0 + byc * nvx * dt);
-    st(P, P_SZZ_Z, o, bh * qz1 + byc * nvz * dt);
+    acc.st(GP, P_SXZ_Z, b * qz0 + byc * nvx * dt);
+    acc.st(GP, P_SZZ_Z, bh * qz1 + byc * nvz * dt);
     // Src/velocity_adj.cu:285 decays psi_dsyz_dz with b_z_half (the forward uses b_z)
-    st(P, P_SYZ_Z, o, (FORM == 1 ? b : bh) * qz2 + byc * nvy * dt);
+    acc.st(GP, P_SYZ_Z, (FORM == 1 ? b : bh) * qz2 + byc * nvy * dt);
   }
   if (py) {
     const float b = pr(C, 2, R_B, y), bh = pr(C, 2, R_BH, y);
-    st(P, P_SXY_Y, o, b * qy0 + byc * nvx * dt);
-    st(P, P_SYZ_Y, o, b * qy1 + byc * nvz * dt);
+    acc.st(GP, P_SXY_Y, b * qy0 + byc * nvx * dt);
+    acc.st(GP, P_SYZ_Y, b * qy1 + byc * nvz * dt);
     // Src/velocity_adj.cu:288 decays psi_dsyy_dy with b_y (the forward uses b_y_half)
-    st(P, P_SYY_Y, o, (FORM == 1 ? bh : b) * qy2 + byc * nvy * dt);
+    acc.st(GP, P_SYY_Y, (FORM == 1 ? bh : b) * qy2 + byc * nvy * dt);
   }
 }
 
@@ -456,10 +476,9 @@ __device__ __forceinline__ void velocity_bwd_cell(const Geom& G, const Grp& W, c
 // [2, n-3] the adjoint stress (Src/stress_adj.cu:1-146). The imaging terms
 // read the adjoint stress before the adjoint part updates it.
 // ---------------------------------------------------------------------------
-template <bool RV, bool XZ, bool Y>
-__device__ __forceinline__ void stress_bwd_cell(const Geom& G, const Grp& W, const Grp& A, const Grp& P,
-                                                const Grp& M, const Prof& C, const Cell& c, unsigned src_o,
-                                                float amp) {
+template <bool RV, bool XZ, bool Y, class Acc>
+__device__ __forceinline__ void stress_bwd_cell(const Geom& G, const Acc& acc, const Prof& C, const Cell& c,
+                                                unsigned src_o, float amp) {
   const unsigned o = c.o;
   const int z = c.z, x = c.x, y = c.y, p = G.npml;
   const float dt = G.dt;
@@ -467,27 +486,27 @@ __device__ __forceinline__ void stress_bwd_cell(const Geom& G, const Grp& W, con
   const bool py = Y && in_pml(y, G.ny, p);
   const bool px = XZ && in_pml(x, G.nx, p), pz = XZ && in_pml(z, G.nz, p);
   const bool rev = RV && !XZ && !py;
-  const float lambda = ld(M, M_LAM, o), mu = ld(M, M_MU, o), byc = ld(M, M_BYC, o);
+  const float lambda = acc.own(GM, M_LAM), mu = acc.own(GM, M_MU), byc = acc.own(GM, M_BYC);
   // ---- loads of the reverse part
   float dvz_dz = 0, dvx_dx = 0, dvy_dy = 0, dvx_dz = 0, dvx_dy = 0, dvz_dx = 0, dvz_dy = 0, dvy_dx = 0, dvy_dz = 0;
   float szz = 0, sxx = 0, syy = 0, sxz = 0, sxy = 0, syz = 0, glam = 0, gmu = 0;
   if (rev) {
-    dvz_dz = D.z(n_b(W, F_VZ, o, c.zn));
-    dvx_dx = D.x(n_b(W, F_VX, o, c.xn));
-    dvy_dy = D.y(n_b(W, F_VY, o, c.yn));
-    dvx_dz = D.z(n_f(W, F_VX, o, c.zn));
-    dvx_dy = D.y(n_f(W, F_VX, o, c.yn));
-    dvz_dx = D.x(n_f(W, F_VZ, o, c.xn));
-    dvz_dy = D.y(n_f(W, F_VZ, o, c.yn));
-    dvy_dx = D.x(n_f(W, F_VY, o, c.xn));
-    dvy_dz = D.z(n_f(W, F_VY, o, c.zn));
-    szz = ld(W, F_SZZ, o); sxx = ld(W, F_SXX, o); syy = ld(W, F_SYY, o);
-    sxz = ld(W, F_SXZ, o); sxy = ld(W, F_SXY, o); syz = ld(W, F_SYZ, o);
-    glam = ld(M, M_GLAM, o); gmu = ld(M, M_GMU, o);
+    dvz_dz = D.z(n_b(acc, GW, F_VZ, AZ));
+    dvx_dx = D.x(n_b(acc, GW, F_VX, AX));
+    dvy_dy = D.y(n_b(acc, GW, F_VY, AY));
+    dvx_dz = D.z(n_f(acc, GW, F_VX, AZ));
+    dvx_dy = D.y(n_f(acc, GW, F_VX, AY));
+    dvz_dx = D.x(n_f(acc, GW, F_VZ, AX));
+    dvz_dy = D.y(n_f(acc, GW, F_VZ, AY));
+    dvy_dx = D.x(n_f(acc, GW, F_VY, AX));
+    dvy_dz = D.z(n_f(acc, GW, F_VY, AZ));
+    szz = acc.own(GW, F_SZZ); sxx = acc.own(GW, F_SXX); syy = acc.own(GW, F_SYY);
+    sxz = acc.own(GW, F_SXZ); sxy = acc.own(GW, F_SXY); syz = acc.own(GW, F_SYZ);
+    glam = acc.own(GM, M_GLAM); gmu = acc.own(GM, M_GMU);
   }
   // ---- loads of the adjoint part
-  const float aszz = ld(A, F_SZZ, o), asxx = ld(A, F_SXX, o), asyy = ld(A, F_SYY, o);
-  const float asxz = ld(A, F_SXZ, o), asxy = ld(A, F_SXY, o), asyz = ld(A, F_SYZ, o);
+  const float aszz = acc.own(GA, F_SZZ), asxx = acc.own(GA, F_SXX), asyy = acc.own(GA, F_SYY);
+  const float asxz = acc.own(GA, F_SXZ), asxy = acc.own(GA, F_SXY), asyz = acc.own(GA, F_SYZ);
   const float ax = XZ ? pr(C, 1, R_A, x) : 0.0f, az = XZ ? pr(C, 0, R_A, z) : 0.0f;
   const float axh = XZ ? pr(C, 1, R_AH, x) : 0.0f, azh = XZ ? pr(C, 0, R_AH, z) : 0.0f;
   const float ay = Y ? pr(C, 2, R_A, y) : 0.0f, ayh = Y ? pr(C, 2, R_AH, y) : 0.0f;
@@ -497,29 +516,29 @@ __device__ __forceinline__ void stress_bwd_cell(const Geom& G, const Grp& W, con
   const float Kxh = XZ ? pr(C, 1, R_KH, x) : 1.0f;
   const float Kzh = XZ ? pr(C, 0, R_KH, z) : 1.0f;
   const float Kyh = Y ? pr(C, 2, R_KH, y) : 1.0f;
-  const float xz1 = apsi_f<XZ>(ax, P, P_SXZ_X, c, c.xn, G.dx);
-  const float xz_vz = D.x(n_af(A, F_VZ, o, c.xn));
-  const float xz3 = apsi_f<XZ>(az, P, P_SXZ_Z, c, c.zn, G.dz);
-  const float xz_vx = D.z(n_af(A, F_VX, o, c.zn));
-  const float xy1 = apsi_f<XZ>(ax, P, P_SXY_X, c, c.xn, G.dx);
-  const float xy_vy = D.x(n_af(A, F_VY, o, c.xn));
-  const float xy3 = apsi_f<Y>(ay, P, P_SXY_Y, c, c.yn, G.dy);
-  const float xy_vx = D.y(n_af(A, F_VX, o, c.yn));
-  const float yz1 = apsi_f<Y>(ay, P, P_SYZ_Y, c, c.yn, G.dy);
-  const float yz_vz = D.y(n_af(A, F_VZ, o, c.yn));
-  const float yz3 = apsi_f<XZ>(az, P, P_SYZ_Z, c, c.zn, G.dz);
-  const float yz_vy = D.z(n_af(A, F_VY, o, c.zn));
-  const float txx = apsi_b<XZ>(axh, P, P_SXX_X, c, c.xn, G.dx);
-  const float xx_vx = D.x(n_ab(A, F_VX, o, c.xn));
-  const float tzz = apsi_b<XZ>(azh, P, P_SZZ_Z, c, c.zn, G.dz);
-  const float zz_vz = D.z(n_ab(A, F_VZ, o, c.zn));
-  const float tyy = apsi_b<Y>(ayh, P, P_SYY_Y, c, c.yn, G.dy);
-  const float yy_vy = D.y(n_ab(A, F_VY, o, c.yn));
+  const float xz1 = apsi_f<XZ>(acc, ax, P_SXZ_X, AX, G.dx);
+  const float xz_vz = D.x(n_af(acc, GA, F_VZ, AX));
+  const float xz3 = apsi_f<XZ>(acc, az, P_SXZ_Z, AZ, G.dz);
+  const float xz_vx = D.z(n_af(acc, GA, F_VX, AZ));
+  const float xy1 = apsi_f<XZ>(acc, ax, P_SXY_X, AX, G.dx);
+  const float xy_vy = D.x(n_af(acc, GA, F_VY, AX));
+  const float xy3 = apsi_f<Y>(acc, ay, P_SXY_Y, AY, G.dy);
+  const float xy_vx = D.y(n_af(acc, GA, F_VX, AY));
+  const float yz1 = apsi_f<Y>(acc, ay, P_SYZ_Y, AY, G.dy);
+  const float yz_vz = D.y(n_af(acc, GA, F_VZ, AY));
+  const float yz3 = apsi_f<XZ>(acc, az, P_SYZ_Z, AZ, G.dz);
+  const float yz_vy = D.z(n_af(acc, GA, F_VY, AZ));
+  const float txx = apsi_b<XZ>(acc, axh, P_SXX_X, AX, G.dx);
+  const float xx_vx = D.x(n_ab(acc, GA, F_VX, AX));
+  const float tzz = apsi_b<XZ>(acc, azh, P_SZZ_Z, AZ, G.dz);
+  const float zz_vz = D.z(n_ab(acc, GA, F_VZ, AZ));
+  const float tyy = apsi_b<Y>(acc, ayh, P_SYY_Y, AY, G.dy);
+  const float yy_vy = D.y(n_ab(acc, GA, F_VY, AY));
   // psi values updated by this step
   float qx0 = 0, qx1 = 0, qx2 = 0, qz0 = 0, qz1 = 0, qz2 = 0, qy0 = 0, qy1 = 0, qy2 = 0;
-  if (px) { qx0 = ld(P, P_VZ_X, o); qx1 = ld(P, P_VY_X, o); qx2 = ld(P, P_VX_X, o); }
-  if (pz) { qz0 = ld(P, P_VX_Z, o); qz1 = ld(P, P_VY_Z, o); qz2 = ld(P, P_VZ_Z, o); }
-  if (py) { qy0 = ld(P, P_VX_Y, o); qy1 = ld(P, P_VZ_Y, o); qy2 = ld(P, P_VY_Y, o); }
+  if (px) { qx0 = acc.own(GP, P_VZ_X); qx1 = acc.own(GP, P_VY_X); qx2 = acc.own(GP, P_VX_X); }
+  if (pz) { qz0 = acc.own(GP, P_VX_Z); qz1 = acc.own(GP, P_VY_Z); qz2 = acc.own(GP, P_VZ_Z); }
+  if (py) { qy0 = acc.own(GP, P_VX_Y); qy1 = acc.own(GP, P_VZ_Y); qy2 = acc.own(GP, P_VY_Y); }
   // ---- arithmetic
   const float l2m = lambda + 2.0f * mu;
   const float nsxz =
@@ -536,39 +555,39 @@ __device__ __forceinline__ void stress_bwd_cell(const Geom& G, const Grp& W, con
     if (o == src_o) {
       szz -= amp; sxx -= amp; syy -= amp;
     }
-    st(W, F_SZZ, o, szz - (l2m * dvz_dz + lambda * dvx_dx + lambda * dvy_dy) * dt);
-    st(W, F_SXX, o, sxx - (lambda * dvz_dz + l2m * dvx_dx + lambda * dvy_dy) * dt);
-    st(W, F_SYY, o, syy - (lambda * dvz_dz + lambda * dvx_dx + l2m * dvy_dy) * dt);
-    st(W, F_SXZ, o, sxz - mu * (dvx_dz + dvz_dx) * dt);
-    st(W, F_SXY, o, sxy - mu * (dvx_dy + dvy_dx) * dt);
-    st(W, F_SYZ, o, syz - mu * (dvy_dz + dvz_dy) * dt);
-    st(M, M_GLAM, o, glam + -(aszz + asxx + asyy) * (dvz_dz + dvx_dx + dvy_dy) * dt);
-    st(M, M_GMU, o, gmu + -(2.0f * aszz * dvz_dz + 2.0f * asxx * dvx_dx + 2.0f * asyy * dvy_dy +
+    acc.st(GW, F_SZZ, szz - (l2m * dvz_dz + lambda * dvx_dx + lambda * dvy_dy) * dt);
+    acc.st(GW, F_SXX, sxx - (lambda * dvz_dz + l2m * dvx_dx + lambda * dvy_dy) * dt);
+    acc.st(GW, F_SYY, syy - (lambda * dvz_dz + lambda * dvx_dx + l2m * dvy_dy) * dt);
+    acc.st(GW, F_SXZ, sxz - mu * (dvx_dz + dvz_dx) * dt);
+    acc.st(GW, F_SXY, sxy - mu * (dvx_dy + dvy_dx) * dt);
+    acc.st(GW, F_SYZ, syz - mu * (dvy_dz + dvz_dy) * dt);
+    acc.st(GM, M_GLAM, glam + -(aszz + asxx + asyy) * (dvz_dz + dvx_dx + dvy_dy) * dt);
+    acc.st(GM, M_GMU, gmu + -(2.0f * aszz * dvz_dz + 2.0f * asxx * dvx_dx + 2.0f * asyy * dvy_dy +
                             asxz * (dvx_dz + dvz_dx) + asyz * (dvy_dz + dvz_dy) + asxy * (dvy_dx + dvx_dy)) * dt);
   }
-  st(A, F_SXZ, o, nsxz);
-  st(A, F_SXY, o, nsxy);
-  st(A, F_SYZ, o, nsyz);
-  st(A, F_SXX, o, nsxx);
-  st(A, F_SZZ, o, nszz);
-  st(A, F_SYY, o, nsyy);
+  acc.st(GA, F_SXZ, nsxz);
+  acc.st(GA, F_SXY, nsxy);
+  acc.st(GA, F_SYZ, nsyz);
+  acc.st(GA, F_SXX, nsxx);
+  acc.st(GA, F_SZZ, nszz);
+  acc.st(GA, F_SYY, nsyy);
   if (px) {
     const float b = pr(C, 1, R_B, x), bh = pr(C, 1, R_BH, x);
-    st(P, P_VZ_X, o, bh * qx0 + nsxz * mu * dt);
-    st(P, P_VY_X, o, bh * qx1 + nsxy * mu * dt);
-    st(P, P_VX_X, o, b * qx2 + lambda * nszz * dt + l2m * nsxx * dt + lambda * nsyy * dt);
+    acc.st(GP, P_VZ_X, bh * qx0 + nsxz * mu * dt);
+    acc.st(GP, P_VY_X, bh * qx1 + nsxy * mu * dt);
+    acc.st(GP, P_VX_X, b * qx2 + lambda * nszz * dt + l2m * nsxx * dt + lambda * nsyy * dt);
   }
   if (pz) {
     const float b = pr(C, 0, R_B, z), bh = pr(C, 0, R_BH, z);
-    st(P, P_VX_Z, o, bh * qz0 + nsxz * mu * dt);
-    st(P, P_VY_Z, o, bh * qz1 + nsyz * mu * dt);
-    st(P, P_VZ_Z, o, b * qz2 + l2m * nszz * dt + lambda * nsxx * dt + lambda * nsyy * dt);
+    acc.st(GP, P_VX_Z, bh * qz0 + nsxz * mu * dt);
+    acc.st(GP, P_VY_Z, bh * qz1 + nsyz * mu * dt);
+    acc.st(GP, P_VZ_Z, b * qz2 + l2m * nszz * dt + lambda * nsxx * dt + lambda * nsyy * dt);
   }
   if (py) {
     const float b = pr(C, 2, R_B, y), bh = pr(C, 2, R_BH, y);
-    st(P, P_VX_Y, o, bh * qy0 + nsxy * mu * dt);
-    st(P, P_VZ_Y, o, bh * qy1 + nsyz * mu * dt);
-    st(P, P_VY_Y, o, b * qy2 + lambda * nszz * dt + lambda * nsxx * dt + l2m * nsyy * dt);
+    acc.st(GP, P_VX_Y, bh * qy0 + nsxy * mu * dt);
+    acc.st(GP, P_VZ_Y, bh * qy1 + nsyz * mu * dt);
+    acc.st(GP, P_VY_Y, b * qy2 + lambda * nszz * dt + lambda * nsxx * dt + l2m * nsyy * dt);
   }
 }
 
@@ -595,33 +614,35 @@ __device__ __forceinline__ unsigned source_off(const Source& S) {
 // ---------------------------------------------------------------------------
 // Kernels.
 // ---------------------------------------------------------------------------
+__device__ __forceinline__ FlatAcc flat_acc(const Geom& G, const Groups& F, const Cell& c) {
+  return FlatAcc{group(F.w, F_WAVE_COUNT, G.fs), group(F.a, F_WAVE_COUNT, G.fs), group(F.p, P_PSI_COUNT, G.fs),
+                 group(F.m, M_COUNT, G.fs), c.o, {c.zn, c.xn, c.yn}};
+}
+
 __global__ __launch_bounds__(256) EFWI_FWD_ATTR void k_stress_fwd(Sweep Sw, Geom G, Groups F, Prof C, Source S, int it) {
-  const Grp W = group(F.w, F_WAVE_COUNT, G.fs), P = group(F.p, P_PSI_COUNT, G.fs), M = group(F.m, M_COUNT, G.fs);
   const unsigned so = source_off(S);
   const float amp = source_amp(S, G, it);
-  sweep(Sw, G, [&](const Cell& c) { stress_fwd_cell(G, W, P, M, C, c, so, amp); });
+  sweep(Sw, G, [&](const Cell& c) { stress_fwd_cell(G, flat_acc(G, F, c), C, c, so, amp); });
 }
 
 template <int FORM>
 __global__ __launch_bounds__(256) EFWI_FWD_ATTR void k_velocity_fwd(Sweep Sw, Geom G, Groups F, Prof C) {
-  const Grp W = group(F.w, F_WAVE_COUNT, G.fs), P = group(F.p, P_PSI_COUNT, G.fs), M = group(F.m, M_COUNT, G.fs);
-  sweep(Sw, G, [&](const Cell& c) { velocity_fwd_cell<FORM>(G, W, P, M, C, c); });
+  sweep(Sw, G, [&](const Cell& c) { velocity_fwd_cell<FORM>(G, flat_acc(G, F, c), C, c); });
 }
 
 // Backward velocity step (RV = true): reverse velocity + density imaging
 // (interior), then the adjoint velocity ([2, n-3]); RV = false: adjoint only.
 template <int FORM, bool RV>
 __global__ __launch_bounds__(256) EFWI_BWD_ATTR void k_velocity_bwd(Sweep Sw, Geom G, Groups F, Prof C) {
-  const Grp W = group(F.w, F_WAVE_COUNT, G.fs), A = group(F.a, F_WAVE_COUNT, G.fs);
-  const Grp P = group(F.p, P_PSI_COUNT, G.fs), M = group(F.m, M_COUNT, G.fs);
   sweep(Sw, G, [&](const Cell& c) {
+    const FlatAcc acc = flat_acc(G, F, c);
     const Flags f = flags_of(G, c);
     if (f.xz) {
-      if (f.y) velocity_bwd_cell<FORM, RV, true, true>(G, W, A, P, M, C, c);
-      else velocity_bwd_cell<FORM, RV, true, false>(G, W, A, P, M, C, c);
+      if (f.y) velocity_bwd_cell<FORM, RV, true, true>(G, acc, C, c);
+      else velocity_bwd_cell<FORM, RV, true, false>(G, acc, C, c);
     } else {
-      if (f.y) velocity_bwd_cell<FORM, RV, false, true>(G, W, A, P, M, C, c);
-      else velocity_bwd_cell<FORM, RV, false, false>(G, W, A, P, M, C, c);
+      if (f.y) velocity_bwd_cell<FORM, RV, false, true>(G, acc, C, c);
+      else velocity_bwd_cell<FORM, RV, false, false>(G, acc, C, c);
     }
   });
 }
@@ -631,18 +652,17 @@ __global__ __launch_bounds__(256) EFWI_BWD_ATTR void k_velocity_bwd(Sweep Sw, Ge
 // RV = false: adjoint only.
 template <bool RV>
 __global__ __launch_bounds__(256) EFWI_BWD_ATTR void k_stress_bwd(Sweep Sw, Geom G, Groups F, Prof C, Source S, int it) {
-  const Grp W = group(F.w, F_WAVE_COUNT, G.fs), A = group(F.a, F_WAVE_COUNT, G.fs);
-  const Grp P = group(F.p, P_PSI_COUNT, G.fs), M = group(F.m, M_COUNT, G.fs);
   const unsigned so = RV ? source_off(S) : 0xFFFFFFFFu;
   const float amp = RV ? source_amp(S, G, it) : 0.0f;
   sweep(Sw, G, [&](const Cell& c) {
+    const FlatAcc acc = flat_acc(G, F, c);
     const Flags f = flags_of(G, c);
     if (f.xz) {
-      if (f.y) stress_bwd_cell<RV, true, true>(G, W, A, P, M, C, c, so, amp);
-      else stress_bwd_cell<RV, true, false>(G, W, A, P, M, C, c, so, amp);
+      if (f.y) stress_bwd_cell<RV, true, true>(G, acc, C, c, so, amp);
+      else stress_bwd_cell<RV, true, false>(G, acc, C, c, so, amp);
     } else {
-      if (f.y) stress_bwd_cell<RV, false, true>(G, W, A, P, M, C, c, so, amp);
-      else stress_bwd_cell<RV, false, false>(G, W, A, P, M, C, c, so, amp);
+      if (f.y) stress_bwd_cell<RV, false, true>(G, acc, C, c, so, amp);
+      else stress_bwd_cell<RV, false, false>(G, acc, C, c, so, amp);
     }
   });
 }
