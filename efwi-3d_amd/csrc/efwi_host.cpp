@@ -647,8 +647,8 @@ int Ctx::autotune(int reps, hipStream_t s, double* report) {
   HIP_TRY(hipSetDevice(cfg.device));
   if (!model_set) return fail(EFWI_ERR_STATE, "efwi_set_model was not called");
   if (reps < 1) reps = 1;
-  static const Tile cands[] = {{1, 1}, {1, 2}, {1, 4}, {1, 8}, {2, 1}, {2, 2}, {2, 4}, {2, 8},
-                               {4, 1}, {4, 2}, {4, 4}, {4, 8}};
+  static const Tile cands[] = {{1, 1}, {2, 1}, {4, 1}, {1, 2}, {1, 4}, {2, 2},
+                               {4, 16}, {4, 32}, {4, 64}, {8, 16}, {8, 32}, {8, 64}};
   const int ncand = sizeof(cands) / sizeof(cands[0]);
   zero_wave(W, N, s);
   zero_wave(A, N, s);
@@ -791,8 +791,11 @@ int efwi_profile_read(efwi_ctx* ctx, double* ms, long long* launches, int reset)
 int efwi_set_tile(efwi_ctx* ctx, int kernel, int rows, int zdepth) {
   if (!ctx) return fail(EFWI_ERR_ARG, "NULL context");
   if (kernel < 0 || kernel > efwi::K_STRESS_ADJ) return fail(EFWI_ERR_ARG, "kernel index out of range");
-  if (rows != 1 && rows != 2 && rows != 4) return fail(EFWI_ERR_ARG, "rows must be 1, 2 or 4");
-  if (zdepth < 1 || zdepth > 64) return fail(EFWI_ERR_ARG, "zdepth must be in [1, 64]");
+  if (zdepth < 1 || zdepth > 256) return fail(EFWI_ERR_ARG, "zdepth must be in [1, 256]");
+  if (zdepth < 16 && rows != 1 && rows != 2 && rows != 4)
+    return fail(EFWI_ERR_ARG, "rows must be 1, 2 or 4 (flat sweep, zdepth < 16)");
+  if (zdepth >= 16 && rows != 1 && rows != 2 && rows != 4 && rows != 8)
+    return fail(EFWI_ERR_ARG, "rows must be 1, 2, 4 or 8 (z-marching sweep, zdepth >= 16)");
   reinterpret_cast<Ctx*>(ctx)->tiles[kernel] = efwi::Tile{rows, zdepth};
   return EFWI_OK;
 }

@@ -667,6 +667,214 @@ __global__ __launch_bounds__(256) EFWI_BWD_ATTR void k_stress_bwd(Sweep Sw, Geom
   });
 }
 
+// ---------------------------------------------------------------------------
+// z-marching sweep. A workgroup of 64 x R threads owns R rows x 64 y-cells and
+// walks a chunk of planes in z. Fields with z derivatives keep their column's
+// planes z-2..z+2 in registers (a queue refilled one plane ahead), fields with
+// x or y derivatives are staged per plane in an LDS tile of (R+4) x 68 values
+// (the tile plus a 2-cell halo), and every other value (own-cell fields, psi
+// in the C-PML) is a buffer load as in the flat sweep. Each field value then
+// crosses the memory pipeline about once per cell instead of once per
+// neighbour. The cell functions are the flat sweep's, reading through the
+// accessor below, so the arithmetic is the same expression for expression.
+// ---------------------------------------------------------------------------
+constexpr int kTileW = 68;  // 64 + 2 halo columns on each side
+
+// Staged fields of each kernel: slot s = (group G(s), field K(s)) with a z
+// queue index Q(s) (or -1) and an LDS tile index L(s) (or -1).
+struct StressFwdSlots {
+  static constexpr int N = 3, NQ = 3, NL = 3;
+  __host__ __device__ static constexpr int G(int) { return GW; }
+  __host__ __device__ static constexpr int K(int s) { return s == 0 ? F_VZ : s == 1 ? F_VX : F_VY; }
+  __host__ __device__ static constexpr int Q(int s) { return s; }
+  __host__ __device__ static constexpr int L(int s) { return s; }
+};
+struct VelocityFwdSlots {  // szz (z), sxz (x,z), syz (y,z), sxx (x), sxy (x,y), syy (y)
+  static constexpr int N = 6, NQ = 3, NL = 5;
+  __host__ __device__ static constexpr int G(int) { return GW; }
+  __host__ __device__ static constexpr int K(int s) {
+    return s == 0 ? F_SZZ : s == 1 ? F_SXZ : s == 2 ? F_SYZ : s == 3 ? F_SXX : s == 4 ? F_SXY : F_SYY;
+  }
+  __host__ __device__ static constexpr int Q(int s) { return s < 3 ? s : -1; }
+  __host__ __device__ static constexpr int L(int s) { return s == 0 ? -1 : s - 1; }
+};
+struct StressBwdSlots {  // forward and adjoint v, all three axes each
+  static constexpr int N = 6, NQ = 6, NL = 6;
+  __host__ __device__ static constexpr int G(int s) { return s < 3 ? GW : GA; }
+  __host__ __device__ static constexpr int K(int s) { return s % 3 == 0 ? F_VZ : s % 3 == 1 ? F_VX : F_VY; }
+  __host__ __device__ static constexpr int Q(int s) { return s; }
+  __host__ __device__ static constexpr int L(int s) { return s; }
+};
+// forward sigma (reverse part): szz (z), sxz (x,z), syz (y,z), sxx (x), sxy (x,y), syy (y);
+// adjoint sigma: szz, sxx, syy (x,y,z), sxz (x,z), syz (y,z), sxy (x,y)
+struct VelocityBwdSlots {
+  static constexpr int N = 12, NQ = 8, NL = 11;
+  __host__ __device__ static constexpr int G(int s) { return s < 6 ? GW : GA; }
+  __host__ __device__ static constexpr int K(int s) {
+    return s == 0 ? F_SZZ : s == 1 ? F_SXZ : s == 2 ? F_SYZ : s == 3 ? F_SXX : s == 4 ? F_SXY : s == 5 ? F_SYY
+         : s == 6 ? F_SZZ : s == 7 ? F_SXX : s == 8 ? F_SYY : s == 9 ? F_SXZ : s == 10 ? F_SYZ : F_SXY;
+  }
+  __host__ __device__ static constexpr int Q(int s) { return s < 3 ? s : s < 6 ? -1 : s < 11 ? s - 3 : -1; }
+  __host__ __device__ static constexpr int L(int s) { return s == 0 ? -1 : s - 1; }
+};
+template <class S>
+__host__ __device__ constexpr int slot_of(int g, int k) {
+  for (int s = 0; s < S::N; s++)
+    if (S::G(s) == g && S::K(s) == k) return s;
+  return -1;
+}
+
+template <class S, int R>
+struct MarchAcc {
+  FlatAcc f;
+  const float (*q)[5];  // z queues of this thread's column
+  const float* t;       // this cell in LDS tile 0
+  __device__ __forceinline__ float own(int g, int k) const { return at(g, k, AZ, 0); }
+  __device__ __forceinline__ float at(int g, int k, int ax, int d) const {
+    const int s = slot_of<S>(g, k);
+    if (s < 0) return f.at(g, k, ax, d);
+    if ((ax == AZ || d == 0) && S::Q(s) >= 0) return q[S::Q(s)][2 + d];
+    if ((ax == AZ && d != 0) || S::L(s) < 0) return f.at(g, k, ax, d);  // not staged along this axis
+    const float* p = t + S::L(s) * (R + 4) * kTileW;
+    return ax == AX ? p[d * kTileW] : ax == AY ? p[d] : p[0];
+  }
+  __device__ __forceinline__ void st(int g, int k, float v) const { f.st(g, k, v); }
+};
+
+template <class S, int R, typename Body>
+__device__ __forceinline__ void march(const Sweep& Sw, const Geom& G, const Groups& F, Body&& body) {
+  __shared__ float lds[S::NL * (R + 4) * kTileW];
+  const Grp gw = group(F.w, F_WAVE_COUNT, G.fs), ga = group(F.a, F_WAVE_COUNT, G.fs);
+  const Grp gp = group(F.p, P_PSI_COUNT, G.fs), gm = group(F.m, M_COUNT, G.fs);
+  auto grp = [&](int g) -> const Grp& { return g == GW ? gw : g == GA ? ga : g == GP ? gp : gm; };
+  const int b = blockIdx.x;
+  const int xcd = b & 7, j = b >> 3;
+  const int per_plane = Sw.rg_max * Sw.nyb;
+  const int zc = j / per_plane;
+  const int r = j - zc * per_plane;
+  const int rg = r / Sw.nyb;
+  const int yb = r - rg * Sw.nyb;
+  const int rows = Sw.rows_base + (xcd < Sw.rows_rem ? 1 : 0);
+  const int row = __builtin_amdgcn_readfirstlane((int)threadIdx.y);  // blockDim.x == 64
+  const int lane = (int)threadIdx.x;
+  const int x_base = Sw.x0 + xcd * Sw.rows_base + min(xcd, Sw.rows_rem) + rg * R;
+  const int y_base = yb * 64;
+  Cell c;
+  c.x = x_base + row;
+  c.y = y_base + lane;
+  const bool active = rg * R + row < rows && c.y >= Sw.y0 && c.y <= Sw.y1;
+  const unsigned sx = 4u * (unsigned)G.py, sz = 4u * (unsigned)G.pxy;
+  // this thread's column, clamped into the grid (inactive threads stage real values)
+  const int xc = min(c.x, G.nx - 1), yc = min(c.y, G.ny - 1);
+  const unsigned col = 4u * ((unsigned)xc * (unsigned)G.py + (unsigned)yc);
+  const int za = Sw.z0 + zc * Sw.Z;
+  const int zb = min(za + Sw.Z, Sw.z0 + Sw.nz);
+  float q[S::NQ > 0 ? S::NQ : 1][5], nxt[S::NQ > 0 ? S::NQ : 1];
+#pragma unroll
+  for (int s = 0; s < S::N; s++)
+    if (S::Q(s) >= 0)
+#pragma unroll
+      for (int k = 0; k < 5; k++) q[S::Q(s)][k] = ld(grp(S::G(s)), S::K(s), (unsigned)(za - 2 + k) * sz + col);
+  const int tid = row * 64 + lane;
+  constexpr int kHalo = 4 * kTileW + 4 * R;
+  float* const me = lds + (row + 2) * kTileW + lane + 2;
+#pragma unroll 1
+  for (int z = za; z < zb; z++) {
+    const unsigned plane = (unsigned)z * sz;
+    // refill of the queues for the next plane
+    if (z + 1 < zb) {
+#pragma unroll
+      for (int s = 0; s < S::N; s++)
+        if (S::Q(s) >= 0) nxt[S::Q(s)] = ld(grp(S::G(s)), S::K(s), plane + 3 * sz + col);
+    }
+    // stage this plane: own column, then the halo
+#pragma unroll
+    for (int s = 0; s < S::N; s++)
+      if (S::L(s) >= 0)
+        me[S::L(s) * (R + 4) * kTileW] =
+            S::Q(s) >= 0 ? q[S::Q(s)][2] : ld(grp(S::G(s)), S::K(s), plane + col);
+    for (int h = tid; h < kHalo; h += 64 * R) {
+      int tr, tc;
+      if (h < 4 * kTileW) {
+        const int hr = h / kTileW;
+        tr = hr < 2 ? hr : R + hr;
+        tc = h - hr * kTileW;
+      } else {
+        const int e = h - 4 * kTileW;
+        tr = 2 + (e >> 2);
+        tc = (e & 3) < 2 ? (e & 3) : 62 + (e & 3);
+      }
+      const int hx = min(max(x_base - 2 + tr, 0), G.nx - 1);
+      const int hy = min(max(y_base - 2 + tc, 0), G.ny - 1);
+      const unsigned off = plane + 4u * ((unsigned)hx * (unsigned)G.py + (unsigned)hy);
+#pragma unroll
+      for (int s = 0; s < S::N; s++)
+        if (S::L(s) >= 0) lds[S::L(s) * (R + 4) * kTileW + tr * kTileW + tc] = ld(grp(S::G(s)), S::K(s), off);
+    }
+    __syncthreads();
+    if (active) {
+      const unsigned o = plane + 4u * ((unsigned)c.x * (unsigned)G.py + (unsigned)c.y);
+      c.z = z;
+      c.o = o;
+      c.zn = Nb{o - 2 * sz, o - sz, o + sz, o + 2 * sz};
+      c.xn = Nb{o - 2 * sx, o - sx, o + sx, o + 2 * sx};
+      c.yn = Nb{o - 8u, o - 4u, o + 4u, o + 8u};
+      const MarchAcc<S, R> acc{FlatAcc{gw, ga, gp, gm, o, {c.zn, c.xn, c.yn}}, q, me};
+      body(c, acc);
+    }
+    __syncthreads();
+#pragma unroll
+    for (int s = 0; s < S::N; s++)
+      if (S::Q(s) >= 0) {
+#pragma unroll
+        for (int k = 0; k < 4; k++) q[S::Q(s)][k] = q[S::Q(s)][k + 1];
+        q[S::Q(s)][4] = nxt[S::Q(s)];
+      }
+  }
+}
+
+template <int R>
+__global__ __launch_bounds__(64 * R) void k_stress_fwd_m(Sweep Sw, Geom G, Groups F, Prof C, Source S, int it) {
+  const unsigned so = source_off(S);
+  const float amp = source_amp(S, G, it);
+  march<StressFwdSlots, R>(Sw, G, F, [&](const Cell& c, const auto& acc) { stress_fwd_cell(G, acc, C, c, so, amp); });
+}
+
+template <int FORM, int R>
+__global__ __launch_bounds__(64 * R) void k_velocity_fwd_m(Sweep Sw, Geom G, Groups F, Prof C) {
+  march<VelocityFwdSlots, R>(Sw, G, F, [&](const Cell& c, const auto& acc) { velocity_fwd_cell<FORM>(G, acc, C, c); });
+}
+
+template <int FORM, int R>
+__global__ __launch_bounds__(64 * R) void k_velocity_bwd_m(Sweep Sw, Geom G, Groups F, Prof C) {
+  march<VelocityBwdSlots, R>(Sw, G, F, [&](const Cell& c, const auto& acc) {
+    const Flags f = flags_of(G, c);
+    if (f.xz) {
+      if (f.y) velocity_bwd_cell<FORM, true, true, true>(G, acc, C, c);
+      else velocity_bwd_cell<FORM, true, true, false>(G, acc, C, c);
+    } else {
+      if (f.y) velocity_bwd_cell<FORM, true, false, true>(G, acc, C, c);
+      else velocity_bwd_cell<FORM, true, false, false>(G, acc, C, c);
+    }
+  });
+}
+
+template <int R>
+__global__ __launch_bounds__(64 * R) void k_stress_bwd_m(Sweep Sw, Geom G, Groups F, Prof C, Source S, int it) {
+  const unsigned so = source_off(S);
+  const float amp = source_amp(S, G, it);
+  march<StressBwdSlots, R>(Sw, G, F, [&](const Cell& c, const auto& acc) {
+    const Flags f = flags_of(G, c);
+    if (f.xz) {
+      if (f.y) stress_bwd_cell<true, true, true>(G, acc, C, c, so, amp);
+      else stress_bwd_cell<true, true, false>(G, acc, C, c, so, amp);
+    } else {
+      if (f.y) stress_bwd_cell<true, false, true>(G, acc, C, c, so, amp);
+      else stress_bwd_cell<true, false, false>(G, acc, C, c, so, amp);
+    }
+  });
+}
+
 }  // namespace
 
 // ---------------------------------------------------------------------------
@@ -695,23 +903,72 @@ static inline Sweep sweep_full(const Geom& G, Tile t) { return make_sweep(2, G.n
 static inline dim3 grid_of(const Sweep& S) { return dim3((unsigned)S.blocks); }
 static inline dim3 block_of(const Sweep& S) { return dim3(64, S.R); }
 
+// Launch shapes with zdepth >= 16 run the z-marching kernels (rows 4 or 8 per
+// workgroup); smaller ones the flat sweep (rows 1, 2 or 4).
+static inline bool marching(Tile t) { return t.zdepth >= 16; }
+static inline Sweep sweep_march(const Geom& G, Tile t) {
+  Sweep S = sweep_full(G, Tile{1, t.zdepth});
+  const int R = t.rows >= 8 ? 8 : 4;
+  S.R = R;
+  const int rows_max = S.rows_base + (S.rows_rem ? 1 : 0);
+  S.rg_max = (rows_max + R - 1) / R;
+  S.blocks = 8L * S.rg_max * S.nyb * S.nzc;
+  return S;
+}
+#define EFWI_MARCH(KERNEL, ...)                                                                   \
+  do {                                                                                            \
+    const Sweep Sw = sweep_march(G, t);                                                           \
+    if (Sw.R == 8) hipLaunchKernelGGL((KERNEL<8>), grid_of(Sw), block_of(Sw), 0, s, __VA_ARGS__); \
+    else hipLaunchKernelGGL((KERNEL<4>), grid_of(Sw), block_of(Sw), 0, s, __VA_ARGS__);          \
+  } while (0)
+
 void launch_stress_fwd(Tile t, const Geom& G, const Groups& F, const Prof& C, const Source& S, int it,
                        hipStream_t s) {
+  if (marching(t)) {
+    EFWI_MARCH(k_stress_fwd_m, Sw, G, F, C, S, it);
+    return;
+  }
   const Sweep Sw = sweep_full(G, t);
   hipLaunchKernelGGL(k_stress_fwd, grid_of(Sw), block_of(Sw), 0, s, Sw, G, F, C, S, it);
 }
 void launch_velocity_fwd(Tile t, int form, const Geom& G, const Groups& F, const Prof& C, hipStream_t s) {
+  if (marching(t)) {
+    const Sweep Sw = sweep_march(G, t);
+    if (form == 1) {
+      if (Sw.R == 8) hipLaunchKernelGGL((k_velocity_fwd_m<1, 8>), grid_of(Sw), block_of(Sw), 0, s, Sw, G, F, C);
+      else hipLaunchKernelGGL((k_velocity_fwd_m<1, 4>), grid_of(Sw), block_of(Sw), 0, s, Sw, G, F, C);
+    } else {
+      if (Sw.R == 8) hipLaunchKernelGGL((k_velocity_fwd_m<0, 8>), grid_of(Sw), block_of(Sw), 0, s, Sw, G, F, C);
+      else hipLaunchKernelGGL((k_velocity_fwd_m<0, 4>), grid_of(Sw), block_of(Sw), 0, s, Sw, G, F, C);
+    }
+    return;
+  }
   const Sweep Sw = sweep_full(G, t);
   if (form == 1) hipLaunchKernelGGL(k_velocity_fwd<1>, grid_of(Sw), block_of(Sw), 0, s, Sw, G, F, C);
   else hipLaunchKernelGGL(k_velocity_fwd<0>, grid_of(Sw), block_of(Sw), 0, s, Sw, G, F, C);
 }
 void launch_velocity_bwd(Tile t, int form, const Geom& G, const Groups& F, const Prof& C, hipStream_t s) {
+  if (marching(t)) {
+    const Sweep Sw = sweep_march(G, t);
+    if (form == 1) {
+      if (Sw.R == 8) hipLaunchKernelGGL((k_velocity_bwd_m<1, 8>), grid_of(Sw), block_of(Sw), 0, s, Sw, G, F, C);
+      else hipLaunchKernelGGL((k_velocity_bwd_m<1, 4>), grid_of(Sw), block_of(Sw), 0, s, Sw, G, F, C);
+    } else {
+      if (Sw.R == 8) hipLaunchKernelGGL((k_velocity_bwd_m<0, 8>), grid_of(Sw), block_of(Sw), 0, s, Sw, G, F, C);
+      else hipLaunchKernelGGL((k_velocity_bwd_m<0, 4>), grid_of(Sw), block_of(Sw), 0, s, Sw, G, F, C);
+    }
+    return;
+  }
   const Sweep Sw = sweep_full(G, t);
   if (form == 1) hipLaunchKernelGGL((k_velocity_bwd<1, true>), grid_of(Sw), block_of(Sw), 0, s, Sw, G, F, C);
   else hipLaunchKernelGGL((k_velocity_bwd<0, true>), grid_of(Sw), block_of(Sw), 0, s, Sw, G, F, C);
 }
 void launch_stress_bwd(Tile t, const Geom& G, const Groups& F, const Prof& C, const Source& S, int it,
                        hipStream_t s) {
+  if (marching(t)) {
+    EFWI_MARCH(k_stress_bwd_m, Sw, G, F, C, S, it);
+    return;
+  }
   const Sweep Sw = sweep_full(G, t);
   hipLaunchKernelGGL(k_stress_bwd<true>, grid_of(Sw), block_of(Sw), 0, s, Sw, G, F, C, S, it);
 }

@@ -23,7 +23,8 @@ KERNELS = ("stress_fwd", "velocity_fwd", "stress_bwd", "velocity_bwd", "velocity
            "ring", "other")
 CALC_MISFIT, CALC_GRADIENT, CALC_SYNTHETIC = 0, 1, 2
 TUNED_KERNELS = ("stress_fwd", "velocity_fwd", "stress_bwd", "velocity_bwd")
-TILE_CANDIDATES = tuple((r, z) for r in (1, 2, 4) for z in (1, 2, 4, 8))
+# efwi_autotune's candidates, in its report order: flat sweep (zdepth < 16), then z-marching
+TILE_CANDIDATES = ((1, 1), (2, 1), (4, 1), (1, 2), (1, 4), (2, 2), (4, 16), (4, 32), (4, 64), (8, 16), (8, 32), (8, 64))
 
 
 class EfwiError(RuntimeError):

@@ -116,13 +116,16 @@ int efwi_profile_read(efwi_ctx* ctx, double* ms, long long* launches, int reset)
 
 /* Launch shape of the four time-loop stencil kernels (kernel index as in
  * efwi_profile_read: 0 stress_fwd, 1 velocity_fwd, 2 stress_bwd,
- * 3 velocity_bwd): workgroups of 64 x rows threads (rows 1, 2, 4), each
- * thread updating zdepth consecutive z planes. Results do not depend on it.
- * efwi_autotune times the 12 shapes {1,2,4} x {1,2,4,8} of each kernel on this
- * grid with HIP events (reps launches each) and keeps the fastest; report, if
- * not NULL, receives the 4 x 12 average milliseconds. No reference
- * counterpart (the reference launches fixed 32 x 1 x 32 blocks,
- * Src/libCUFD.cu:93-97). */
+ * 3 velocity_bwd). zdepth < 16: flat sweep, workgroups of 64 x rows threads
+ * (rows 1, 2, 4), each thread updating zdepth consecutive z planes.
+ * zdepth >= 16: z-marching sweep, workgroups of 64 x rows (rows 4 or 8; 1 and
+ * 2 mean 4) marching zdepth planes with register z queues and LDS x/y tiles.
+ * Results do not depend on the shape. efwi_autotune times 12 shapes of each
+ * kernel on this grid (flat {1,1} {2,1} {4,1} {1,2} {1,4} {2,2}, marching
+ * {4,16} {4,32} {4,64} {8,16} {8,32} {8,64}) with HIP events (reps launches
+ * each) and keeps the fastest; report, if not NULL, receives the 4 x 12
+ * average milliseconds in that order. No reference counterpart (the
+ * reference launches fixed 32 x 1 x 32 blocks, Src/libCUFD.cu:93-97). */
 int efwi_set_tile(efwi_ctx* ctx, int kernel, int rows, int zdepth);
 int efwi_get_tile(const efwi_ctx* ctx, int kernel, int* rows, int* zdepth);
 int efwi_autotune(efwi_ctx* ctx, int reps, double* report, void* stream);

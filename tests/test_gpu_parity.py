@@ -92,10 +92,11 @@ def test_checkpointed_ring_is_bit_identical(orc, small_case, every):
         assert np.array_equal(a, b)
 
 
-@pytest.mark.parametrize("rows,zdepth", [(2, 3), (4, 8), (1, 64)])
+@pytest.mark.parametrize("rows,zdepth", [(2, 3), (4, 8), (1, 64), (4, 16), (8, 16), (8, 40)])
 def test_tile_shapes_are_bit_identical(orc, small_case, rows, zdepth):
-    """Launch shapes only remap cells to threads: gradients and seismograms
-    are bit-identical to the default shape."""
+    """Launch shapes only remap cells to threads (zdepth >= 16: the z-marching
+    kernels with register queues and LDS tiles): gradients and seismograms are
+    bit-identical to the default shape."""
     from efwi import synth
     w0, obs = small_case
     w = synth.small(n=32, npml=6, nt=140)
