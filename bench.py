@@ -77,6 +77,92 @@ def cpu_baseline(w, nsteps):
                       f"fwd+adj+grad, 1 shot, fp32 oracle, {r.seconds:.1f} s"}
 
 
+def bench_batch(args, w):
+    """Batched shots (cfg3): all shots in one run_batch (persistent (shot, tile)
+    launches, one captured hipGraph per time step); the same shots one after
+    another through run_shot for comparison."""
+    import torch
+    from efwi.propagator import Propagator
+    g = w.grid
+    n = len(w.shots)
+    print(f"[bench] {w.name}: padded grid {g.shape}, nt={g.nt}, {n} shots, nrec {min(s.nrec for s in w.shots)}-"
+          f"{max(s.nrec for s in w.shots)}", file=sys.stderr, flush=True)
+    P = Propagator(g, 0)
+    P.set_model(*(torch.from_numpy(a) for a in w.true))
+    _, obs = P.run_batch(w.shots, 2)
+    P.set_model(*(torch.from_numpy(a) for a in w.model))
+    if not args.no_autotune:
+        tune = P.autotune(reps=3)
+        print("[bench] autotune:", {k: min(v, key=v.get) for k, v in tune.items()}, file=sys.stderr, flush=True)
+    # autotune the batch: launch shape, shots per batch and queue split, one
+    # timed pass each (the batched launches run flat tiles)
+    tuned = P.tiles()
+    kinds = ("stress_fwd", "velocity_fwd", "stress_bwd", "velocity_bwd")
+
+    def timed(cfg):
+        tile, size, take, static_pct = cfg
+        for k in kinds:
+            P.set_tile(k, *tile)
+        P.batch_config(take, static_pct)
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        for k in range(0, n, size):
+            P.run_batch(w.shots[k:k + size], 1, obs[k:k + size])
+        torch.cuda.synchronize()
+        return time.perf_counter() - t0
+    trials = {}
+    cands = [(tile, n, take, 0) for tile in ((2, 1), (4, 1), (2, 2), (4, 2)) for take in (1, 2, 4)]
+    cands += [((4, 2), n, 2, 50), ((4, 2), max(1, n // 4), 2, 0), ((4, 2), 1, 2, 90)]
+    for cfg in cands:
+        timed(cfg)  # warm-up (graph capture)
+        trials[cfg] = timed(cfg)
+        print(f"[bench] tile {cfg[0]} batch {cfg[1]} take {cfg[2]} static {cfg[3]}%: "
+              f"{n / trials[cfg]:.3f} shots/s", file=sys.stderr, flush=True)
+    best = min(trials, key=trials.get)
+    for _ in range(args.warmup):
+        timed(best)
+    t0 = time.perf_counter()
+    for k in range(args.steps):
+        dt = timed(best)
+        print(f"[bench] step {k}: {dt:.3f} s", file=sys.stderr, flush=True)
+    dt_step = (time.perf_counter() - t0) / args.steps
+    for k, tile in tuned.items():
+        P.set_tile(k, *tile)
+    # the same shots one by one through run_shot (one timed pass)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    P.zero_gradient()
+    for s, o in zip(w.shots, obs):
+        P.run_shot(s, 1, o)
+    torch.cuda.synchronize()
+    dt_seq = time.perf_counter() - t0
+    out = {
+        "metric": "batched fwd+adj+grad shots/s",
+        "value": round(n / dt_step, 4),
+        "unit": "shots/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": round(dt_step * 1e3, 2),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "fp32",
+        "data": "synthetic",
+        "config": {"workload": f"{w.name}: {g.nz - 2 * g.npml}x{g.nx - 2 * g.npml}x{g.ny - 2 * g.npml} physical + "
+                               f"{g.npml} PML cells, padded {g.nz}x{g.nx}x{g.ny}, nt={g.nt}, {n} shots in one batch, "
+                               f"{sum(s.nrec for s in w.shots)} receivers in all, graph per time step, "
+                               f"tile {best[0][0]}x{best[0][1]}, batches of {best[1]} shots, queue take {best[2]}, "
+                               f"static share {best[3]}%",
+                   "batch_trials_shots_per_s": {f"t{c[0][0]}x{c[0][1]}/b{c[1]}/k{c[2]}/s{c[3]}": round(n / t, 4)
+                                                for c, t in trials.items()},
+                   "gcell_updates_per_s": round(g.ncells * g.nt * n / dt_step / 1e9, 3),
+                   "sequential_shots_per_s": round(n / dt_seq, 4)},
+    }
+    print(json.dumps(out), flush=True)
+    P.close()
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -116,6 +202,8 @@ def main():
     from efwi.propagator import Propagator
 
     w = synth.WORKLOADS[args.workload](**({} if args.nt is None else {"nt": args.nt}))
+    if len(w.shots) > 1:
+        return bench_batch(args, w)
     g = w.grid
     shot = w.shots[0]
     print(f"[bench] {w.name}: padded grid {g.shape}, nt={g.nt}, nrec={shot.nrec}", file=sys.stderr, flush=True)

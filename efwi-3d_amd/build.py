@@ -43,10 +43,18 @@ def sources():
     return sorted(os.path.join(CSRC, f) for f in os.listdir(CSRC) if f.endswith((".hip", ".cpp")))
 
 
+# Per-file options. efwi_stencil.hip: the batched launches claim queue items
+# with one atomic per workgroup and use the result only after the current
+# items; the atomic optimizer's wave-level rewrite would wait for it at once.
+EXTRA = {
+    "efwi_stencil.hip": ["-mllvm", "-amdgpu-atomic-optimizer-strategy=None"],
+}
+
+
 def _compile(src: str) -> str:
     obj = os.path.join(OBJ_DIR, os.path.basename(src) + ".o")
     lang = ["-x", "hip"] if src.endswith(".hip") else []
-    cmd = [hipcc()] + lang + COMMON + ["-c", src, "-o", obj]
+    cmd = [hipcc()] + lang + COMMON + EXTRA.get(os.path.basename(src), []) + ["-c", src, "-o", obj]
     r = subprocess.run(cmd, capture_output=True, text=True)
     if r.returncode != 0:
         raise RuntimeError(f"compile failed: {' '.join(cmd)}\n{r.stdout}\n{r.stderr}")

@@ -215,6 +215,52 @@ inline int blocks_for(long n, int threads, int cap) {
   return (int)b;
 }
 
+// Batched per-step kernels over the shot table (grid.y or grid.z = shot); the
+// time step is read from device memory so that a captured step replays.
+__device__ __forceinline__ int step_of(const int* it) {
+  return __hip_atomic_load(it, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+// receiver gather of step *it + off, as k_record
+__global__ void k_record_b(const ShotDev* __restrict__ shots, const int* it_dev, int off) {
+  const ShotDev& sd = shots[blockIdx.y];
+  const int r = blockIdx.x * blockDim.x + threadIdx.x;
+  if (r >= sd.nrec) return;
+  const int it = step_of(it_dev) + off;
+  const long i = sd.rec[r];
+  float* o = sd.syn_t + (long)it * 3 * sd.nrec;
+  o[r] = sd.w9[1][i];
+  o[sd.nrec + r] = sd.w9[2][i];
+  o[2 * sd.nrec + r] = sd.w9[0][i];
+}
+// adjoint-source injection of step *it, as k_inject
+__global__ void k_inject_b(const ShotDev* __restrict__ shots, const int* it_dev) {
+  const ShotDev& sd = shots[blockIdx.y];
+  const int r = blockIdx.x * blockDim.x + threadIdx.x;
+  if (r >= sd.nrec) return;
+  const int it = step_of(it_dev);
+  const long i = sd.rec[r];
+  const float* q = sd.res_t + (long)it * 3 * sd.nrec;
+  atomicAdd(sd.a9[1] + i, q[r]);
+  atomicAdd(sd.a9[2] + i, q[sd.nrec + r]);
+  atomicAdd(sd.a9[0] + i, q[2 * sd.nrec + r]);
+}
+// ring slot *it of fields k0 + blockIdx.y, shot blockIdx.z, as k_ring
+__global__ void k_ring_b(Band B, Geom G, const ShotDev* __restrict__ shots, const int* it_dev, int k0, int restore) {
+  const ShotDev& sd = shots[blockIdx.z];
+  const long slot = step_of(it_dev);
+  const int f = k0 + blockIdx.y;
+  float* __restrict__ fld = sd.w9[f];
+  float* __restrict__ dst = sd.ring + (slot * 9 + f) * B.n;
+  for (long k = (long)blockIdx.x * blockDim.x + threadIdx.x; k < B.n; k += (long)gridDim.x * blockDim.x) {
+    const long i = band_cell(B, G, k);
+    if (restore) fld[i] = dst[k];
+    else dst[k] = fld[i];
+  }
+}
+__global__ void k_step_add(int* it, int delta) {
+  if (threadIdx.x == 0) *it += delta;
+}
+
 }  // namespace
 
 void launch_fill(float* p, long n, float v, hipStream_t s) {
@@ -263,6 +309,24 @@ void launch_to_time_major(const float* src, float* dst, int nrec, int nt, hipStr
 void launch_from_time_major(const float* src, float* dst, int nrec, int nt, hipStream_t s) {
   dim3 grid((nrec + 63) / 64, (nt + 63) / 64, 3);
   hipLaunchKernelGGL(k_transpose_back, grid, dim3(64, 4), 0, s, src, dst, nrec, nt);
+}
+
+void launch_record_batch(const ShotDev* shots, int nshots, int max_nrec, const int* it, int it_offset,
+                         hipStream_t s) {
+  if (max_nrec <= 0) return;
+  hipLaunchKernelGGL(k_record_b, dim3((max_nrec + 255) / 256, nshots), dim3(256), 0, s, shots, it, it_offset);
+}
+void launch_inject_batch(const ShotDev* shots, int nshots, int max_nrec, const int* it, hipStream_t s) {
+  if (max_nrec <= 0) return;
+  hipLaunchKernelGGL(k_inject_b, dim3((max_nrec + 255) / 256, nshots), dim3(256), 0, s, shots, it);
+}
+void launch_ring_batch(const Band& B, const Geom& G, const ShotDev* shots, int nshots, const int* it, int k0,
+                       int k1, int restore, hipStream_t s) {
+  dim3 grid(blocks_for(B.n, 256, 512), k1 - k0, nshots);
+  hipLaunchKernelGGL(k_ring_b, grid, dim3(256), 0, s, B, G, shots, it, k0, restore);
+}
+void launch_step_add(int* it, int delta, hipStream_t s) {
+  hipLaunchKernelGGL(k_step_add, dim3(1), dim3(64), 0, s, it, delta);
 }
 
 }  // namespace efwi

@@ -76,4 +76,33 @@ struct Source {
   const float* stf;     // tapered source time function, device, nt samples
 };
 
+// One shot of a batch as the batched kernels read it (device table).
+struct ShotDev {
+  Groups F;           // the shot's field groups (its own model copy and gradients in F.m)
+  long cell;          // source cell, inside the reverse-reconstruction interior
+  const float* stf;   // tapered source time function [nt]
+  float* w9[9];       // forward wavefield fields (ring, record), ring order vz vx vy szz sxx sxz syy sxy syz
+  float* a9[9];       // adjoint wavefield fields (inject)
+  const long* rec;    // receiver cells [nrec]
+  int nrec;
+  float* syn_t;       // [nt][3][nrec]
+  const float* res_t; // [nt][3][nrec]
+  float* ring;        // [nt-1][9][band]
+};
+
+// A batched launch: the shot table, the eight per-XCD work-queue heads (zeroed
+// by a fill kernel before the launch) and the device-resident time step.
+// Queue heads of a batched launch sit kQueueStride unsigned apart (one cache
+// line each), 8 heads per launch.
+constexpr int kQueueStride = 32;
+struct BatchLaunch {
+  const ShotDev* shots;
+  int nshots;
+  unsigned* counters;
+  const int* it;
+  int cus;   // compute units; persistent workgroups = cus x resident workgroups per CU
+  int take;  // items taken per atomic
+  int stat;  // launcher input: static share in percent; kernel input: static items per workgroup
+};
+
 }  // namespace efwi

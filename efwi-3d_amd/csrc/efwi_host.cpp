@@ -27,12 +27,6 @@ int fail(int code, const std::string& msg) {
   return code;
 }
 
-#define HIP_TRY(call)                                                                   \
-  do {                                                                                  \
-    hipError_t e_ = (call);                                                             \
-    if (e_ != hipSuccess)                                                               \
-      return fail(EFWI_ERR_HIP, std::string(#call) + ": " + hipGetErrorString(e_));     \
-  } while (0)
 
 // ---------------------------------------------------------------------------
 // 1-D C-PML profile of one axis, Src/utilities.cu:204-320. The float
@@ -165,6 +159,7 @@ Ctx::~Ctx() {
   hipGetDevice(&prev);
   hipSetDevice(cfg.device);
   prof.release();
+  release_batch();
   for (void* p : allocs) hipFree(p);
   if (ring) hipFree(ring);
   if (ckpt) hipFree(ckpt);
@@ -183,7 +178,7 @@ int Ctx::alloc(void** p, size_t bytes) {
   return EFWI_OK;
 }
 
-static int grow(Ctx* c, void** p, size_t* cap, size_t need) {
+int grow(Ctx* c, void** p, size_t* cap, size_t need) {
   if (*cap >= need) return EFWI_OK;
   if (*p) {
     for (auto& q : c->allocs)
@@ -197,11 +192,11 @@ static int grow(Ctx* c, void** p, size_t* cap, size_t need) {
 }
 
 // Field pointers of a wave group / psi group (fields fsf floats apart).
-static void wave_of(float* base, size_t fsf, Wave& W) {
+void wave_of(float* base, size_t fsf, Wave& W) {
   float** f[9] = {&W.vz, &W.vx, &W.vy, &W.szz, &W.sxx, &W.sxz, &W.syy, &W.sxy, &W.syz};
   for (int k = 0; k < F_WAVE_COUNT; k++) *f[k] = base + k * fsf;
 }
-static void psi_of(float* base, size_t fsf, Psi& P) {
+void psi_of(float* base, size_t fsf, Psi& P) {
   float** f = reinterpret_cast<float**>(&P);
   for (int k = 0; k < P_PSI_COUNT; k++) f[k] = base + k * fsf;
 }
@@ -289,18 +284,18 @@ int Ctx::init() {
   return EFWI_OK;
 }
 
-static void fields9(const Wave& W, Fields9& F) {
+void fields9(const Wave& W, Fields9& F) {
   F.f[0] = W.vz; F.f[1] = W.vx; F.f[2] = W.vy; F.f[3] = W.szz; F.f[4] = W.sxx;
   F.f[5] = W.sxz; F.f[6] = W.syy; F.f[7] = W.sxy; F.f[8] = W.syz;
 }
 
-static void zero_wave(const Wave& W, long n, hipStream_t s) {
+void zero_wave(const Wave& W, long n, hipStream_t s) {
   Fields9 F;
   fields9(W, F);
   for (int k = 0; k < 9; k++) launch_fill(F.f[k], n, 0.0f, s);
 }
 
-static void zero_psi(const Psi& P, long n, hipStream_t s) {
+void zero_psi(const Psi& P, long n, hipStream_t s) {
   float* const* p = reinterpret_cast<float* const*>(&P);
   for (int k = 0; k < 18; k++) launch_fill(p[k], n, 0.0f, s);
 }

@@ -12,6 +12,13 @@
 
 namespace efwi {
 
+#define HIP_TRY(call)                                                                   \
+  do {                                                                                  \
+    hipError_t e_ = (call);                                                             \
+    if (e_ != hipSuccess)                                                               \
+      return fail(EFWI_ERR_HIP, std::string(#call) + ": " + hipGetErrorString(e_));     \
+  } while (0)
+
 int fail(int code, const std::string& msg);
 void cpml_profile_1d(int n, int npml, float h, float f0, float dt, float* K, float* a, float* b,
                      float* Kh, float* ah, float* bh);
@@ -34,6 +41,27 @@ struct Profiler {
   void end(hipStream_t s);
   int collect();  // synchronises on the recorded events and accumulates
   void release();
+};
+
+// Batched shots (efwi_run_batch): per-shot state, the device shot table, the
+// per-XCD queue heads, the device time step, a capture stream and the two
+// captured per-step graphs.
+struct BatchState {
+  int n = 0;
+  std::vector<Groups> F;
+  std::vector<float*> ring, stf, syn_t, obs_t, res_t;
+  std::vector<long*> rec;
+  std::vector<size_t> seis_cap, rec_cap;
+  std::vector<int> nrec;
+  ShotDev* shots_dev = nullptr;
+  unsigned* counters = nullptr;  // 2 launches per step x 8 queues
+  int* it_dev = nullptr;
+  hipStream_t cs = nullptr;
+  hipGraphExec_t gf = nullptr, gb = nullptr;
+  std::string key_f, key_b;
+  int cus = 0;
+  int take = 2;        // queue items per atomic (efwi_batch_config)
+  int static_pct = 0;  // static share of the queues, percent
 };
 
 struct Ctx {
@@ -101,6 +129,20 @@ struct Ctx {
                double* misfit, hipStream_t s);
   int get_gradient(float* gl, float* gm, float* gr, int on_host, hipStream_t s);
   int autotune(int reps, hipStream_t s, double* report);
+  // batched shots (efwi_batch.cpp)
+  BatchState batch;
+  int ensure_batch(int n);
+  int run_batch(const efwi_shot* shots, int n, int calc_id, const float* const* obs, float* const* syn,
+                double* misfits, int use_graph, hipStream_t s);
+  int batch_gradient(int shot, float* gl, float* gm, float* gr, int on_host, hipStream_t s);
+  void release_batch();
 };
+
+int grow(Ctx* c, void** p, size_t* cap, size_t need);
+void wave_of(float* base, size_t fsf, Wave& W);
+void psi_of(float* base, size_t fsf, Psi& P);
+void fields9(const Wave& W, Fields9& F);
+void zero_wave(const Wave& W, long n, hipStream_t s);
+void zero_psi(const Psi& P, long n, hipStream_t s);
 
 }  // namespace efwi

@@ -48,6 +48,15 @@ void launch_stress_bwd(Tile t, const Geom& G, const Groups& F, const Prof& C, co
 void launch_velocity_adj(int form, const Geom& G, const Groups& F, const Prof& C, hipStream_t s);
 void launch_stress_adj(const Geom& G, const Groups& F, const Prof& C, hipStream_t s);
 
+// batched persistent stencil kernels: one launch covers the (shot, tile) items
+// of all shots of L, with the flat sweep's tiles (t.zdepth < 16)
+void launch_stress_fwd_batch(Tile t, const Geom& G, const Prof& C, const BatchLaunch& L, hipStream_t s);
+void launch_velocity_fwd_batch(Tile t, int form, const Geom& G, const Prof& C, const BatchLaunch& L, hipStream_t s);
+void launch_velocity_bwd_batch(Tile t, int form, const Geom& G, const Prof& C, const BatchLaunch& L, hipStream_t s);
+void launch_stress_bwd_batch(Tile t, const Geom& G, const Prof& C, const BatchLaunch& L, hipStream_t s);
+// workgroups a persistent launch keeps resident per CU
+int device_cus(int device);
+
 // auxiliary kernels (efwi_aux.hip)
 void launch_fill(float* p, long n, float v, hipStream_t s);
 void launch_byc(const Geom& G, const float* rho, float* byc, hipStream_t s);
@@ -67,6 +76,13 @@ int launch_misfit(const float* obs_t, const float* syn_t, float* res_t, const fl
 void launch_sum_partials(const double* partials, int n, double* out, hipStream_t s);
 // FP64 dot product of float vectors; returns the number of partials written.
 int launch_dot64(const float* a, const float* b, long n, double* partials, int max_blocks, hipStream_t s);
+// batched per-step aux kernels of the shot table (time step read on the device)
+void launch_record_batch(const ShotDev* shots, int nshots, int max_nrec, const int* it, int it_offset,
+                         hipStream_t s);
+void launch_inject_batch(const ShotDev* shots, int nshots, int max_nrec, const int* it, hipStream_t s);
+void launch_ring_batch(const Band& B, const Geom& G, const ShotDev* shots, int nshots, const int* it, int k0,
+                       int k1, int restore, hipStream_t s);
+void launch_step_add(int* it, int delta, hipStream_t s);
 // [3][nrec][nt] <-> [nt][3][nrec]
 void launch_to_time_major(const float* src, float* dst, int nrec, int nt, hipStream_t s);
 void launch_from_time_major(const float* src, float* dst, int nrec, int nt, hipStream_t s);

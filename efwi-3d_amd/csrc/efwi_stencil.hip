@@ -46,6 +46,8 @@
 //           off-by-one x/y C-PML predicate, Src/velocity.cu:46-90; swapped
 //           adjoint C-PML coefficients, Src/velocity_adj.cu:139,142).
 // FORM = 1: the same three places corrected.
+#include <algorithm>
+
 #include "efwi_common.h"
 #include "efwi_kernels.h"
 
@@ -202,8 +204,7 @@ __device__ __forceinline__ Pv pv(const Prof& C, int ax, bool half, int i) {
 // XCD b % 8 and Z consecutive planes; the strip is walked plane chunk by
 // plane chunk (z slowest), so each XCD's L2 sees a compact, z-ordered region.
 template <typename Body>
-__device__ __forceinline__ void sweep(const Sweep& S, const Geom& G, Body&& body) {
-  const int b = blockIdx.x;
+__device__ __forceinline__ void sweep_block(const Sweep& S, const Geom& G, int b, Body&& body) {
   const int xcd = b & 7, j = b >> 3;
   const int per_plane = S.rg_max * S.nyb;
   const int zc = j / per_plane;
@@ -236,6 +237,10 @@ __device__ __forceinline__ void sweep(const Sweep& S, const Geom& G, Body&& body
     body(c);
     oy += sz;
   }
+}
+template <typename Body>
+__device__ __forceinline__ void sweep(const Sweep& S, const Geom& G, Body&& body) {
+  sweep_block(S, G, (int)blockIdx.x, body);
 }
 
 // ---------------------------------------------------------------------------
@@ -875,6 +880,149 @@ __global__ __launch_bounds__(64 * R) void k_stress_bwd_m(Sweep Sw, Geom G, Group
   });
 }
 
+// ---------------------------------------------------------------------------
+// Batched shots: one persistent launch per stencil step covers every (shot,
+// tile) item of the batch. Items are the flat sweep's workgroup tiles; those of
+// XCD strip q (tile index b with b % 8 == q) of every shot form queue q. A
+// workgroup drains the queue of the XCD it runs on (hardware register
+// XCC_ID), taking `take` consecutive items per atomic, then steals from the
+// other queues, so every item is processed once whatever the placement. The time step is
+// read from device memory, so a captured step replays unchanged.
+// ---------------------------------------------------------------------------
+__device__ __forceinline__ int xcc_id() {
+  // s_getreg_b32 hwreg(HW_REG_XCC_ID, 0, 4): bits [3:0] hold the XCC of the wave
+  return __builtin_amdgcn_s_getreg((3 << 11) | (0 << 6) | 20) & 7;
+}
+
+// The value of `v` held by thread 0, in every thread of the workgroup.
+__device__ __forceinline__ int wg_broadcast(int v, bool one_wave, int* slot) {
+  if (one_wave) return __builtin_amdgcn_readfirstlane(v);
+  __syncthreads();
+  if (threadIdx.x == 0 && threadIdx.y == 0) *slot = v;
+  __syncthreads();
+  return __builtin_amdgcn_readfirstlane(*slot);
+}
+
+template <typename Item>
+__device__ __forceinline__ void persistent_items(const BatchLaunch& L, int per_xcd, Item&& item) {
+  __shared__ int slot;
+  const int take = L.take;
+  const int nq = L.nshots * per_xcd;
+  const int me = xcc_id();
+  const bool one_wave = blockDim.y == 1;
+  const bool leader = threadIdx.x == 0 && threadIdx.y == 0;
+  auto run = [&](int q, int i) {
+    const int shot = i / per_xcd;
+    item(shot, 8 * (i - shot * per_xcd) + q);
+  };
+  // claim the first dynamic items now: the atomic's round trip overlaps the
+  // static share
+  unsigned* head = L.counters + me * kQueueStride;
+  int got = 0;
+  if (leader) got = (int)atomicAdd(head, (unsigned)take);
+  // static share: workgroup w runs items j, j + wq, ... (L.stat of them) of
+  // queue w % 8, j = w / 8, wq = workgroups per queue; the grid is a multiple
+  // of 8 and L.stat * wq <= every queue's length
+  const int wq = (int)(gridDim.x >> 3);
+  const int done = L.stat * wq;
+  for (int k = 0, i = (int)(blockIdx.x >> 3); k < L.stat; k++, i += wq) run((int)(blockIdx.x & 7), i);
+  // the rest of every queue: `take` items per atomic, own XCD's queue first
+  for (int d = 0; d < 8; d++) {
+    const int q = (me + d) & 7;
+    head = L.counters + q * kQueueStride;
+    if (d > 0) {
+      // another XCD's queue: skip it without an atomic once it is drained (one
+      // load by thread 0, so that every wave of the workgroup takes the same path)
+      int h = 0;
+      if (leader) h = (int)__hip_atomic_load(head, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+      if (done + wg_broadcast(h, one_wave, &slot) >= nq) continue;
+      if (leader) got = (int)atomicAdd(head, (unsigned)take);
+    }
+    int cur = done + wg_broadcast(got, one_wave, &slot);
+    while (cur < nq) {
+      // claim the next items before working on these
+      int next = 0;
+      if (leader) next = (int)atomicAdd(head, (unsigned)take);
+      const int end = min(cur + take, nq);
+      for (int i = cur; i < end; i++) run(q, i);
+      cur = done + wg_broadcast(next, one_wave, &slot);
+    }
+  }
+}
+
+__device__ __forceinline__ int device_step(const int* it) {
+  return __builtin_amdgcn_readfirstlane(__hip_atomic_load(it, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT));
+}
+
+// The shot table is read with vector loads (the compiler cannot prove that the
+// kernel's stores leave it unchanged); the values are wave-uniform, and moving
+// them to scalar registers keeps the buffer descriptors built from them scalar
+// (descriptors in vector registers turn every buffer access into a loop).
+template <class T>
+__device__ __forceinline__ T* uniform_ptr(T* p) {
+  const unsigned long long v = (unsigned long long)p;
+  const unsigned lo = __builtin_amdgcn_readfirstlane((unsigned)v);
+  const unsigned hi = __builtin_amdgcn_readfirstlane((unsigned)(v >> 32));
+  return (T*)(((unsigned long long)hi << 32) | lo);
+}
+__device__ __forceinline__ long uniform_long(long x) {
+  return (long)(unsigned long long)uniform_ptr((char*)x);
+}
+
+// KIND: 0 stress_fwd, 1 velocity_fwd, 2 velocity_bwd, 3 stress_bwd
+template <int KIND, int FORM>
+__global__ __launch_bounds__(256) void k_batch(Sweep Sw, Geom G, Prof C, BatchLaunch L, int per_xcd) {
+  const int it = device_step(L.it);
+  int have = -1;
+  Groups F{};
+  Source S{};
+  persistent_items(L, per_xcd, [&](int shot, int b) {
+    if (shot != have) {  // consecutive items mostly belong to one shot
+      const ShotDev& sd = L.shots[shot];
+      F.w = uniform_ptr(sd.F.w);
+      F.a = uniform_ptr(sd.F.a);
+      F.p = uniform_ptr(sd.F.p);
+      F.m = uniform_ptr(sd.F.m);
+      S.cell = uniform_long(sd.cell);
+      S.stf = uniform_ptr(sd.stf);
+      have = shot;
+    }
+    if constexpr (KIND == 0) {
+      const unsigned so = source_off(S);
+      const float amp = source_amp(S, G, it);
+      sweep_block(Sw, G, b, [&](const Cell& c) { stress_fwd_cell(G, flat_acc(G, F, c), C, c, so, amp); });
+    } else if constexpr (KIND == 1) {
+      sweep_block(Sw, G, b, [&](const Cell& c) { velocity_fwd_cell<FORM>(G, flat_acc(G, F, c), C, c); });
+    } else if constexpr (KIND == 2) {
+      sweep_block(Sw, G, b, [&](const Cell& c) {
+        const FlatAcc acc = flat_acc(G, F, c);
+        const Flags f = flags_of(G, c);
+        if (f.xz) {
+          if (f.y) velocity_bwd_cell<FORM, true, true, true>(G, acc, C, c);
+          else velocity_bwd_cell<FORM, true, true, false>(G, acc, C, c);
+        } else {
+          if (f.y) velocity_bwd_cell<FORM, true, false, true>(G, acc, C, c);
+          else velocity_bwd_cell<FORM, true, false, false>(G, acc, C, c);
+        }
+      });
+    } else {
+      const unsigned so = source_off(S);
+      const float amp = source_amp(S, G, it);
+      sweep_block(Sw, G, b, [&](const Cell& c) {
+        const FlatAcc acc = flat_acc(G, F, c);
+        const Flags f = flags_of(G, c);
+        if (f.xz) {
+          if (f.y) stress_bwd_cell<true, true, true>(G, acc, C, c, so, amp);
+          else stress_bwd_cell<true, true, false>(G, acc, C, c, so, amp);
+        } else {
+          if (f.y) stress_bwd_cell<true, false, true>(G, acc, C, c, so, amp);
+          else stress_bwd_cell<true, false, false>(G, acc, C, c, so, amp);
+        }
+      });
+    }
+  });
+}
+
 }  // namespace
 
 // ---------------------------------------------------------------------------
@@ -983,6 +1131,68 @@ void launch_stress_adj(const Geom& G, const Groups& F, const Prof& C, hipStream_
   none.cell = -1;
   none.stf = nullptr;
   hipLaunchKernelGGL(k_stress_bwd<false>, grid_of(Sw), block_of(Sw), 0, s, Sw, G, F, C, none, 0);
+}
+
+// Batched launches (flat tiles only; a zdepth >= 16 shape falls back to its
+// flat counterpart rows x 1).
+static inline Sweep sweep_batch(const Geom& G, Tile t) {
+  if (t.zdepth >= 16) t = Tile{t.rows >= 4 ? 4 : t.rows, 1};
+  return sweep_full(G, t);
+}
+int device_cus(int device) {
+  int cus = 256;
+  if (hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, device) != hipSuccess) cus = 256;
+  return cus;
+}
+// Workgroups of `kernel` resident on one CU at `threads` per workgroup (from its
+// register and LDS use), cached per block size.
+template <class K>
+static int resident_per_cu(K kernel, int threads, int* cache) {
+  const int slot = threads / 64 - 1;
+  if (slot >= 0 && slot < 8 && cache[slot] > 0) return cache[slot];
+  int nb = 0;
+  if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, kernel, threads, 0) != hipSuccess || nb < 1) nb = 1;
+  if (slot >= 0 && slot < 8) cache[slot] = nb;
+  return nb;
+}
+// Persistent workgroups: enough to fill every CU, a multiple of 8 (one set per
+// XCD queue), at most one per queue item. Each workgroup first runs its static
+// share of L.stat items (stat_pct percent of its even share), then claims the
+// rest `take` items per atomic; take is lowered so that every workgroup gets at
+// least 2 claims of the dynamic rest.
+static inline dim3 batch_plan(const Sweep& Sw, BatchLaunch& L, int per_cu, int stat_pct) {
+  const long nq = Sw.blocks / 8 * L.nshots;  // items per queue
+  const long wq = std::max(1L, std::min((long)L.cus * per_cu / 8, nq));
+  const long even = nq / wq;
+  L.stat = (int)(even * std::max(0, std::min(stat_pct, 100)) / 100);
+  const long rest = nq - L.stat * wq;
+  L.take = (int)std::max(1L, std::min((long)L.take, rest / (wq * 2)));
+  return dim3((unsigned)(8 * wq));
+}
+#define EFWI_BATCH(KERNEL)                                                                     \
+  do {                                                                                        \
+    static int occ_cache[8] = {0};                                                            \
+    const Sweep Sw = sweep_batch(G, t);                                                       \
+    const int per_xcd = (int)(Sw.blocks / 8);                                                 \
+    const dim3 blk = block_of(Sw);                                                            \
+    const int per_cu = resident_per_cu((KERNEL), (int)(blk.x * blk.y * blk.z), occ_cache);    \
+    BatchLaunch Lk = L;                                                                       \
+    const dim3 grid = batch_plan(Sw, Lk, per_cu, L.stat);                                             \
+    hipLaunchKernelGGL((KERNEL), grid, blk, 0, s, Sw, G, C, Lk, per_xcd);                     \
+  } while (0)
+void launch_stress_fwd_batch(Tile t, const Geom& G, const Prof& C, const BatchLaunch& L, hipStream_t s) {
+  EFWI_BATCH((k_batch<0, 0>));
+}
+void launch_velocity_fwd_batch(Tile t, int form, const Geom& G, const Prof& C, const BatchLaunch& L, hipStream_t s) {
+  if (form == 1) EFWI_BATCH((k_batch<1, 1>));
+  else EFWI_BATCH((k_batch<1, 0>));
+}
+void launch_velocity_bwd_batch(Tile t, int form, const Geom& G, const Prof& C, const BatchLaunch& L, hipStream_t s) {
+  if (form == 1) EFWI_BATCH((k_batch<2, 1>));
+  else EFWI_BATCH((k_batch<2, 0>));
+}
+void launch_stress_bwd_batch(Tile t, const Geom& G, const Prof& C, const BatchLaunch& L, hipStream_t s) {
+  EFWI_BATCH((k_batch<3, 0>));
 }
 
 }  // namespace efwi

@@ -213,3 +213,56 @@ class Propagator:
             N.check(N.lib().efwi_run_shot(self._h, C.byref(cs), int(calc_id), N.vptr(obs_t), N.vptr(syn),
                                           N.vptr(res), C.byref(j), self._stream()), "efwi_run_shot")
         return (None if calc_id == N.CALC_SYNTHETIC else j.value), syn, res
+
+    def run_batch(self, shots, calc_id: int, obs=None, want_syn=False, use_graph=True):
+        """All `shots` together: one persistent launch per stencil step over
+        every (shot, tile) item, each time step replayed from a captured
+        hipGraph (use_graph). Returns (misfits, syns): a list of FP64 misfits
+        (None for calc_id 2) and, when requested, [3][nrec][nt] CUDA tensors.
+        calc_id 1 keeps one gradient per shot (batch_gradient)."""
+        g = self.grid
+        n = len(shots)
+        dev = f"cuda:{self.device}"
+        keep = []
+        obs_p = (C.c_void_p * n)()
+        syn_p = (C.c_void_p * n)()
+        syns = []
+        for k, sh in enumerate(shots):
+            shp = (3, sh.nrec, g.nt)
+            if calc_id != N.CALC_SYNTHETIC:
+                if obs is None:
+                    raise ValueError("observed data required for calc_id 0/1")
+                o = self._dev(obs[k], shp)
+                keep.append(o)
+                obs_p[k] = N.vptr(o)
+            if want_syn or calc_id == N.CALC_SYNTHETIC:
+                y = torch.empty(shp, dtype=torch.float32, device=dev)
+                syns.append(y)
+                syn_p[k] = N.vptr(y)
+            else:
+                syns.append(None)
+        cs = (N.Shot * n)()
+        for k, sh in enumerate(shots):
+            cs[k] = sh.c_struct()
+            keep.append(sh._keep)
+        mis = (C.c_double * n)()
+        with torch.cuda.device(self.device):
+            N.check(N.lib().efwi_run_batch(self._h, cs, n, int(calc_id), obs_p, syn_p, mis, int(bool(use_graph)),
+                                           self._stream()), "efwi_run_batch")
+        misfits = None if calc_id == N.CALC_SYNTHETIC else [mis[k] for k in range(n)]
+        return misfits, (syns if (want_syn or calc_id == N.CALC_SYNTHETIC) else None)
+
+    def batch_config(self, take: int = 2, static_pct: int = 0):
+        """Work distribution of the batched launches: each persistent workgroup
+        runs static_pct percent of an even share of its XCD's queue, then
+        claims the rest `take` tiles per atomic."""
+        N.check(N.lib().efwi_batch_config(self._h, int(take), int(static_pct)), "efwi_batch_config")
+
+    def batch_gradient(self, shot: int):
+        """(g_lambda, g_mu, g_rho) of one shot of the last run_batch."""
+        g = self.grid
+        out = [torch.empty(g.shape, dtype=torch.float32, device=f"cuda:{self.device}") for _ in range(3)]
+        with torch.cuda.device(self.device):
+            N.check(N.lib().efwi_batch_gradient(self._h, int(shot), *(N.vptr(t) for t in out), 0, self._stream()),
+                    "efwi_batch_gradient")
+        return tuple(out)

@@ -154,4 +154,33 @@ def small(n=32, npml=6, nt=120, nrec_side=8, seed=3, formulation=0, h=10.0, dt=1
                    [(3, nx // 2, ny // 2)], rec, formulation)
 
 
-WORKLOADS = {"cfg1": cfg1, "cfg2": cfg2}
+def cfg3(nt=500, nshots=16, nx=160, ny=160, nz=64, npml=10, seed=3):
+    """Batched shots on the plain-method recipe over a wider surface: 64x160x160
+    physical grid (10 PML cells), dx 10 m, dt 1 ms, 500 steps; 16 seeded shots
+    at z 5, each with a square receiver aperture at z 3 centred on its source,
+    side drawn from [30, 140] cells at 1-cell spacing (clipped to the grid), so
+    900 to 19,600 receivers per shot."""
+    rng = np.random.default_rng(seed)
+    interfaces = np.sort(rng.integers(nz // 8, nz - nz // 8, size=3))
+    vp_b = layered(nz, nx, ny, [2000.0, 2500.0, 3000.0, 3500.0], interfaces)
+    vs_b = vp_b / 1.73
+    rho_b = 310.0 * vp_b ** 0.25
+    vp_a = vp_b * (1.0 + 0.10 * gaussian(nz, nx, ny, (nz // 2, nx // 2, ny // 2), 5.0))
+    w = _finish("cfg3", nz, nx, ny, npml, 10.0, 1e-3, nt, 10.0, vp_b, vs_b, rho_b, vp_a, vs_b, rho_b,
+                [(5, nx // 2, ny // 2)], np.zeros((1, 3), np.int32) + 3)
+    shots = []
+    for k in range(nshots):
+        sx, sy = (int(v) for v in rng.integers(8, [nx - 8, ny - 8]))
+        side = int(rng.integers(30, 141))
+        xs = np.arange(sx - side // 2, sx - side // 2 + side)
+        ys = np.arange(sy - side // 2, sy - side // 2 + side)
+        xs = xs[(xs >= 0) & (xs < nx)]
+        ys = ys[(ys >= 0) & (ys < ny)]
+        gx, gy = np.meshgrid(xs, ys, indexing="ij")
+        rec = np.stack([np.full(gx.size, 3), gx.ravel(), gy.ravel()], axis=1).astype(np.int32) + npml
+        shots.append(ShotSpec((5 + npml, sx + npml, sy + npml), w.shots[0].stf, rec))
+    w.shots = shots
+    return w
+
+
+WORKLOADS = {"cfg1": cfg1, "cfg2": cfg2, "cfg3": cfg3}

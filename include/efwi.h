@@ -130,6 +130,31 @@ int efwi_set_tile(efwi_ctx* ctx, int kernel, int rows, int zdepth);
 int efwi_get_tile(const efwi_ctx* ctx, int kernel, int* rows, int* zdepth);
 int efwi_autotune(efwi_ctx* ctx, int reps, double* report, void* stream);
 
+/* Batched shots: run `nshots` shots of this grid and model together. Each
+ * stencil step of all shots is one persistent launch over a (shot, tile) work
+ * queue (per-XCD queues with stealing), the ring, receiver and injection work
+ * of all shots one launch each, and with use_graph != 0 each time step's
+ * launch sequence is captured once in a hipGraph and replayed (the time step
+ * lives in device memory). obs[k] / syn[k] are device arrays [3][nrec_k][nt]
+ * (syn may be NULL or hold NULLs); misfits[k] receives shot k's misfit.
+ * calc_id 1 keeps a separate gradient per shot (efwi_batch_gradient) and
+ * needs the full ring (memory_mode 0 or automatic) for every shot; sources
+ * must lie inside the C-PML-free interior; if_win must be 0. Per shot the
+ * results equal efwi_run_shot's bit for bit. The call returns when the batch
+ * is done. No reference counterpart (the reference runs shots one after
+ * another, Src/libCUFD.cu:237-620). */
+int efwi_run_batch(efwi_ctx* ctx, const efwi_shot* shots, int nshots, int calc_id, const float* const* obs,
+                   float* const* syn, double* misfits, int use_graph, void* stream);
+/* Work distribution of the batched stencil launches. Each persistent
+ * workgroup first runs a static share of its XCD's queue, static_pct percent
+ * (0..100) of an even split, and then claims the rest of the queues, at most
+ * `take` tiles per atomic (1..4096; lowered for small batches). Defaults:
+ * take 2, static_pct 0. Results do not depend on either. */
+int efwi_batch_config(efwi_ctx* ctx, int take, int static_pct);
+/* Gradient (lambda, mu, rho) of shot `shot` of the last efwi_run_batch. */
+int efwi_batch_gradient(efwi_ctx* ctx, int shot, float* g_lambda, float* g_mu, float* g_rho, int on_host,
+                        void* stream);
+
 /* FP64 dot product of two float device vectors on `stream` (per-block FP64
  * partial sums, then one FP64 reduction); *out is a host double. Used by the
  * L-BFGS two-loop recursion and gradient norms. No reference counterpart (the
