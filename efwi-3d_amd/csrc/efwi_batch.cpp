@@ -137,9 +137,10 @@ int Ctx::run_batch(const efwi_shot* shots, int n, int calc_id, const float* cons
     launch_fill(batch.syn_t[k], (long)nt * 3 * std::max(batch.nrec[k], 1), 0.0f, q);
   }
   HIP_TRY(hipGetLastError());
-  BatchLaunch L0{batch.shots_dev, n, batch.counters, batch.it_dev, batch.cus, batch.take, batch.static_pct};
+  BatchLaunch L0{batch.shots_dev, n, batch.counters, batch.it_dev, batch.cus, batch.take, batch.static_pct, 0};
   BatchLaunch L1 = L0;
   L1.counters = batch.counters + 8 * kQueueStride;
+  L1.reverse = 1;
   const std::string tiles_key = std::to_string(tiles[K_STRESS_FWD].rows) + "x" + std::to_string(tiles[K_STRESS_FWD].zdepth) +
                                 "," + std::to_string(tiles[K_VELOCITY_FWD].rows) + "x" + std::to_string(tiles[K_VELOCITY_FWD].zdepth) +
                                 "," + std::to_string(tiles[K_VELOCITY_BWD].rows) + "x" + std::to_string(tiles[K_VELOCITY_BWD].zdepth) +

@@ -103,6 +103,8 @@ struct BatchLaunch {
   int cus;   // compute units; persistent workgroups = cus x resident workgroups per CU
   int take;  // items taken per atomic
   int stat;  // launcher input: static share in percent; kernel input: static items per workgroup
+  int reverse;  // visit the shots last to first (consecutive launches alternate, so that
+                // each starts on the shot whose fields the previous one touched last)
 };
 
 }  // namespace efwi

@@ -913,7 +913,7 @@ __device__ __forceinline__ void persistent_items(const BatchLaunch& L, int per_x
   const bool leader = threadIdx.x == 0 && threadIdx.y == 0;
   auto run = [&](int q, int i) {
     const int shot = i / per_xcd;
-    item(shot, 8 * (i - shot * per_xcd) + q);
+    item(L.reverse ? L.nshots - 1 - shot : shot, 8 * (i - shot * per_xcd) + q);
   };
   // claim the first dynamic items now: the atomic's round trip overlaps the
   // static share
