@@ -1,13 +1,18 @@
 """Benchmark: fwd+adj+grad of the single-shot production configuration on one GPU.
 
     python bench.py [--gpus 1] [--steps K] [--warmup W] [--workload cfg2] [--nt NT]
-                    [--no-cpu-baseline] [--profile]
+                    [--no-cpu-baseline] [--no-profile] [--no-autotune]
 
 A "step" is one gradient evaluation of one shot: the forward loop, the misfit
 and adjoint source, and the backward loop (reverse reconstruction + adjoint +
 lambda/mu/rho imaging) over all nt time steps of the 200x200x120 (padded
 240x240x160), 3000-step workload. Inputs (model, observed data) are resident
 in HBM before the timed region. Prints one JSON line.
+
+Other workloads: --workload cfg3 (16 shots, batched launches against the same
+shots one by one, in shots/s) and --workload cfg4 (320x320x160, 4000 steps, 4
+shots one after another with sqrt(nt) checkpoints; a step is the summed
+gradient of the 4 shots).
 """
 from __future__ import annotations
 
@@ -38,6 +43,15 @@ def alg_bytes_per_launch(kernel, grid):
 def alg_bytes_pipeline(grid):
     npad, nint, nt = grid.ncells, grid.interior_cells(), grid.nt
     return 120.0 * npad * (nt - 1) + 180.0 * nint * (nt - 1) + 132.0 * npad * nt
+
+
+def ring_bytes(grid):
+    """Bytes of the boundary ring of all nt - 1 steps (5-cell band around the
+    C-PML-free interior, 9 fields), as the library sizes it."""
+    p = grid.npml
+    nzb, nxb, nyb = (n - 2 * p + 4 for n in grid.shape)
+    band = 10 * nxb * nyb + 10 * (nzb - 10) * nyb + 10 * (nzb - 10) * (nxb - 10)
+    return band * 9 * (grid.nt - 1) * 4
 
 
 def hbm_traffic(kernel, grid, workload):
@@ -202,15 +216,18 @@ def main():
     from efwi.propagator import Propagator
 
     w = synth.WORKLOADS[args.workload](**({} if args.nt is None else {"nt": args.nt}))
-    if len(w.shots) > 1:
-        return bench_batch(args, w)
     g = w.grid
-    shot = w.shots[0]
-    print(f"[bench] {w.name}: padded grid {g.shape}, nt={g.nt}, nrec={shot.nrec}", file=sys.stderr, flush=True)
+    if len(w.shots) > 1 and g.memory_mode != 1:
+        return bench_batch(args, w)
+    # one step: the summed gradient of every shot of the workload, shot after shot
+    shots = w.shots
+    nrec = sum(s.nrec for s in shots)
+    print(f"[bench] {w.name}: padded grid {g.shape}, nt={g.nt}, {len(shots)} shot(s), nrec={nrec}",
+          file=sys.stderr, flush=True)
     P = Propagator(g, 0)
     # observed data from the true model (untimed set-up)
     P.set_model(*(torch.from_numpy(a) for a in w.true))
-    _, obs, _ = P.run_shot(shot, 2)
+    obs = [P.run_shot(s, 2)[1] for s in shots]
     P.set_model(*(torch.from_numpy(a) for a in w.model))
     torch.cuda.synchronize()
 
@@ -226,8 +243,7 @@ def main():
 
     def step():
         P.zero_gradient()
-        j, _, _ = P.run_shot(shot, 1, obs)
-        return j
+        return sum(P.run_shot(s, 1, o)[0] for s, o in zip(shots, obs))
 
     for _ in range(args.warmup):
         step()
@@ -245,8 +261,9 @@ def main():
     t1 = time.perf_counter()
     prof = P.profile_read(reset=True) if not args.no_profile else {}
     dt_step = (t1 - t0) / args.steps
-    gcell = g.ncells * g.nt / dt_step / 1e9
-    pipe_gbs = alg_bytes_pipeline(g) / dt_step / 1e9
+    gcell = g.ncells * g.nt * len(shots) / dt_step / 1e9
+    pipe_gbs = alg_bytes_pipeline(g) * len(shots) / dt_step / 1e9
+    device_gb = P.device_bytes / 1e9
 
     roof = None
     kernels = {}
@@ -270,6 +287,12 @@ def main():
         print("[bench] cpu baseline ...", file=sys.stderr, flush=True)
         cpu = cpu_baseline(w, args.cpu_steps)
 
+    if g.memory_mode == 1:
+        seg = g.checkpoint_every or int(np.ceil(np.sqrt(g.nt - 1)))
+        recon = (f"sqrt(nt) checkpoints (segments of {seg} steps, boundary ring rebuilt per segment); "
+                 f"the ring of all steps would need {ring_bytes(g) / 1e9:.0f} GB")
+    else:
+        recon = "ring reconstruction"
     out = {
         "metric": "fwd+adj+grad Gcell-updates/s & %HBM peak",
         "value": round(gcell, 3),
@@ -285,7 +308,8 @@ def main():
         "data": "synthetic",
         "config": {"workload": f"{w.name}: {g.nz - 2 * g.npml}x{g.nx - 2 * g.npml}x{g.ny - 2 * g.npml} physical "
                                f"(z,x,y) + {g.npml} PML cells, padded {g.nz}x{g.nx}x{g.ny}, nt={g.nt}, "
-                               f"1 shot, {shot.nrec} receivers, fwd+adj+grad, ring reconstruction",
+                               f"{len(shots)} shot(s), {nrec} receivers, fwd+adj+grad, {recon}",
+                   "device_GB": round(device_gb, 2),
                    "tiles": {k: f"{r}x{z}" for k, (r, z) in tiles.items()},
                    "pipeline_alg_GBps": round(pipe_gbs, 1),
                    "pipeline_hbm_frac": round(pipe_gbs / HBM_PEAK_GBS, 4),

@@ -16,6 +16,7 @@ All coordinates returned by `workload` are on the padded grid.
 """
 from __future__ import annotations
 
+import dataclasses
 from dataclasses import dataclass
 
 import numpy as np
@@ -183,4 +184,31 @@ def cfg3(nt=500, nshots=16, nx=160, ny=160, nz=64, npml=10, seed=3):
     return w
 
 
-WORKLOADS = {"cfg1": cfg1, "cfg2": cfg2, "cfg3": cfg3}
+def cfg4(nt=4000, nshots=4, scale=1.0, seed=4):
+    """Differentiation with checkpointing: 160x320x320 physical grid (20 PML
+    cells), dx 10 m, dt 0.8 ms, 4000 steps; the plain-method recipe (4 seeded
+    layers, +10 % Gaussian vp anomaly at the centre with sigma 5 cells per 64,
+    vs = vp/1.73, rho = 310 vp^0.25) with vp up to 4200 m/s; 4 seeded shots
+    (Ricker, f0 10 Hz) at z 5, receivers every 2 cells at z 3. The boundary
+    ring of all steps would need ~300 GB, so the gradient runs with sqrt(nt)
+    checkpoints (memory_mode 1)."""
+    rng = np.random.default_rng(seed)
+    nz, nx, ny = int(round(160 * scale)), int(round(320 * scale)), int(round(320 * scale))
+    npml = max(4, int(round(20 * scale)))
+    interfaces = np.sort(rng.integers(nz // 8, nz - nz // 8, size=3))
+    vp_b = layered(nz, nx, ny, [2000.0, 2700.0, 3400.0, 4200.0], interfaces)
+    vs_b = vp_b / 1.73
+    rho_b = 310.0 * vp_b ** 0.25
+    vp_a = vp_b * (1.0 + 0.10 * gaussian(nz, nx, ny, (nz // 2, nx // 2, ny // 2), 5.0 * nx / 64))
+    xs = np.arange(0, nx, 2)
+    ys = np.arange(0, ny, 2)
+    gx, gy = np.meshgrid(xs, ys, indexing="ij")
+    rec = np.stack([np.full(gx.size, 3), gx.ravel(), gy.ravel()], axis=1)
+    src = [(5, int(a), int(b)) for a, b in zip(rng.integers(nx // 4, 3 * nx // 4, nshots),
+                                               rng.integers(ny // 4, 3 * ny // 4, nshots))]
+    w = _finish("cfg4", nz, nx, ny, npml, 10.0, 8e-4, nt, 10.0, vp_b, vs_b, rho_b, vp_a, vs_b, rho_b, src, rec)
+    w.grid = dataclasses.replace(w.grid, memory_mode=1)
+    return w
+
+
+WORKLOADS = {"cfg1": cfg1, "cfg2": cfg2, "cfg3": cfg3, "cfg4": cfg4}
