@@ -13,6 +13,17 @@ __global__ void k_fill(float* __restrict__ p, long n, float v) {
     p[i] = v;
 }
 
+// Source illumination of the pseudo-Hessian (diagonal of J^T J for a point
+// source, Shin et al. 2001): illum += vz^2 + vx^2 + vy^2 of the forward field
+// of this step. No reference counterpart.
+__global__ void k_illum(const float* __restrict__ vz, const float* __restrict__ vx, const float* __restrict__ vy,
+                        float* __restrict__ illum, long n) {
+  for (long i = (long)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (long)gridDim.x * blockDim.x) {
+    const float a = vz[i], b = vx[i], c = vy[i];
+    illum[i] += a * a + b * b + c * c;
+  }
+}
+
 // Byc = 1/rho on [2, n-3]^3 and 1/1000 elsewhere (Src/Model.cu:65,
 // Src/utilities.cu:117-128). The reference divides in double and stores
 // float; a correctly rounded float division gives the same float because
@@ -265,6 +276,9 @@ __global__ void k_step_add(int* it, int delta) {
 
 void launch_fill(float* p, long n, float v, hipStream_t s) {
   hipLaunchKernelGGL(k_fill, dim3(blocks_for(n, 256, 8192)), dim3(256), 0, s, p, n, v);
+}
+void launch_illum(const float* vz, const float* vx, const float* vy, float* illum, long n, hipStream_t s) {
+  hipLaunchKernelGGL(k_illum, dim3(blocks_for(n, 256, 8192)), dim3(256), 0, s, vz, vx, vy, illum, n);
 }
 void launch_byc(const Geom& G, const float* rho, float* byc, hipStream_t s) {
   hipLaunchKernelGGL(k_byc, dim3(blocks_for((long)G.nz * G.pxy, 256, 8192)), dim3(256), 0, s, G, rho, byc);

@@ -461,6 +461,7 @@ int Ctx::zero_gradient(hipStream_t s) {
   launch_fill(Gr.glam, N, 0.0f, s);
   launch_fill(Gr.gmu, N, 0.0f, s);
   launch_fill(Gr.grho, N, 0.0f, s);
+  if (illum_on) launch_fill(illum, N, 0.0f, s);
   HIP_TRY(hipGetLastError());
   return EFWI_OK;
 }
@@ -567,6 +568,7 @@ int Ctx::run_shot(const efwi_shot* sh, int calc_id, const float* obs, float* syn
       prof.end(s);
     }
     forward_step(F, W, Sf, it, true, s);
+    if (illum_on) launch_illum(W.vz, W.vx, W.vy, illum, N, s);
   }
   HIP_TRY(hipGetLastError());
   if (syn && nrec > 0) launch_from_time_major(syn_t, syn, nrec, nt, s);
@@ -758,6 +760,32 @@ int efwi_get_gradient(efwi_ctx* ctx, float* g_lambda, float* g_mu, float* g_rho,
                       void* stream) {
   if (!ctx) return fail(EFWI_ERR_ARG, "NULL context");
   return reinterpret_cast<Ctx*>(ctx)->get_gradient(g_lambda, g_mu, g_rho, on_host, (hipStream_t)stream);
+}
+
+int efwi_set_illumination(efwi_ctx* ctx, int enable, void* stream) {
+  if (!ctx) return fail(EFWI_ERR_ARG, "NULL context");
+  Ctx* c = reinterpret_cast<Ctx*>(ctx);
+  HIP_TRY(hipSetDevice(c->cfg.device));
+  c->illum_on = enable != 0;
+  if (c->illum_on) {
+    int rc;
+    if (!c->illum && (rc = c->alloc((void**)&c->illum, sizeof(float) * (size_t)c->N))) return rc;
+    efwi::launch_fill(c->illum, c->N, 0.0f, (hipStream_t)stream);
+    HIP_TRY(hipGetLastError());
+  }
+  return EFWI_OK;
+}
+
+int efwi_get_illumination(efwi_ctx* ctx, float* out, int on_host, void* stream) {
+  if (!ctx || !out) return fail(EFWI_ERR_ARG, "NULL argument");
+  Ctx* c = reinterpret_cast<Ctx*>(ctx);
+  if (!c->illum) return fail(EFWI_ERR_STATE, "illumination is off (efwi_set_illumination)");
+  HIP_TRY(hipSetDevice(c->cfg.device));
+  hipStream_t s = (hipStream_t)stream;
+  HIP_TRY(hipMemcpyAsync(out, c->illum, sizeof(float) * (size_t)c->N,
+                         on_host ? hipMemcpyDeviceToHost : hipMemcpyDeviceToDevice, s));
+  if (on_host) HIP_TRY(hipStreamSynchronize(s));
+  return EFWI_OK;
 }
 
 int efwi_profile(efwi_ctx* ctx, int enable) {

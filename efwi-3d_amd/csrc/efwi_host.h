@@ -103,6 +103,8 @@ struct Ctx {
   size_t res_cap = 0;
   float* fac_dev = nullptr;
   size_t fac_cap = 0;
+  float* illum = nullptr;  // source illumination sum_t |v|^2 over shots (efwi_set_illumination)
+  bool illum_on = false;
   double* partials = nullptr;
   double* msum = nullptr;
   int nrec = 0;

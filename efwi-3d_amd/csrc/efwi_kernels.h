@@ -59,6 +59,7 @@ int device_cus(int device);
 
 // auxiliary kernels (efwi_aux.hip)
 void launch_fill(float* p, long n, float v, hipStream_t s);
+void launch_illum(const float* vz, const float* vx, const float* vy, float* illum, long n, hipStream_t s);
 void launch_byc(const Geom& G, const float* rho, float* byc, hipStream_t s);
 void launch_add_source(float* szz, float* sxx, float* syy, long cell, const float* stf, int it,
                        float dt, int sign, hipStream_t s);

@@ -1,0 +1,548 @@
+// Kernels of the learned preconditioner: a 3-D U-Net with 3x3x3 convolutions,
+// BF16 weights and activations, FP32 accumulation, on gfx950 matrix cores.
+//
+// Tensors are channels-last volumes: element (voxel v, channel c) at
+// p[v * ld + c], voxel v = (z * nx + x) * ny + y (y fastest, as the FWI grid),
+// ld (the channel stride) a multiple of 16, so a tensor can be a channel slice
+// of a wider one (the U-Net's skip concatenations are written in place).
+//
+// Convolution as implicit GEMM on v_mfma_f32_16x16x16_bf16 (D = A B + C,
+// 16x16x16, one wavefront). Lane l holds A[l % 16][4 (l / 16) + j],
+// B[4 (l / 16) + j][l % 16] and D[4 (l / 16) + j][l % 16], j = 0..3. A
+// wavefront owns 16 consecutive voxels along y: A rows are those voxels, the
+// K dimension runs over the 27 taps x input channels in chunks of 16, and B
+// columns are output channels, 16 per accumulator block.
+//   forward:  out[v][n] = sum_t sum_k in[v + d_t][k] w[t][n][k]      (+ bias, ReLU)
+//   dgrad:    din[v][n] = sum_t sum_k dout[v - d_t][k] w[t][k][n]    (x ReLU mask)
+//   wgrad:    dw[t][m][n] = sum_v dpre[v][m] in[v + d_t][n]          (K = voxels)
+// with zero padding at the volume faces. No reference counterpart: the
+// reference has no learned part.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cstdint>
+
+#include "../../include/efwi_nn.h"
+
+namespace {
+
+typedef short s4 __attribute__((ext_vector_type(4)));
+typedef float f4 __attribute__((ext_vector_type(4)));
+
+__device__ __forceinline__ float bf2f(uint16_t h) { return __uint_as_float((unsigned)h << 16); }
+// round to nearest even (finite inputs)
+__device__ __forceinline__ uint16_t f2bf(float f) {
+  unsigned u = __float_as_uint(f);
+  u += 0x7FFFu + ((u >> 16) & 1u);
+  return (uint16_t)(u >> 16);
+}
+
+struct Vol {
+  int nz, nx, ny;
+  __device__ __forceinline__ long vox(int z, int x, int y) const { return ((long)z * nx + x) * ny + y; }
+  __device__ __forceinline__ bool in(int z, int x, int y) const {
+    return z >= 0 && z < nz && x >= 0 && x < nx && y >= 0 && y < ny;
+  }
+};
+
+__device__ __forceinline__ void tap(int t, int& dz, int& dx, int& dy) {
+  dz = t / 9 - 1;
+  dx = (t / 3) % 3 - 1;
+  dy = t % 3 - 1;
+}
+
+// One wavefront per 16-voxel row segment; NB output blocks of 16 channels
+// starting at block blockIdx.y * NB. MODE 0 forward, 1 dgrad.
+template <int NB, int MODE>
+__global__ __launch_bounds__(256) void k_conv3(const uint16_t* __restrict__ src, int lds, int kc,
+                                               const uint16_t* __restrict__ w, const float* __restrict__ bias,
+                                               uint16_t* __restrict__ dst, int ldd, int nc,
+                                               const uint16_t* __restrict__ mask, int ldm, Vol V, int relu) {
+  const int lane = threadIdx.x & 63;
+  const long tile = (long)blockIdx.x * 4 + (threadIdx.x >> 6);
+  const int tpr = (V.ny + 15) / 16;
+  const long rows = (long)V.nz * V.nx;
+  if (tile >= rows * tpr) return;  // wavefront-uniform
+  const int row = (int)(tile / tpr);
+  const int y0 = (int)(tile - (long)row * tpr) * 16;
+  const int z = row / V.nx, x = row - z * V.nx;
+  const int i = lane & 15, kq = (lane >> 4) * 4;
+  const int nb0 = blockIdx.y * NB, nbt = nc / 16;
+  f4 acc[NB];
+#pragma unroll
+  for (int b = 0; b < NB; b++) acc[b] = f4{0.f, 0.f, 0.f, 0.f};
+  const int sgn = MODE == 0 ? 1 : -1;
+  for (int t = 0; t < 27; t++) {
+    int dz, dx, dy;
+    tap(t, dz, dx, dy);
+    const int zs = z + sgn * dz, xs = x + sgn * dx, ys = y0 + i + sgn * dy;
+    const bool ok = V.in(zs, xs, ys);
+    const long vs = ok ? V.vox(zs, xs, ys) : 0;
+    for (int kb = 0; kb < kc; kb += 16) {
+      s4 a = s4{0, 0, 0, 0};
+      if (ok) a = *reinterpret_cast<const s4*>(src + vs * lds + kb + kq);
+#pragma unroll
+      for (int b = 0; b < NB; b++) {
+        if (nb0 + b >= nbt) break;
+        const int n = (nb0 + b) * 16 + i;
+        s4 bw;
+        if (MODE == 0) {
+          bw = *reinterpret_cast<const s4*>(w + ((long)t * nc + n) * kc + kb + kq);  // w[t][n][k]
+        } else {
+#pragma unroll
+          for (int j = 0; j < 4; j++) bw[j] = (short)w[((long)t * kc + kb + kq + j) * nc + n];  // w[t][k][n]
+        }
+        acc[b] = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(a, bw, acc[b], 0, 0, 0);
+      }
+    }
+  }
+#pragma unroll
+  for (int b = 0; b < NB; b++) {
+    if (nb0 + b >= nbt) break;
+    const int n = (nb0 + b) * 16 + i;
+    const float bn = bias ? bias[n] : 0.f;
+#pragma unroll
+    for (int j = 0; j < 4; j++) {
+      const int y = y0 + kq + j;
+      if (y >= V.ny) continue;
+      const long v = V.vox(z, x, y);
+      float r = acc[b][j] + bn;
+      if (relu) r = fmaxf(r, 0.f);
+      if (mask && !(bf2f(mask[v * ldm + n]) > 0.f)) r = 0.f;
+      dst[v * ldd + n] = f2bf(r);
+    }
+  }
+}
+
+// Weight gradient partials: wavefront (chunk, tap, m-block) runs `per` row
+// segments of 16 voxels and writes part[chunk][t][m][n] for its 16 m and all n.
+template <int NBN>
+__global__ __launch_bounds__(64) void k_wgrad(const uint16_t* __restrict__ dpre, int ldp, int mc,
+                                              const uint16_t* __restrict__ src, int lds, int kc,
+                                              float* __restrict__ part, Vol V, int per) {
+  const int lane = threadIdx.x;
+  const int chunk = blockIdx.x, t = blockIdx.y, mb = blockIdx.z;
+  const int i = lane & 15, kq = (lane >> 4) * 4;
+  const int tpr = (V.ny + 15) / 16;
+  const long ntiles = (long)V.nz * V.nx * tpr;
+  int dz, dx, dy;
+  tap(t, dz, dx, dy);
+  f4 acc[NBN];
+#pragma unroll
+  for (int b = 0; b < NBN; b++) acc[b] = f4{0.f, 0.f, 0.f, 0.f};
+  const int m = mb * 16 + i;
+  for (long tile = (long)chunk * per; tile < std::min(ntiles, (long)(chunk + 1) * per); tile++) {
+    const int row = (int)(tile / tpr);
+    const int y0 = (int)(tile - (long)row * tpr) * 16;
+    const int z = row / V.nx, x = row - z * V.nx;
+    s4 a;  // A[m][k] = dpre[voxel k][m]
+#pragma unroll
+    for (int j = 0; j < 4; j++) {
+      const int y = y0 + kq + j;
+      a[j] = y < V.ny ? (short)dpre[V.vox(z, x, y) * ldp + m] : (short)0;
+    }
+    // B[k][n] = src[voxel k + d_t][n]
+    const int zs = z + dz, xs = x + dx;
+#pragma unroll
+    for (int b = 0; b < NBN; b++) {
+      const int n = b * 16 + i;
+      s4 bw;
+#pragma unroll
+      for (int j = 0; j < 4; j++) {
+        const int y = y0 + kq + j, ys = y + dy;
+        bw[j] = (y < V.ny && V.in(zs, xs, ys)) ? (short)src[V.vox(zs, xs, ys) * lds + n] : (short)0;
+      }
+      acc[b] = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(a, bw, acc[b], 0, 0, 0);
+    }
+  }
+  // D[m][n]: lane holds m = mb*16 + kq + j, n = b*16 + i
+  float* out = part + ((long)chunk * 27 + t) * mc * kc;
+#pragma unroll
+  for (int b = 0; b < NBN; b++)
+#pragma unroll
+    for (int j = 0; j < 4; j++) out[(long)(mb * 16 + kq + j) * kc + b * 16 + i] = acc[b][j];
+}
+
+__global__ void k_sum_chunks(const float* __restrict__ part, long width, int nchunks, float* __restrict__ out) {
+  const long k = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (k >= width) return;
+  float s = 0.f;
+  for (int c = 0; c < nchunks; c++) s += part[(long)c * width + k];
+  out[k] = s;
+}
+
+// out[c] += sum_v x[v][c] (bias gradients); blockDim 256, c < 256 / stride groups
+__global__ void k_chan_sum(const uint16_t* __restrict__ x, int ld, int c, long nvox, float* __restrict__ out) {
+  const int lanes = 256 / c;  // voxel lanes per channel
+  const int ch = threadIdx.x % c, vl = threadIdx.x / c;
+  if (vl >= lanes) return;
+  float s = 0.f;
+  for (long v = (long)blockIdx.x * lanes + vl; v < nvox; v += (long)gridDim.x * lanes) s += bf2f(x[v * ld + ch]);
+  atomicAdd(out + ch, s);
+}
+
+// 2x2x2 average pooling; (nz, nx, ny) are the fine dimensions (even)
+__global__ void k_pool2(const uint16_t* __restrict__ src, int lds, uint16_t* __restrict__ dst, int ldd, int c, Vol F) {
+  const Vol Cv{F.nz / 2, F.nx / 2, F.ny / 2};
+  const long n = (long)Cv.nz * Cv.nx * Cv.ny * c;
+  for (long k = (long)blockIdx.x * blockDim.x + threadIdx.x; k < n; k += (long)gridDim.x * blockDim.x) {
+    const int ch = (int)(k % c);
+    long v = k / c;
+    const int y = (int)(v % Cv.ny); v /= Cv.ny;
+    const int x = (int)(v % Cv.nx);
+    const int z = (int)(v / Cv.nx);
+    float s = 0.f;
+    for (int q = 0; q < 8; q++) s += bf2f(src[F.vox(2 * z + (q >> 2), 2 * x + ((q >> 1) & 1), 2 * y + (q & 1)) * lds + ch]);
+    dst[Cv.vox(z, x, y) * ldd + ch] = f2bf(0.125f * s);
+  }
+}
+// gradient of the pooling: d_fine[v][c] (=|+=) d_coarse[parent][c] / 8
+__global__ void k_pool2_bwd(const uint16_t* __restrict__ dc, int ldc, uint16_t* __restrict__ df, int ldf, int c, Vol F,
+                            int accumulate) {
+  const long n = (long)F.nz * F.nx * F.ny * c;
+  for (long k = (long)blockIdx.x * blockDim.x + threadIdx.x; k < n; k += (long)gridDim.x * blockDim.x) {
+    const int ch = (int)(k % c);
+    long v = k / c;
+    const int y = (int)(v % F.ny); v /= F.ny;
+    const int x = (int)(v % F.nx);
+    const int z = (int)(v / F.nx);
+    const Vol Cv{F.nz / 2, F.nx / 2, F.ny / 2};
+    float g = 0.125f * bf2f(dc[Cv.vox(z / 2, x / 2, y / 2) * ldc + ch]);
+    const long o = F.vox(z, x, y) * ldf + ch;
+    if (accumulate) g += bf2f(df[o]);
+    df[o] = f2bf(g);
+  }
+}
+// nearest upsampling x2: fine[v][c] = coarse[parent][c]
+__global__ void k_up2(const uint16_t* __restrict__ cs, int ldc, uint16_t* __restrict__ fd, int ldf, int c, Vol F) {
+  const long n = (long)F.nz * F.nx * F.ny * c;
+  const Vol Cv{F.nz / 2, F.nx / 2, F.ny / 2};
+  for (long k = (long)blockIdx.x * blockDim.x + threadIdx.x; k < n; k += (long)gridDim.x * blockDim.x) {
+    const int ch = (int)(k % c);
+    long v = k / c;
+    const int y = (int)(v % F.ny); v /= F.ny;
+    const int x = (int)(v % F.nx);
+    const int z = (int)(v / F.nx);
+    fd[F.vox(z, x, y) * ldf + ch] = cs[Cv.vox(z / 2, x / 2, y / 2) * ldc + ch];
+  }
+}
+// gradient of the upsampling: d_coarse[p][c] = sum of the 8 children, times the ReLU mask of the coarse activation
+__global__ void k_up2_bwd(const uint16_t* __restrict__ df, int ldf, uint16_t* __restrict__ dc, int ldc, int c, Vol F,
+                          const uint16_t* __restrict__ mask, int ldm) {
+  const Vol Cv{F.nz / 2, F.nx / 2, F.ny / 2};
+  const long n = (long)Cv.nz * Cv.nx * Cv.ny * c;
+  for (long k = (long)blockIdx.x * blockDim.x + threadIdx.x; k < n; k += (long)gridDim.x * blockDim.x) {
+    const int ch = (int)(k % c);
+    long v = k / c;
+    const int y = (int)(v % Cv.ny); v /= Cv.ny;
+    const int x = (int)(v % Cv.nx);
+    const int z = (int)(v / Cv.nx);
+    float s = 0.f;
+    for (int q = 0; q < 8; q++) s += bf2f(df[F.vox(2 * z + (q >> 2), 2 * x + ((q >> 1) & 1), 2 * y + (q & 1)) * ldf + ch]);
+    const long o = Cv.vox(z, x, y);
+    if (mask && !(bf2f(mask[o * ldm + ch]) > 0.f)) s = 0.f;
+    dc[o * ldc + ch] = f2bf(s);
+  }
+}
+// a[v][c] = (a[v][c] + b[v][c]) x (mask[v][c] > 0 when given)
+__global__ void k_add_masked(uint16_t* __restrict__ a, int lda, const uint16_t* __restrict__ b, int ldb, int c, long nvox,
+                             const uint16_t* __restrict__ mask, int ldm) {
+  const long n = nvox * c;
+  for (long k = (long)blockIdx.x * blockDim.x + threadIdx.x; k < n; k += (long)gridDim.x * blockDim.x) {
+    const long v = k / c;
+    const int ch = (int)(k - v * c);
+    float s = bf2f(a[v * lda + ch]) + (b ? bf2f(b[v * ldb + ch]) : 0.f);
+    if (mask && !(bf2f(mask[v * ldm + ch]) > 0.f)) s = 0.f;
+    a[v * lda + ch] = f2bf(s);
+  }
+}
+
+// 1x1x1 output head in FP32: out[v] = sum_c w[c] act[v][c] + b
+__global__ void k_head(const uint16_t* __restrict__ act, int ld, int c, const float* __restrict__ w, const float* __restrict__ b,
+                       float* __restrict__ out, long nvox) {
+  for (long v = (long)blockIdx.x * blockDim.x + threadIdx.x; v < nvox; v += (long)gridDim.x * blockDim.x) {
+    float s = b[0];
+    for (int ch = 0; ch < c; ch++) s += w[ch] * bf2f(act[v * ld + ch]);
+    out[v] = s;
+  }
+}
+// head backward: dact[v][c] = w[c] dout[v] (x ReLU mask of act); dw[c] += sum_v act dout; db += sum_v dout
+__global__ void k_head_bwd(const uint16_t* __restrict__ act, int ld, int c, const float* __restrict__ w,
+                           const float* __restrict__ dout, uint16_t* __restrict__ dact, int ldd, float* __restrict__ dw,
+                           float* __restrict__ db, long nvox) {
+  __shared__ float red[17][64];
+  float acc[17];
+  for (int k = 0; k < 17; k++) acc[k] = 0.f;
+  for (long v = (long)blockIdx.x * blockDim.x + threadIdx.x; v < nvox; v += (long)gridDim.x * blockDim.x) {
+    const float g = dout[v];
+    for (int ch = 0; ch < c; ch++) {
+      const float a = bf2f(act[v * ld + ch]);
+      dact[v * ldd + ch] = f2bf(a > 0.f ? w[ch] * g : 0.f);
+      acc[ch] += a * g;
+    }
+    acc[16] += g;
+  }
+  // blockDim.x == 64
+  for (int k = 0; k < 17; k++) red[k][threadIdx.x] = acc[k];
+  __syncthreads();
+  if (threadIdx.x < 17) {
+    float s = 0.f;
+    for (int q = 0; q < 64; q++) s += red[threadIdx.x][q];
+    if (threadIdx.x < c) atomicAdd(dw + threadIdx.x, s);
+    if (threadIdx.x == 16) atomicAdd(db, s);
+  }
+}
+
+// masked mean-square loss: dout = 2 (out - target) m / sum(m); loss += sum m (out - target)^2 / sum(m)
+__global__ void k_mse(const float* __restrict__ out, const float* __restrict__ target, const float* __restrict__ m,
+                      long nvox, float inv_count, float* __restrict__ dout, double* __restrict__ loss) {
+  __shared__ double red[256];
+  double s = 0.0;
+  for (long v = (long)blockIdx.x * blockDim.x + threadIdx.x; v < nvox; v += (long)gridDim.x * blockDim.x) {
+    const float d = (out[v] - target[v]) * m[v];
+    dout[v] = 2.f * d * inv_count;
+    s += (double)d * d;
+  }
+  red[threadIdx.x] = s;
+  __syncthreads();
+  for (int k = 128; k > 0; k >>= 1) {
+    if (threadIdx.x < k) red[threadIdx.x] += red[threadIdx.x + k];
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) atomicAdd(loss, red[0] * inv_count);
+}
+
+// Adam on FP32 master weights; the BF16 copy (when given) is refreshed
+__global__ void k_adam(float* __restrict__ w, const float* __restrict__ g, float* __restrict__ m1, float* __restrict__ m2,
+                       long n, float lr, float b1, float b2, float eps, float c1, float c2, uint16_t* __restrict__ w16) {
+  for (long k = (long)blockIdx.x * blockDim.x + threadIdx.x; k < n; k += (long)gridDim.x * blockDim.x) {
+    const float gk = g[k];
+    const float a = b1 * m1[k] + (1.f - b1) * gk;
+    const float b = b2 * m2[k] + (1.f - b2) * gk * gk;
+    m1[k] = a;
+    m2[k] = b;
+    const float wk = w[k] - lr * (a * c1) / (sqrtf(b * c2) + eps);
+    w[k] = wk;
+    if (w16) w16[k] = f2bf(wk);
+  }
+}
+
+__global__ void k_to_bf16(const float* __restrict__ x, uint16_t* __restrict__ y, long n) {
+  for (long k = (long)blockIdx.x * blockDim.x + threadIdx.x; k < n; k += (long)gridDim.x * blockDim.x) y[k] = f2bf(x[k]);
+}
+
+// max |x| as float bits (non-negative floats order as unsigned integers)
+__global__ void k_absmax(const float* __restrict__ x, long n, unsigned* __restrict__ out) {
+  __shared__ float red[256];
+  float s = 0.f;
+  for (long k = (long)blockIdx.x * blockDim.x + threadIdx.x; k < n; k += (long)gridDim.x * blockDim.x) s = fmaxf(s, fabsf(x[k]));
+  red[threadIdx.x] = s;
+  __syncthreads();
+  for (int k = 128; k > 0; k >>= 1) {
+    if (threadIdx.x < k) red[threadIdx.x] = fmaxf(red[threadIdx.x], red[threadIdx.x + k]);
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) atomicMax(out, __float_as_uint(red[0]));
+}
+
+// Network input from a padded FWI grid (npml cells per face): channel 0 =
+// g / gmax, channel 1 = log10(illum / imax + 1e-6) / 6 + 1, channels 2..15 zero.
+__global__ void k_pack(const float* __restrict__ g, const float* __restrict__ illum, Vol P, int p, float inv_g,
+                       float inv_i, uint16_t* __restrict__ feat) {
+  const Vol V{P.nz - 2 * p, P.nx - 2 * p, P.ny - 2 * p};
+  const long n = (long)V.nz * V.nx * V.ny;
+  for (long v = (long)blockIdx.x * blockDim.x + threadIdx.x; v < n; v += (long)gridDim.x * blockDim.x) {
+    const int y = (int)(v % V.ny);
+    const long r = v / V.ny;
+    const int x = (int)(r % V.nx), z = (int)(r / V.nx);
+    const long q = P.vox(z + p, x + p, y + p);
+    uint16_t* f = feat + v * 16;
+    f[0] = f2bf(g[q] * inv_g);
+    f[1] = f2bf(log10f(illum[q] * inv_i + 1e-6f) / 6.f + 1.f);
+    for (int c = 2; c < 16; c++) f[c] = 0;
+  }
+}
+
+__global__ void k_zero(unsigned* __restrict__ p, int n) {
+  const int k = blockIdx.x * blockDim.x + threadIdx.x;
+  if (k < n) p[k] = 0u;
+}
+
+inline unsigned grid_for(long n, int block, long cap = 16384) {
+  return (unsigned)std::max(1L, std::min(cap, (n + block - 1) / block));
+}
+
+}  // namespace
+
+extern "C" {
+
+static int nn_check(hipStream_t s) {
+  (void)s;
+  return hipGetLastError() == hipSuccess ? 0 : EFWI_NN_ERR_HIP;
+}
+
+int efwi_nn_conv3(const uint16_t* src, int lds, int kc, const uint16_t* w, const float* bias, uint16_t* dst, int ldd,
+                  int nc, const uint16_t* mask, int ldm, int nz, int nx, int ny, int mode, int relu, void* stream) {
+  if (!src || !w || !dst || kc <= 0 || nc <= 0 || kc % 16 || nc % 16 || lds % 16 || ldd % 16 || lds < kc ||
+      ldd < nc || (mask && (ldm % 16 || ldm < nc)) || nz <= 0 || nx <= 0 || ny <= 0 || (mode != 0 && mode != 1))
+    return EFWI_NN_ERR_ARG;
+  hipStream_t s = (hipStream_t)stream;
+  const Vol V{nz, nx, ny};
+  const long tiles = (long)nz * nx * ((ny + 15) / 16);
+  const int nbt = nc / 16;
+  const int NB = nbt >= 4 ? 4 : nbt;  // output blocks per wavefront
+  const dim3 grid((unsigned)((tiles + 3) / 4), (unsigned)((nbt + NB - 1) / NB));
+#define EFWI_NN_CONV(B)                                                                                          \
+  if (mode == 0)                                                                                               \
+    hipLaunchKernelGGL((k_conv3<B, 0>), grid, dim3(256), 0, s, src, lds, kc, w, bias, dst, ldd, nc, mask, ldm, V, relu); \
+  else                                                                                                         \
+    hipLaunchKernelGGL((k_conv3<B, 1>), grid, dim3(256), 0, s, src, lds, kc, w, bias, dst, ldd, nc, mask, ldm, V, relu)
+  if (NB == 1) EFWI_NN_CONV(1);
+  else if (NB == 2) EFWI_NN_CONV(2);
+  else if (NB == 3) EFWI_NN_CONV(3);
+  else EFWI_NN_CONV(4);
+#undef EFWI_NN_CONV
+  return nn_check(s);
+}
+
+long efwi_nn_wgrad_scratch(int mc, int kc, int nz, int nx, int ny, int per) {
+  const long tiles = (long)nz * nx * ((ny + 15) / 16);
+  const long chunks = (tiles + per - 1) / per;
+  return chunks * 27L * mc * kc;
+}
+
+int efwi_nn_wgrad(const uint16_t* dpre, int ldp, int mc, const uint16_t* src, int lds, int kc, float* scratch,
+                  long scratch_floats, float* dw, int nz, int nx, int ny, int per, void* stream) {
+  if (!dpre || !src || !scratch || !dw || mc % 16 || kc % 16 || mc <= 0 || kc <= 0 || kc > 96 || per <= 0 ||
+      ldp % 16 || lds % 16 || ldp < mc || lds < kc)
+    return EFWI_NN_ERR_ARG;
+  if (efwi_nn_wgrad_scratch(mc, kc, nz, nx, ny, per) > scratch_floats) return EFWI_NN_ERR_ARG;
+  hipStream_t s = (hipStream_t)stream;
+  const Vol V{nz, nx, ny};
+  const long tiles = (long)nz * nx * ((ny + 15) / 16);
+  const int chunks = (int)((tiles + per - 1) / per);
+  const dim3 grid((unsigned)chunks, 27, (unsigned)(mc / 16));
+  switch (kc / 16) {
+    case 1: hipLaunchKernelGGL((k_wgrad<1>), grid, dim3(64), 0, s, dpre, ldp, mc, src, lds, kc, scratch, V, per); break;
+    case 2: hipLaunchKernelGGL((k_wgrad<2>), grid, dim3(64), 0, s, dpre, ldp, mc, src, lds, kc, scratch, V, per); break;
+    case 3: hipLaunchKernelGGL((k_wgrad<3>), grid, dim3(64), 0, s, dpre, ldp, mc, src, lds, kc, scratch, V, per); break;
+    case 4: hipLaunchKernelGGL((k_wgrad<4>), grid, dim3(64), 0, s, dpre, ldp, mc, src, lds, kc, scratch, V, per); break;
+    case 5: hipLaunchKernelGGL((k_wgrad<5>), grid, dim3(64), 0, s, dpre, ldp, mc, src, lds, kc, scratch, V, per); break;
+    default: hipLaunchKernelGGL((k_wgrad<6>), grid, dim3(64), 0, s, dpre, ldp, mc, src, lds, kc, scratch, V, per); break;
+  }
+  const long width = 27L * mc * kc;
+  hipLaunchKernelGGL(k_sum_chunks, dim3(grid_for(width, 256, 1L << 30)), dim3(256), 0, s, scratch, width, chunks, dw);
+  return nn_check(s);
+}
+
+int efwi_nn_chan_sum(const uint16_t* x, int ld, int c, long nvox, float* out, void* stream) {
+  if (!x || !out || c <= 0 || c > 256 || 256 % c) return EFWI_NN_ERR_ARG;
+  hipStream_t s = (hipStream_t)stream;
+  hipLaunchKernelGGL(k_zero, dim3((c + 255) / 256), dim3(256), 0, s, reinterpret_cast<unsigned*>(out), c);
+  hipLaunchKernelGGL(k_chan_sum, dim3(grid_for(nvox * c, 256, 2048)), dim3(256), 0, s, x, ld, c, nvox, out);
+  return nn_check(s);
+}
+
+int efwi_nn_pool2(const uint16_t* src, int lds, uint16_t* dst, int ldd, int c, int nz, int nx, int ny, void* stream) {
+  if (!src || !dst || nz % 2 || nx % 2 || ny % 2 || c <= 0) return EFWI_NN_ERR_ARG;
+  hipStream_t s = (hipStream_t)stream;
+  hipLaunchKernelGGL(k_pool2, dim3(grid_for((long)nz * nx * ny / 8 * c, 256)), dim3(256), 0, s, src, lds, dst, ldd, c,
+                     Vol{nz, nx, ny});
+  return nn_check(s);
+}
+
+int efwi_nn_pool2_bwd(const uint16_t* dc, int ldc, uint16_t* df, int ldf, int c, int nz, int nx, int ny, int accumulate,
+                      void* stream) {
+  if (!dc || !df || nz % 2 || nx % 2 || ny % 2 || c <= 0) return EFWI_NN_ERR_ARG;
+  hipStream_t s = (hipStream_t)stream;
+  hipLaunchKernelGGL(k_pool2_bwd, dim3(grid_for((long)nz * nx * ny * c, 256)), dim3(256), 0, s, dc, ldc, df, ldf, c,
+                     Vol{nz, nx, ny}, accumulate);
+  return nn_check(s);
+}
+
+int efwi_nn_up2(const uint16_t* cs, int ldc, uint16_t* fd, int ldf, int c, int nz, int nx, int ny, void* stream) {
+  if (!cs || !fd || nz % 2 || nx % 2 || ny % 2 || c <= 0) return EFWI_NN_ERR_ARG;
+  hipStream_t s = (hipStream_t)stream;
+  hipLaunchKernelGGL(k_up2, dim3(grid_for((long)nz * nx * ny * c, 256)), dim3(256), 0, s, cs, ldc, fd, ldf, c,
+                     Vol{nz, nx, ny});
+  return nn_check(s);
+}
+
+int efwi_nn_up2_bwd(const uint16_t* df, int ldf, uint16_t* dc, int ldc, int c, int nz, int nx, int ny,
+                    const uint16_t* mask, int ldm, void* stream) {
+  if (!df || !dc || nz % 2 || nx % 2 || ny % 2 || c <= 0) return EFWI_NN_ERR_ARG;
+  hipStream_t s = (hipStream_t)stream;
+  hipLaunchKernelGGL(k_up2_bwd, dim3(grid_for((long)nz * nx * ny / 8 * c, 256)), dim3(256), 0, s, df, ldf, dc, ldc, c,
+                     Vol{nz, nx, ny}, mask, ldm);
+  return nn_check(s);
+}
+
+int efwi_nn_add(uint16_t* a, int lda, const uint16_t* b, int ldb, int c, long nvox, const uint16_t* mask, int ldm,
+                void* stream) {
+  if (!a || c <= 0) return EFWI_NN_ERR_ARG;
+  hipStream_t s = (hipStream_t)stream;
+  hipLaunchKernelGGL(k_add_masked, dim3(grid_for(nvox * c, 256)), dim3(256), 0, s, a, lda, b, ldb, c, nvox, mask, ldm);
+  return nn_check(s);
+}
+
+int efwi_nn_head(const uint16_t* act, int ld, int c, const float* w, const float* b, float* out, long nvox, void* stream) {
+  if (!act || !w || !b || !out || c <= 0 || c > 16) return EFWI_NN_ERR_ARG;
+  hipStream_t s = (hipStream_t)stream;
+  hipLaunchKernelGGL(k_head, dim3(grid_for(nvox, 256)), dim3(256), 0, s, act, ld, c, w, b, out, nvox);
+  return nn_check(s);
+}
+
+int efwi_nn_head_bwd(const uint16_t* act, int ld, int c, const float* w, const float* dout, uint16_t* dact, int ldd,
+                     float* dw, float* db, long nvox, void* stream) {
+  if (!act || !w || !dout || !dact || !dw || !db || c <= 0 || c > 16) return EFWI_NN_ERR_ARG;
+  hipStream_t s = (hipStream_t)stream;
+  hipLaunchKernelGGL(k_zero, dim3(1), dim3(64), 0, s, reinterpret_cast<unsigned*>(dw), c);
+  hipLaunchKernelGGL(k_zero, dim3(1), dim3(64), 0, s, reinterpret_cast<unsigned*>(db), 1);
+  hipLaunchKernelGGL(k_head_bwd, dim3(grid_for(nvox, 64, 2048)), dim3(64), 0, s, act, ld, c, w, dout, dact, ldd, dw, db, nvox);
+  return nn_check(s);
+}
+
+int efwi_nn_mse(const float* out, const float* target, const float* mask, long nvox, double count, float* dout,
+                      double* loss, void* stream) {
+  if (!out || !target || !mask || !dout || !loss || !(count > 0)) return EFWI_NN_ERR_ARG;
+  hipStream_t s = (hipStream_t)stream;
+  hipLaunchKernelGGL(k_zero, dim3(1), dim3(64), 0, s, reinterpret_cast<unsigned*>(loss), 2);
+  hipLaunchKernelGGL(k_mse, dim3(grid_for(nvox, 256, 1024)), dim3(256), 0, s, out, target, mask, nvox,
+                     (float)(1.0 / count), dout, loss);
+  return nn_check(s);
+}
+
+int efwi_nn_adam(float* w, const float* g, float* m1, float* m2, long n, float lr, float b1, float b2, float eps, int step,
+                 uint16_t* w16, void* stream) {
+  if (!w || !g || !m1 || !m2 || n <= 0 || step < 1) return EFWI_NN_ERR_ARG;
+  hipStream_t s = (hipStream_t)stream;
+  const float c1 = 1.f / (1.f - powf(b1, (float)step)), c2 = 1.f / (1.f - powf(b2, (float)step));
+  hipLaunchKernelGGL(k_adam, dim3(grid_for(n, 256)), dim3(256), 0, s, w, g, m1, m2, n, lr, b1, b2, eps, c1, c2, w16);
+  return nn_check(s);
+}
+
+int efwi_nn_to_bf16(const float* x, uint16_t* y, long n, void* stream) {
+  if (!x || !y || n < 0) return EFWI_NN_ERR_ARG;
+  hipStream_t s = (hipStream_t)stream;
+  hipLaunchKernelGGL(k_to_bf16, dim3(grid_for(n, 256)), dim3(256), 0, s, x, y, n);
+  return nn_check(s);
+}
+
+int efwi_nn_absmax(const float* x, long n, float* out, void* stream) {
+  if (!x || !out || n <= 0) return EFWI_NN_ERR_ARG;
+  hipStream_t s = (hipStream_t)stream;
+  hipLaunchKernelGGL(k_zero, dim3(1), dim3(64), 0, s, reinterpret_cast<unsigned*>(out), 1);
+  hipLaunchKernelGGL(k_absmax, dim3(grid_for(n, 256, 1024)), dim3(256), 0, s, x, n, reinterpret_cast<unsigned*>(out));
+  return nn_check(s);
+}
+
+int efwi_nn_pack(const float* g, const float* illum, int nz, int nx, int ny, int npml, float inv_g, float inv_i,
+                 uint16_t* feat, void* stream) {
+  if (!g || !illum || !feat || nz <= 2 * npml || nx <= 2 * npml || ny <= 2 * npml) return EFWI_NN_ERR_ARG;
+  hipStream_t s = (hipStream_t)stream;
+  const long n = (long)(nz - 2 * npml) * (nx - 2 * npml) * (ny - 2 * npml);
+  hipLaunchKernelGGL(k_pack, dim3(grid_for(n, 256)), dim3(256), 0, s, g, illum, Vol{nz, nx, ny}, npml, inv_g, inv_i, feat);
+  return nn_check(s);
+}
+
+}  // extern "C"

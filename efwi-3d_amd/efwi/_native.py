@@ -45,7 +45,7 @@ class Shot(C.Structure):
                 ("weights", C.POINTER(C.c_float)), ("src_weight", C.c_float)]
 
 
-# Every symbol include/efwi.h declares, with its ctypes signature.
+# Every symbol include/efwi.h and include/efwi_nn.h declare, with its ctypes signature.
 _vp, _fp, _dp, _ip = C.c_void_p, C.POINTER(C.c_float), C.POINTER(C.c_double), C.POINTER(C.c_int)
 SIGNATURES = {
     "efwi_version": (C.c_int, []),
@@ -71,6 +71,30 @@ SIGNATURES = {
     "efwi_taper_w2": (C.c_int, [C.c_int, C.c_float, C.c_float, _fp]),
     "efwi_cufd": (C.c_int, [_fp, _fp, _fp, _fp, _fp, _fp, _fp, _fp, C.c_int, C.c_int, C.c_int, _ip,
                             C.c_char_p]),
+    "efwi_set_illumination": (C.c_int, [_vp, C.c_int, _vp]),
+    "efwi_get_illumination": (C.c_int, [_vp, _vp, C.c_int, _vp]),
+    # learned preconditioner (include/efwi_nn.h)
+    "efwi_nn_conv3": (C.c_int, [_vp, C.c_int, C.c_int, _vp, _vp, _vp, C.c_int, C.c_int, _vp, C.c_int,
+                                C.c_int, C.c_int, C.c_int, C.c_int, C.c_int, _vp]),
+    "efwi_nn_wgrad_scratch": (C.c_long, [C.c_int, C.c_int, C.c_int, C.c_int, C.c_int, C.c_int]),
+    "efwi_nn_wgrad": (C.c_int, [_vp, C.c_int, C.c_int, _vp, C.c_int, C.c_int, _vp, C.c_long, _vp,
+                                C.c_int, C.c_int, C.c_int, C.c_int, _vp]),
+    "efwi_nn_chan_sum": (C.c_int, [_vp, C.c_int, C.c_int, C.c_long, _vp, _vp]),
+    "efwi_nn_pool2": (C.c_int, [_vp, C.c_int, _vp, C.c_int, C.c_int, C.c_int, C.c_int, C.c_int, _vp]),
+    "efwi_nn_pool2_bwd": (C.c_int, [_vp, C.c_int, _vp, C.c_int, C.c_int, C.c_int, C.c_int, C.c_int, C.c_int,
+                                    _vp]),
+    "efwi_nn_up2": (C.c_int, [_vp, C.c_int, _vp, C.c_int, C.c_int, C.c_int, C.c_int, C.c_int, _vp]),
+    "efwi_nn_up2_bwd": (C.c_int, [_vp, C.c_int, _vp, C.c_int, C.c_int, C.c_int, C.c_int, C.c_int, _vp,
+                                  C.c_int, _vp]),
+    "efwi_nn_add": (C.c_int, [_vp, C.c_int, _vp, C.c_int, C.c_int, C.c_long, _vp, C.c_int, _vp]),
+    "efwi_nn_head": (C.c_int, [_vp, C.c_int, C.c_int, _vp, _vp, _vp, C.c_long, _vp]),
+    "efwi_nn_head_bwd": (C.c_int, [_vp, C.c_int, C.c_int, _vp, _vp, _vp, C.c_int, _vp, _vp, C.c_long, _vp]),
+    "efwi_nn_mse": (C.c_int, [_vp, _vp, _vp, C.c_long, C.c_double, _vp, _vp, _vp]),
+    "efwi_nn_adam": (C.c_int, [_vp, _vp, _vp, _vp, C.c_long, C.c_float, C.c_float, C.c_float, C.c_float,
+                               C.c_int, _vp, _vp]),
+    "efwi_nn_to_bf16": (C.c_int, [_vp, _vp, C.c_long, _vp]),
+    "efwi_nn_absmax": (C.c_int, [_vp, C.c_long, _vp, _vp]),
+    "efwi_nn_pack": (C.c_int, [_vp, _vp, C.c_int, C.c_int, C.c_int, C.c_int, C.c_float, C.c_float, _vp, _vp]),
 }
 
 _lib = None

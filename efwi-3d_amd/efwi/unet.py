@@ -1,0 +1,258 @@
+"""Learned preconditioner: a 3-D U-Net on the library's BF16 matrix-core
+kernels (include/efwi_nn.h), trained and run on the GPU.
+
+It maps the raw FWI gradient plus the source illumination (2 input channels,
+padded to 16) to a model update (1 output channel). No reference
+counterpart: EFWI-3D has no learned part.
+
+Architecture (3 levels, 16 base channels, 3x3x3 convolutions with ReLU,
+2x2x2 average pooling, nearest upsampling, skip concatenation):
+
+    enc1a 16->16, enc1b 16->16 ------------------------------ cat1[32:48]
+      pool -> enc2a 16->32, enc2b 32->32 --------------- cat2[64:96]    |
+        pool -> bot_a 32->64, bot_b 64->64 -> up -> cat2[0:64]          |
+                  dec2a 96->32, dec2b 32->32 -> up -> cat1[0:32] -------+
+                                    dec1a 48->16, dec1b 16->16 -> head 16->1 (FP32)
+
+Weights and activations are BF16 with FP32 accumulation; FP32 master weights
+and Adam state; activation gradients BF16, weight gradients FP32. The
+skip concatenations are channel slices written in place; the ReLU derivative
+of each activation is applied as a mask inside the input-gradient kernels.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import math
+
+import numpy as np
+import torch
+
+from . import _native as N
+
+# (name, cin, cout, level)
+LAYERS = [
+    ("enc1a", 16, 16, 1), ("enc1b", 16, 16, 1),
+    ("enc2a", 16, 32, 2), ("enc2b", 32, 32, 2),
+    ("bot_a", 32, 64, 3), ("bot_b", 64, 64, 3),
+    ("dec2a", 96, 32, 2), ("dec2b", 32, 32, 2),
+    ("dec1a", 48, 16, 1), ("dec1b", 16, 16, 1),
+]
+IN_CHANNELS = 2       # raw gradient, log illumination
+WGRAD_PER = 64        # 16-voxel segments per weight-gradient partial
+
+
+def nn_check(rc: int, what: str):
+    if rc != 0:
+        raise N.EfwiError(f"{what} failed ({rc}: {'invalid argument' if rc == 1 else 'kernel launch'})")
+
+
+def _p(t, offset_elems=0):
+    return C.c_void_p(t.data_ptr() + offset_elems * t.element_size())
+
+
+class UNet3D:
+    def __init__(self, device: int = 0, seed: int = 0):
+        self.dev = torch.device(f"cuda:{device}")
+        self.lib = N.lib()
+        g = torch.Generator().manual_seed(seed)
+        self.w, self.b, self.w16 = {}, {}, {}
+        for name, cin, cout, _ in LAYERS:
+            fan_in = 27 * (IN_CHANNELS if name == "enc1a" else cin)
+            w = torch.randn(27, cout, cin, generator=g) * math.sqrt(2.0 / fan_in)
+            if name == "enc1a":
+                w[:, :, IN_CHANNELS:] = 0.0
+            self.w[name] = w.to(self.dev).contiguous()
+            self.b[name] = torch.zeros(cout, device=self.dev)
+        self.wh = (torch.randn(16, generator=g) * math.sqrt(1.0 / 16)).to(self.dev)
+        self.bh = torch.zeros(1, device=self.dev)
+        self._refresh_bf16()
+        self.m1 = {k: torch.zeros_like(v) for k, v in self._params().items()}
+        self.m2 = {k: torch.zeros_like(v) for k, v in self._params().items()}
+        self.steps = 0
+        self._bufs = {}
+        self._scratch = None
+
+    # ---- parameters
+    def _params(self):
+        out = {}
+        for name, *_ in LAYERS:
+            out[name + ".w"] = self.w[name]
+            out[name + ".b"] = self.b[name]
+        out["head.w"] = self.wh
+        out["head.b"] = self.bh
+        return out
+
+    def _refresh_bf16(self):
+        for name, *_ in LAYERS:
+            self.w16[name] = self.w[name].to(torch.bfloat16)
+
+    def state_dict(self):
+        return {k: v.detach().cpu().clone() for k, v in self._params().items()}
+
+    def load_state_dict(self, sd):
+        for k, v in self._params().items():
+            v.copy_(sd[k].to(self.dev))
+        self._refresh_bf16()
+
+    def nparams(self):
+        return sum(v.numel() for v in self._params().values())
+
+    # ---- buffers per volume shape
+    def _buffers(self, dims):
+        if dims in self._bufs:
+            return self._bufs[dims]
+        nz, nx, ny = dims
+        if nz % 4 or nx % 4 or ny % 4:
+            raise ValueError(f"U-Net volume {dims}: every size must be a multiple of 4")
+        v1 = nz * nx * ny
+        v2, v3 = v1 // 8, v1 // 64
+        bf = lambda n, c: torch.zeros(n, c, dtype=torch.bfloat16, device=self.dev)
+        B = {"a1": bf(v1, 16), "cat1": bf(v1, 48), "p1": bf(v2, 16), "a2": bf(v2, 32), "cat2": bf(v2, 96),
+             "p2": bf(v3, 32), "a3": bf(v3, 64), "b3": bf(v3, 64), "d2a": bf(v2, 32), "d2b": bf(v2, 32),
+             "d1a": bf(v1, 16), "d1b": bf(v1, 16), "out": torch.zeros(v1, device=self.dev),
+             # gradients
+             "g1": bf(v1, 16), "g1b": bf(v1, 16), "gcat1": bf(v1, 48), "gcat2": bf(v2, 96), "g2": bf(v2, 32),
+             "g2b": bf(v2, 32), "gp1": bf(v2, 16), "gp2": bf(v3, 32), "g3": bf(v3, 64), "g3b": bf(v3, 64),
+             "dout": torch.zeros(v1, device=self.dev), "loss": torch.zeros(1, dtype=torch.float64, device=self.dev)}
+        self._bufs[dims] = B
+        need = 0
+        for name, cin, cout, lev in LAYERS:
+            d = tuple(s >> (lev - 1) for s in dims)
+            need = max(need, int(self.lib.efwi_nn_wgrad_scratch(cout, cin, *d, WGRAD_PER)))
+        if self._scratch is None or self._scratch.numel() < need:
+            self._scratch = torch.empty(need, device=self.dev)
+        self.grads = {k: torch.zeros_like(v) for k, v in self._params().items()}
+        return B
+
+    def _stream(self):
+        return C.c_void_p(torch.cuda.current_stream(self.dev).cuda_stream)
+
+    def _conv(self, name, src, lds, dst, ldd, dims, relu=True, soff=0, doff=0):
+        _, cin, cout, _ = next(l for l in LAYERS if l[0] == name)
+        nn_check(self.lib.efwi_nn_conv3(_p(src, soff), lds, cin, _p(self.w16[name]), _p(self.b[name]),
+                                        _p(dst, doff), ldd, cout, None, 0, *dims, 0, int(relu), self._stream()),
+                 f"conv {name}")
+
+    def _dgrad(self, name, gout, ldg, gin, ldi, mask, ldm, dims, goff=0, ioff=0, moff=0):
+        _, cin, cout, _ = next(l for l in LAYERS if l[0] == name)
+        nn_check(self.lib.efwi_nn_conv3(_p(gout, goff), ldg, cout, _p(self.w16[name]), None, _p(gin, ioff), ldi,
+                                        cin, None if mask is None else _p(mask, moff), ldm, *dims, 1, 0,
+                                        self._stream()), f"dgrad {name}")
+
+    def _wgrad(self, name, gpre, ldg, src, lds, dims, goff=0, soff=0):
+        _, cin, cout, _ = next(l for l in LAYERS if l[0] == name)
+        s = self._stream()
+        nn_check(self.lib.efwi_nn_wgrad(_p(gpre, goff), ldg, cout, _p(src, soff), lds, cin, _p(self._scratch),
+                                        self._scratch.numel(), _p(self.grads[name + ".w"]), *dims, WGRAD_PER, s),
+                 f"wgrad {name}")
+        nn_check(self.lib.efwi_nn_chan_sum(_p(gpre, goff), ldg, cout, dims[0] * dims[1] * dims[2],
+                                           _p(self.grads[name + ".b"]), s), f"bias grad {name}")
+
+    # ---- forward / backward
+    def forward(self, feat, dims):
+        """feat: [V, 16] bfloat16 (channels-last, V = nz*nx*ny); returns the FP32 output [V]."""
+        B = self._buffers(dims)
+        s = self._stream()
+        d1, d2 = dims, tuple(x // 2 for x in dims)
+        d3 = tuple(x // 4 for x in dims)
+        v1 = d1[0] * d1[1] * d1[2]
+        self._conv("enc1a", feat, 16, B["a1"], 16, d1)
+        self._conv("enc1b", B["a1"], 16, B["cat1"], 48, d1, doff=32)
+        nn_check(self.lib.efwi_nn_pool2(_p(B["cat1"], 32), 48, _p(B["p1"]), 16, 16, *d1, s), "pool1")
+        self._conv("enc2a", B["p1"], 16, B["a2"], 32, d2)
+        self._conv("enc2b", B["a2"], 32, B["cat2"], 96, d2, doff=64)
+        nn_check(self.lib.efwi_nn_pool2(_p(B["cat2"], 64), 96, _p(B["p2"]), 32, 32, *d2, s), "pool2")
+        self._conv("bot_a", B["p2"], 32, B["a3"], 64, d3)
+        self._conv("bot_b", B["a3"], 64, B["b3"], 64, d3)
+        nn_check(self.lib.efwi_nn_up2(_p(B["b3"]), 64, _p(B["cat2"]), 96, 64, *d2, s), "up2")
+        self._conv("dec2a", B["cat2"], 96, B["d2a"], 32, d2)
+        self._conv("dec2b", B["d2a"], 32, B["d2b"], 32, d2)
+        nn_check(self.lib.efwi_nn_up2(_p(B["d2b"]), 32, _p(B["cat1"]), 48, 32, *d1, s), "up1")
+        self._conv("dec1a", B["cat1"], 48, B["d1a"], 16, d1)
+        self._conv("dec1b", B["d1a"], 16, B["d1b"], 16, d1)
+        nn_check(self.lib.efwi_nn_head(_p(B["d1b"]), 16, 16, _p(self.wh), _p(self.bh), _p(B["out"]), v1, s), "head")
+        self._feat, self._dims = feat, dims
+        return B["out"]
+
+    def backward(self, dout):
+        """Gradients of every parameter (self.grads) from dL/dout [V] (FP32)."""
+        B = self._buffers(self._dims)
+        s = self._stream()
+        d1 = self._dims
+        d2, d3 = tuple(x // 2 for x in d1), tuple(x // 4 for x in d1)
+        v1 = d1[0] * d1[1] * d1[2]
+        v2 = v1 // 8
+        nn_check(self.lib.efwi_nn_head_bwd(_p(B["d1b"]), 16, 16, _p(self.wh), _p(dout), _p(B["g1b"]), 16,
+                                           _p(self.grads["head.w"]), _p(self.grads["head.b"]), v1, s), "head bwd")
+        # level 1 decoder
+        self._wgrad("dec1b", B["g1b"], 16, B["d1a"], 16, d1)
+        self._dgrad("dec1b", B["g1b"], 16, B["g1"], 16, B["d1a"], 16, d1)
+        self._wgrad("dec1a", B["g1"], 16, B["cat1"], 48, d1)
+        self._dgrad("dec1a", B["g1"], 16, B["gcat1"], 48, B["cat1"], 48, d1)
+        nn_check(self.lib.efwi_nn_up2_bwd(_p(B["gcat1"]), 48, _p(B["g2b"]), 32, 32, *d1, None, 0, s), "up1 bwd")
+        # level 2 decoder
+        self._wgrad("dec2b", B["g2b"], 32, B["d2a"], 32, d2)
+        self._dgrad("dec2b", B["g2b"], 32, B["g2"], 32, B["d2a"], 32, d2)
+        self._wgrad("dec2a", B["g2"], 32, B["cat2"], 96, d2)
+        self._dgrad("dec2a", B["g2"], 32, B["gcat2"], 96, B["cat2"], 96, d2)
+        nn_check(self.lib.efwi_nn_up2_bwd(_p(B["gcat2"]), 96, _p(B["g3b"]), 64, 64, *d2, None, 0, s), "up2 bwd")
+        # bottleneck
+        self._wgrad("bot_b", B["g3b"], 64, B["a3"], 64, d3)
+        self._dgrad("bot_b", B["g3b"], 64, B["g3"], 64, B["a3"], 64, d3)
+        self._wgrad("bot_a", B["g3"], 64, B["p2"], 32, d3)
+        self._dgrad("bot_a", B["g3"], 64, B["gp2"], 32, None, 0, d3)
+        # level 2 encoder: skip gradient (masked) + pooled gradient, then the ReLU mask
+        nn_check(self.lib.efwi_nn_pool2_bwd(_p(B["gp2"]), 32, _p(B["gcat2"], 64), 96, 32, *d2, 1, s), "pool2 bwd")
+        nn_check(self.lib.efwi_nn_add(_p(B["gcat2"], 64), 96, None, 0, 32, v2, _p(B["cat2"], 64), 96, s), "mask2")
+        self._wgrad("enc2b", B["gcat2"], 96, B["a2"], 32, d2, goff=64)
+        self._dgrad("enc2b", B["gcat2"], 96, B["g2"], 32, B["a2"], 32, d2, goff=64)
+        self._wgrad("enc2a", B["g2"], 32, B["p1"], 16, d2)
+        self._dgrad("enc2a", B["g2"], 32, B["gp1"], 16, None, 0, d2)
+        # level 1 encoder
+        nn_check(self.lib.efwi_nn_pool2_bwd(_p(B["gp1"]), 16, _p(B["gcat1"], 32), 48, 16, *d1, 1, s), "pool1 bwd")
+        nn_check(self.lib.efwi_nn_add(_p(B["gcat1"], 32), 48, None, 0, 16, v1, _p(B["cat1"], 32), 48, s), "mask1")
+        self._wgrad("enc1b", B["gcat1"], 48, B["a1"], 16, d1, goff=32)
+        self._dgrad("enc1b", B["gcat1"], 48, B["g1"], 16, B["a1"], 16, d1, goff=32)
+        self._wgrad("enc1a", B["g1"], 16, self._feat, 16, d1)
+        return self.grads
+
+    def loss(self, out, target, mask, count):
+        B = self._buffers(self._dims)
+        nn_check(self.lib.efwi_nn_mse(_p(out), _p(target), _p(mask), out.numel(), float(count), _p(B["dout"]),
+                                      _p(B["loss"]), self._stream()), "mse")
+        return B["loss"], B["dout"]
+
+    def adam(self, lr=1e-3, b1=0.9, b2=0.999, eps=1e-8):
+        self.steps += 1
+        s = self._stream()
+        for k, w in self._params().items():
+            name = k.split(".")[0]
+            w16 = self.w16[name] if k.endswith(".w") and name in self.w16 else None
+            nn_check(self.lib.efwi_nn_adam(_p(w), _p(self.grads[k]), _p(self.m1[k]), _p(self.m2[k]), w.numel(),
+                                           lr, b1, b2, eps, self.steps, None if w16 is None else _p(w16), s),
+                     f"adam {k}")
+
+    def train_step(self, feat, target, mask, count, dims, lr=1e-3):
+        out = self.forward(feat, dims)
+        loss, dout = self.loss(out, target, mask, count)
+        self.backward(dout)
+        self.adam(lr)
+        return loss
+
+
+def pack_input(g, illum, npml, device=0):
+    """Network input [V, 16] bf16 from padded FWI-grid gradient and illumination
+    (CUDA float32 tensors), with the scales of channel 0 and 1 taken from the
+    volumes' absolute maxima (over the whole padded grid)."""
+    lib = N.lib()
+    s = C.c_void_p(torch.cuda.current_stream(g.device).cuda_stream)
+    mx = torch.zeros(2, device=g.device)
+    nn_check(lib.efwi_nn_absmax(_p(g), g.numel(), _p(mx, 0), s), "absmax g")
+    nn_check(lib.efwi_nn_absmax(_p(illum), illum.numel(), _p(mx, 1), s), "absmax illum")
+    gmax, imax = (float(v) for v in mx.cpu())
+    nz, nx, ny = g.shape
+    dims = (nz - 2 * npml, nx - 2 * npml, ny - 2 * npml)
+    feat = torch.empty(dims[0] * dims[1] * dims[2], 16, dtype=torch.bfloat16, device=g.device)
+    nn_check(lib.efwi_nn_pack(_p(g), _p(illum), nz, nx, ny, npml, 1.0 / max(gmax, 1e-30), 1.0 / max(imax, 1e-30),
+                              _p(feat), s), "pack")
+    return feat, dims

@@ -103,6 +103,15 @@ int efwi_run_shot(efwi_ctx* ctx, const efwi_shot* shot, int calc_id, const float
 int efwi_get_gradient(efwi_ctx* ctx, float* g_lambda, float* g_mu, float* g_rho, int on_host,
                       void* stream);
 
+/* Source illumination (pseudo-Hessian diagonal, Shin et al. 2001): with
+ * enable != 0, every efwi_run_shot adds sum over its forward steps of
+ * vz^2 + vx^2 + vy^2 per cell to a device array (padded grid, float) that
+ * efwi_zero_gradient clears; the array is zeroed when enabled. Read it with
+ * efwi_get_illumination (on_host: out is host memory). Inputs of the learned
+ * preconditioner; no reference counterpart. */
+int efwi_set_illumination(efwi_ctx* ctx, int enable, void* stream);
+int efwi_get_illumination(efwi_ctx* ctx, float* out, int on_host, void* stream);
+
 /* Per-kernel timing with HIP events recorded on the launch stream around
  * every stencil and ring launch of efwi_run_shot (no reference counterpart;
  * the reference only prints wall-clock totals, Src/libCUFD.cu:210-216).

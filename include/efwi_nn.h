@@ -1,0 +1,104 @@
+/* C ABI of the learned preconditioner's kernels (libefwi.so, gfx950).
+ *
+ * None of these has a counterpart in the reference (EFWI-3D has no learned
+ * part); they serve the 3-D U-Net that maps the raw FWI gradient plus the
+ * source illumination to a model update (efwi/unet.py).
+ *
+ * Tensors are device pointers to channels-last volumes of nz x nx x ny voxels
+ * (voxel v = (z * nx + x) * ny + y): BF16 values (uint16_t bit patterns) at
+ * p[v * ld + c], with the channel stride ld and the channel counts multiples
+ * of 16 where stated, so a tensor may be a channel slice of a wider one.
+ * Weights of a 3x3x3 convolution: w[t][cout][cin], tap t = 9 (dz+1) + 3 (dx+1)
+ * + (dy+1). Every call is asynchronous on `stream` (a hipStream_t, NULL for
+ * the null stream) and returns 0 or an error code below.
+ */
+#ifndef EFWI_NN_H
+#define EFWI_NN_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define EFWI_NN_OK 0
+#define EFWI_NN_ERR_ARG 1 /* invalid sizes, strides or pointers */
+#define EFWI_NN_ERR_HIP 2 /* a kernel launch failed */
+
+/* 3x3x3 convolution with zero padding, BF16 in and out, FP32 accumulation on
+ * the matrix cores (v_mfma_f32_16x16x16_bf16).
+ *   mode 0 (forward): dst[v][n] = relu?(sum_t,k src[v + d_t][k] w[t][n][k] + bias[n]),
+ *                     kc = cin, nc = cout, bias may be NULL;
+ *   mode 1 (input gradient): dst[v][n] = sum_t,k src[v - d_t][k] w[t][k][n],
+ *                     src = output gradient (kc = cout), nc = cin.
+ * When mask is not NULL, dst[v][n] is zeroed where mask[v * ldm + n] <= 0 (the
+ * ReLU derivative of a layer's activation). kc, nc, lds, ldd, ldm: multiples
+ * of 16, lds >= kc, ldd >= nc. */
+int efwi_nn_conv3(const uint16_t* src, int lds, int kc, const uint16_t* w, const float* bias, uint16_t* dst,
+                  int ldd, int nc, const uint16_t* mask, int ldm, int nz, int nx, int ny, int mode, int relu,
+                  void* stream);
+
+/* Weight gradient of a 3x3x3 convolution, dw[t][m][n] = sum_v dpre[v][m]
+ * src[v + d_t][n] (FP32 result), dpre = gradient of the layer's
+ * pre-activation (mc channels), src = its input (kc <= 96 channels). Each
+ * wavefront reduces `per` segments of 16 voxels into FP32 partials in
+ * `scratch` (efwi_nn_wgrad_scratch floats), which are then summed. */
+long efwi_nn_wgrad_scratch(int mc, int kc, int nz, int nx, int ny, int per);
+int efwi_nn_wgrad(const uint16_t* dpre, int ldp, int mc, const uint16_t* src, int lds, int kc, float* scratch,
+                  long scratch_floats, float* dw, int nz, int nx, int ny, int per, void* stream);
+
+/* out[c] = sum_v x[v][c] (bias gradients), c divides 256. */
+int efwi_nn_chan_sum(const uint16_t* x, int ld, int c, long nvox, float* out, void* stream);
+
+/* 2x2x2 average pooling of a (nz, nx, ny) volume (even sizes) and its
+ * gradient (accumulate: add into df instead of overwriting). */
+int efwi_nn_pool2(const uint16_t* src, int lds, uint16_t* dst, int ldd, int c, int nz, int nx, int ny, void* stream);
+int efwi_nn_pool2_bwd(const uint16_t* dc, int ldc, uint16_t* df, int ldf, int c, int nz, int nx, int ny, int accumulate,
+                      void* stream);
+
+/* Nearest-neighbour upsampling x2 to a (nz, nx, ny) volume and its gradient
+ * (sum over the 8 children, zeroed where mask <= 0 when mask is given). */
+int efwi_nn_up2(const uint16_t* cs, int ldc, uint16_t* fd, int ldf, int c, int nz, int nx, int ny, void* stream);
+int efwi_nn_up2_bwd(const uint16_t* df, int ldf, uint16_t* dc, int ldc, int c, int nz, int nx, int ny,
+                    const uint16_t* mask, int ldm, void* stream);
+
+/* a[v][c] = (a + b)[v][c] (b may be NULL: a alone), zeroed where mask <= 0
+ * when mask is given. */
+int efwi_nn_add(uint16_t* a, int lda, const uint16_t* b, int ldb, int c, long nvox, const uint16_t* mask, int ldm,
+                void* stream);
+
+/* 1x1x1 output head in FP32: out[v] = sum_c w[c] act[v][c] + b[0] (c <= 16),
+ * and its gradient: dact = w dout where act > 0, dw[c] = sum_v act dout,
+ * db = sum_v dout. */
+int efwi_nn_head(const uint16_t* act, int ld, int c, const float* w, const float* b, float* out, long nvox,
+                 void* stream);
+int efwi_nn_head_bwd(const uint16_t* act, int ld, int c, const float* w, const float* dout, uint16_t* dact, int ldd,
+                     float* dw, float* db, long nvox, void* stream);
+
+/* Masked mean-square loss over `count` voxels: *loss = sum m (out - t)^2 /
+ * count (FP64, device), dout = 2 m (out - t) / count. */
+int efwi_nn_mse(const float* out, const float* target, const float* mask, long nvox, double count, float* dout,
+                double* loss, void* stream);
+
+/* Adam step on FP32 master weights (bias-corrected, step >= 1); w16, when
+ * not NULL, receives the BF16 copy. */
+int efwi_nn_adam(float* w, const float* g, float* m1, float* m2, long n, float lr, float b1, float b2, float eps,
+                 int step, uint16_t* w16, void* stream);
+
+/* y = bf16(x), round to nearest even. */
+int efwi_nn_to_bf16(const float* x, uint16_t* y, long n, void* stream);
+
+/* *out = max |x| (FP32, device). */
+int efwi_nn_absmax(const float* x, long n, float* out, void* stream);
+
+/* Network input from padded FWI-grid volumes (npml cells per face), written
+ * for the interior only (16 channels): channel 0 = g * inv_g, channel 1 =
+ * log10(illum * inv_i + 1e-6) / 6 + 1, the other channels 0. */
+int efwi_nn_pack(const float* g, const float* illum, int nz, int nx, int ny, int npml, float inv_g, float inv_i,
+                 uint16_t* feat, void* stream);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* EFWI_NN_H */

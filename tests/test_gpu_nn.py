@@ -1,0 +1,315 @@
+"""The learned preconditioner's BF16 matrix-core kernels against the CPU
+oracle (oracle/nn_oracle.py), and the whole U-Net's forward and weight
+gradients against the oracle's network at the same rounding points.
+
+Inputs are BF16 values; the oracle accumulates them in float64. Bars:
+convolution outputs (rounded to BF16 on the GPU) within 1 BF16 ulp of the
+rounded oracle value, plus a relative 1e-6 of the row's magnitude for the
+FP32 accumulation order; weight gradients (FP32) within 1e-5 relative L2;
+the network's output and per-layer weight gradients within 5e-3 relative L2
+(activation gradients are BF16 between layers, so rounding differences
+propagate through 10 layers; measured on MI355X: at most 1.7e-3)."""
+import numpy as np
+import pytest
+import torch
+
+import nn_oracle as O
+
+pytestmark = pytest.mark.gpu
+DEV = "cuda:0"
+
+
+def _bf(rng, *shape, scale=1.0):
+    return O.bf16_round((rng.standard_normal(shape) * scale).astype(np.float32))
+
+
+def _dev(a):
+    return torch.from_numpy(np.ascontiguousarray(a, np.float32)).to(DEV).to(torch.bfloat16)
+
+
+def _host(t):
+    return t.float().cpu().numpy()
+
+
+def _ulps_ok(got, ref, rel=1e-6):
+    ref_b = O.bf16_round(ref.astype(np.float32))
+    ulp = np.abs(ref_b).astype(np.float64) * 2.0 ** -7 + 1e-30
+    row = np.abs(ref).max() if ref.size else 1.0
+    return np.abs(got - ref_b) <= ulp + rel * row + 1e-12
+
+
+def _p(t, off=0):
+    import ctypes as C
+    return C.c_void_p(t.data_ptr() + off * t.element_size())
+
+
+def _s():
+    import ctypes as C
+    return C.c_void_p(torch.cuda.current_stream().cuda_stream)
+
+
+@pytest.mark.parametrize("dims,kc,nc,relu", [((6, 10, 37), 16, 16, 1), ((4, 8, 16), 32, 64, 0),
+                                             ((5, 6, 33), 48, 16, 1), ((4, 4, 20), 96, 32, 1)])
+def test_conv3_forward_and_input_gradient(dims, kc, nc, relu):
+    from efwi import _native as N
+    from efwi.unet import nn_check
+    rng = np.random.default_rng(kc + nc)
+    nz, nx, ny = dims
+    V = nz * nx * ny
+    x = _bf(rng, nz, nx, ny, kc)
+    w = _bf(rng, 27, nc, kc, scale=0.1)
+    b = rng.standard_normal(nc).astype(np.float32)
+    print(f"[gpu] conv3 {dims} {kc}->{nc}", flush=True)
+    xd, wd = _dev(x.reshape(V, kc)), _dev(w)
+    bd = torch.from_numpy(b).to(DEV)
+    out = torch.zeros(V, nc, dtype=torch.bfloat16, device=DEV)
+    nn_check(N.lib().efwi_nn_conv3(_p(xd), kc, kc, _p(wd), _p(bd), _p(out), nc, nc, None, 0, nz, nx, ny, 0, relu, _s()),
+             "conv")
+    ref = O.conv3(x, w, bias=b, relu=bool(relu)).reshape(V, nc)
+    got = _host(out)
+    assert _ulps_ok(got, ref).all(), np.abs(got - ref).max()
+    # input gradient with a ReLU mask, written into a channel slice of a wider tensor
+    g = _bf(rng, nz, nx, ny, nc)
+    mask = _bf(rng, nz, nx, ny, kc)
+    wide = torch.zeros(V, kc + 16, dtype=torch.bfloat16, device=DEV)
+    md = _dev(mask.reshape(V, kc))
+    gd = _dev(g.reshape(V, nc))
+    nn_check(N.lib().efwi_nn_conv3(_p(gd), nc, nc, _p(wd), None, _p(wide, 16), kc + 16, kc,
+                                   _p(md), kc, nz, nx, ny, 1, 0, _s()), "dgrad")
+    ref = (O.conv3_dgrad(g, w) * (mask > 0)).reshape(V, kc)
+    got = _host(wide)
+    assert not got[:, :16].any()
+    assert _ulps_ok(got[:, 16:], ref).all(), np.abs(got[:, 16:] - ref).max()
+
+
+@pytest.mark.parametrize("dims,mc,kc", [((6, 10, 37), 16, 16), ((4, 8, 16), 64, 32), ((4, 4, 20), 32, 96)])
+def test_conv3_weight_gradient(dims, mc, kc):
+    from efwi import _native as N
+    from efwi.unet import nn_check
+    rng = np.random.default_rng(mc * kc)
+    nz, nx, ny = dims
+    V = nz * nx * ny
+    dpre = _bf(rng, nz, nx, ny, mc)
+    x = _bf(rng, nz, nx, ny, kc)
+    per = 3
+    need = int(N.lib().efwi_nn_wgrad_scratch(mc, kc, nz, nx, ny, per))
+    scratch = torch.empty(need, device=DEV)
+    dw = torch.zeros(27, mc, kc, device=DEV)
+    print(f"[gpu] wgrad {dims} {mc}x{kc}", flush=True)
+    # (every device operand is held in a variable: a temporary would be freed
+    # before the asynchronous kernel reads it)
+    dd, xd = _dev(dpre.reshape(V, mc)), _dev(x.reshape(V, kc))
+    nn_check(N.lib().efwi_nn_wgrad(_p(dd), mc, mc, _p(xd), kc, kc, _p(scratch), need, _p(dw), nz, nx, ny, per, _s()),
+             "wgrad")
+    ref = O.conv3_wgrad(dpre, x)
+    got = dw.cpu().numpy()
+    assert np.linalg.norm(got - ref) <= 1e-5 * np.linalg.norm(ref)
+    # too small a scratch is refused
+    assert N.lib().efwi_nn_wgrad(_p(dd), mc, mc, _p(xd), kc, kc, _p(scratch), need - 1, _p(dw), nz, nx, ny, per,
+                                 _s()) == 1
+
+
+def test_pool_up_add_head_loss_adam_pack():
+    from efwi import _native as N
+    from efwi.unet import nn_check
+    lib = N.lib()
+    rng = np.random.default_rng(5)
+    nz, nx, ny, c = 4, 6, 10, 16
+    V = nz * nx * ny
+    x = _bf(rng, nz, nx, ny, c)
+    xd = _dev(x.reshape(V, c))
+    # pooling (into a channel slice) and its gradient (accumulating)
+    p = torch.zeros(V // 8, 32, dtype=torch.bfloat16, device=DEV)
+    nn_check(lib.efwi_nn_pool2(_p(xd), c, _p(p, 16), 32, c, nz, nx, ny, _s()), "pool")
+    assert _ulps_ok(_host(p)[:, 16:], O.pool2(x).reshape(-1, c)).all()
+    gc = _bf(rng, nz // 2, nx // 2, ny // 2, c)
+    gcd = _dev(gc.reshape(-1, c))
+    acc = _bf(rng, nz, nx, ny, c)
+    accd = _dev(acc.reshape(V, c))
+    acc0 = _dev(acc.reshape(V, c))
+    nn_check(lib.efwi_nn_pool2_bwd(_p(gcd), c, _p(accd), c, c, nz, nx, ny, 1, _s()), "pool bwd")
+    assert _ulps_ok(_host(accd), (acc + O.pool2_bwd(gc)).reshape(V, c)).all()
+    # upsampling and its gradient with a mask
+    up = torch.zeros(V, c, dtype=torch.bfloat16, device=DEV)
+    nn_check(lib.efwi_nn_up2(_p(gcd), c, _p(up), c, c, nz, nx, ny, _s()), "up")
+    assert np.array_equal(_host(up), O.up2(gc).reshape(V, c))
+    m = _bf(rng, nz // 2, nx // 2, ny // 2, c)
+    mdv = _dev(m.reshape(-1, c))
+    dc = torch.zeros(V // 8, c, dtype=torch.bfloat16, device=DEV)
+    nn_check(lib.efwi_nn_up2_bwd(_p(xd), c, _p(dc), c, c, nz, nx, ny, _p(mdv), c, _s()), "up bwd")
+    assert _ulps_ok(_host(dc), (O.up2_bwd(x) * (m > 0)).reshape(-1, c)).all()
+    # add with mask, and mask alone
+    a = _bf(rng, V, c)
+    ad = _dev(a)
+    nn_check(lib.efwi_nn_add(_p(ad), c, _p(xd), c, c, V, _p(acc0), c, _s()), "add")
+    assert _ulps_ok(_host(ad), (a + x.reshape(V, c)) * (acc.reshape(V, c) > 0)).all()
+    # mask alone, over the first V/8 voxels (the mask tensor's size)
+    nn_check(lib.efwi_nn_add(_p(ad), c, None, 0, c, V // 8, _p(mdv), c, _s()), "mask")
+    assert _ulps_ok(_host(ad)[:V // 8], (O.bf16_round(((a + x.reshape(V, c)) * (acc.reshape(V, c) > 0))
+                                                      .astype(np.float32))[:V // 8] * (m.reshape(-1, c) > 0))).all()
+    # head and its gradient
+    act = np.maximum(x, 0).reshape(V, c)
+    w = rng.standard_normal(16).astype(np.float32)
+    wd, bd = torch.from_numpy(w).to(DEV), torch.tensor([0.25], device=DEV)
+    out = torch.zeros(V, device=DEV)
+    actd = _dev(act)
+    nn_check(lib.efwi_nn_head(_p(actd), c, c, _p(wd), _p(bd), _p(out), V, _s()), "head")
+    assert np.allclose(out.cpu().numpy(), O.head(act, w, 0.25), rtol=1e-5, atol=1e-5)
+    t = rng.standard_normal(V).astype(np.float32)
+    mk = (rng.uniform(size=V) > 0.4).astype(np.float32)
+    td, mkd = torch.from_numpy(t).to(DEV), torch.from_numpy(mk).to(DEV)
+    dout = torch.zeros(V, device=DEV)
+    loss = torch.zeros(1, dtype=torch.float64, device=DEV)
+    nn_check(lib.efwi_nn_mse(_p(out), _p(td), _p(mkd), V, float(mk.sum()), _p(dout), _p(loss), _s()), "mse")
+    lref, dref = O.mse(out.cpu().numpy(), t, mk, mk.sum())
+    assert np.isclose(float(loss), lref, rtol=1e-6)
+    assert np.allclose(dout.cpu().numpy(), dref, rtol=1e-5, atol=1e-9)
+    dact = torch.zeros(V, c, dtype=torch.bfloat16, device=DEV)
+    dw, db = torch.zeros(16, device=DEV), torch.zeros(1, device=DEV)
+    nn_check(lib.efwi_nn_head_bwd(_p(actd), c, c, _p(wd), _p(dout), _p(dact), c, _p(dw), _p(db), V, _s()), "head bwd")
+    rda, rdw, rdb = O.head_bwd(act, w, dout.cpu().numpy())
+    assert _ulps_ok(_host(dact), rda).all()
+    assert np.allclose(dw.cpu().numpy(), rdw, rtol=1e-4, atol=1e-7)
+    assert np.isclose(float(db), rdb, rtol=1e-4, atol=1e-7)
+    # Adam, two steps, and the bf16 copy
+    n = 1000
+    w0 = rng.standard_normal(n).astype(np.float32)
+    gs = [rng.standard_normal(n).astype(np.float32) for _ in range(2)]
+    wdv = torch.from_numpy(w0.copy()).to(DEV)
+    m1, m2 = torch.zeros(n, device=DEV), torch.zeros(n, device=DEV)
+    w16 = torch.zeros(n, dtype=torch.bfloat16, device=DEV)
+    rw, r1, r2 = w0.astype(np.float64), np.zeros(n), np.zeros(n)
+    for k, g in enumerate(gs, 1):
+        gdv = torch.from_numpy(g).to(DEV)
+        nn_check(lib.efwi_nn_adam(_p(wdv), _p(gdv), _p(m1), _p(m2), n, 1e-2, 0.9, 0.999, 1e-8, k, _p(w16), _s()),
+                 "adam")
+        rw, r1, r2 = O.adam(rw, g, r1, r2, k, 1e-2)
+    assert np.allclose(wdv.cpu().numpy(), rw, rtol=1e-5, atol=1e-6)
+    assert np.array_equal(_host(w16), O.bf16_round(wdv.cpu().numpy()))
+    # pack and absmax
+    g3 = rng.standard_normal((12, 13, 14)).astype(np.float32)
+    il = rng.uniform(0.01, 3.0, size=(12, 13, 14)).astype(np.float32)
+    g3d, ild = torch.from_numpy(g3).to(DEV), torch.from_numpy(il).to(DEV)
+    mx = torch.zeros(1, device=DEV)
+    nn_check(lib.efwi_nn_absmax(_p(g3d), g3.size, _p(mx), _s()), "absmax")
+    assert float(mx) == np.abs(g3).max()
+    feat = torch.zeros(8 * 9 * 10, 16, dtype=torch.bfloat16, device=DEV)
+    nn_check(lib.efwi_nn_pack(_p(g3d), _p(ild), 12, 13, 14, 2, 0.5, 1 / 3.0, _p(feat), _s()), "pack")
+    assert _ulps_ok(_host(feat), O.pack(g3, il, 2, 0.5, 1 / 3.0).reshape(-1, 16), rel=1e-5).all()
+
+
+def _oracle_network(net, feat, dims, target, mask, count):
+    """The U-Net's forward and weight gradients with the oracle's operations,
+    rounding to BF16 where the GPU stores BF16."""
+    r = O.bf16_round
+    W = {k: r(net.w[k].cpu().numpy()) for k in net.w}
+    Bs = {k: net.b[k].cpu().numpy() for k in net.b}
+    wh, bh = net.wh.cpu().numpy(), float(net.bh.cpu()[0])
+    f = lambda a: r(a.astype(np.float32))
+    x = _host(feat).reshape(dims + (16,))
+    conv = lambda n, a: f(O.conv3(a, W[n], Bs[n], relu=True))
+    a1 = conv("enc1a", x)
+    e1 = conv("enc1b", a1)
+    p1 = f(O.pool2(e1))
+    a2 = conv("enc2a", p1)
+    e2 = conv("enc2b", a2)
+    p2 = f(O.pool2(e2))
+    a3 = conv("bot_a", p2)
+    b3 = conv("bot_b", a3)
+    cat2 = np.concatenate([O.up2(b3), e2], axis=3)
+    d2a = conv("dec2a", cat2)
+    d2b = conv("dec2b", d2a)
+    cat1 = np.concatenate([O.up2(d2b), e1], axis=3)
+    d1a = conv("dec1a", cat1)
+    d1b = conv("dec1b", d1a)
+    out = O.head(d1b, wh, bh).astype(np.float32)
+    _, dout = O.mse(out, target.reshape(dims), mask.reshape(dims), count)
+    dout = dout.astype(np.float32)
+    G = {}
+    g1b, G["head.w"], _ = O.head_bwd(d1b, wh, dout)
+    g1b = f(g1b)
+
+    def layer(n, gpre, src, mask_src):
+        G[n + ".w"] = O.conv3_wgrad(gpre, src)
+        G[n + ".b"] = gpre.reshape(-1, gpre.shape[3]).sum(0)
+        if mask_src is None:
+            return f(O.conv3_dgrad(gpre, W[n]))
+        return f(O.conv3_dgrad(gpre, W[n]) * (mask_src > 0))
+    g = layer("dec1b", g1b, d1a, d1a)
+    gcat1 = layer("dec1a", g, cat1, cat1)
+    g = f(O.up2_bwd(gcat1[..., :32]))
+    g = layer("dec2b", g, d2a, d2a)
+    gcat2 = layer("dec2a", g, cat2, cat2)
+    g = f(O.up2_bwd(gcat2[..., :64]))
+    g = layer("bot_b", g, a3, a3)
+    gp2 = layer("bot_a", g, p2, None)
+    ge2 = f(f(gcat2[..., 64:] + O.pool2_bwd(gp2)) * (e2 > 0))
+    g = layer("enc2b", ge2, a2, a2)
+    gp1 = layer("enc2a", g, p1, None)
+    ge1 = f(f(gcat1[..., 32:] + O.pool2_bwd(gp1)) * (e1 > 0))
+    g = layer("enc1b", ge1, a1, a1)
+    layer("enc1a", g, x, None)
+    return out, G
+
+
+def test_unet_forward_and_weight_gradients_match_oracle():
+    from efwi.unet import UNet3D
+    rng = np.random.default_rng(9)
+    dims = (8, 12, 16)
+    V = dims[0] * dims[1] * dims[2]
+    net = UNet3D(0, seed=1)
+    # nonzero biases so that every ReLU mask has both signs
+    for k in net.b:
+        net.b[k].copy_(torch.from_numpy(rng.uniform(-0.05, 0.1, net.b[k].numel()).astype(np.float32)))
+    f0 = np.zeros((V, 16), np.float32)
+    f0[:, :2] = rng.standard_normal((V, 2))
+    feat = _dev(f0)
+    target = rng.standard_normal(V).astype(np.float32)
+    mask = np.ones(V, np.float32)
+    print(f"[gpu] U-Net {dims}, {net.nparams()} parameters", flush=True)
+    out = net.forward(feat, dims)
+    loss, dout = net.loss(out, torch.from_numpy(target).to(DEV), torch.from_numpy(mask).to(DEV), V)
+    net.backward(dout)
+    torch.cuda.synchronize()
+    ref_out, G = _oracle_network(net, feat, dims, target, mask, V)
+    got_out = out.cpu().numpy()
+    assert np.linalg.norm(got_out - ref_out.ravel()) <= 5e-3 * np.linalg.norm(ref_out)
+    worst = {}
+    for k, ref in G.items():
+        got = net.grads[k].cpu().numpy().reshape(np.shape(ref))
+        worst[k] = float(np.linalg.norm(got - ref) / max(np.linalg.norm(ref), 1e-30))
+    print("[gpu] relative L2 of the weight gradients:", {k: round(v, 5) for k, v in worst.items()}, flush=True)
+    for k, e in worst.items():
+        assert e < 5e-3, (k, e)
+
+
+def test_unet_training_lowers_the_loss():
+    """A short training run on the GPU's own kernels: learn a pointwise map of
+    the two input channels (the pseudo-Hessian scaling g / illumination) on
+    random smooth volumes; the loss must fall well below its start."""
+    from scipy.ndimage import gaussian_filter
+    from efwi.unet import UNet3D
+    rng = np.random.default_rng(2)
+    dims = (16, 16, 16)
+    V = dims[0] * dims[1] * dims[2]
+    net = UNet3D(0, seed=3)
+    data = []
+    for _ in range(4):
+        g = gaussian_filter(rng.standard_normal(dims), 1.5)
+        il = np.exp(gaussian_filter(rng.standard_normal(dims), 3.0) * 2)
+        g /= np.abs(g).max()
+        il /= il.max()
+        u = -g / (il + 0.05)
+        u /= np.abs(u).max()
+        f = np.zeros((V, 16), np.float32)
+        f[:, 0] = g.ravel()
+        f[:, 1] = np.log10(il.ravel() + 1e-6) / 6 + 1
+        data.append((_dev(f), torch.from_numpy(u.ravel().astype(np.float32)).to(DEV)))
+    mask = torch.ones(V, device=DEV)
+    losses = []
+    for it in range(150):
+        feat, t = data[it % 4]
+        losses.append(float(net.train_step(feat, t, mask, V, dims, lr=3e-3)))
+    print(f"[gpu] U-Net training loss {losses[0]:.4f} -> {np.mean(losses[-8:]):.4f}", flush=True)
+    assert np.isfinite(losses).all()
+    assert np.mean(losses[-8:]) < 0.3 * np.mean(losses[:4])

@@ -14,13 +14,16 @@ import pytest
 import torch
 
 ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
-HEADER = os.path.join(ROOT, "include", "efwi.h")
+HEADERS = [os.path.join(ROOT, "include", f) for f in sorted(os.listdir(os.path.join(ROOT, "include")))
+           if f.endswith(".h")]
 
 
 def _header_functions():
-    text = open(HEADER).read()
-    text = re.sub(r"/\*.*?\*/", "", text, flags=re.S)
-    return sorted(set(re.findall(r"\b(efwi_[a-z0-9_]+)\s*\(", text)))
+    names = set()
+    for h in HEADERS:
+        text = re.sub(r"/\*.*?\*/", "", open(h).read(), flags=re.S)
+        names |= set(re.findall(r"\b(efwi_[a-z0-9_]+)\s*\(", text))
+    return sorted(names)
 
 
 def test_library_exports_every_header_symbol():
@@ -30,7 +33,7 @@ def test_library_exports_every_header_symbol():
     assert len(names) >= 12
     for n in names:
         assert hasattr(lib, n), n
-    assert set(names) == set(N.SIGNATURES), "ctypes signatures out of sync with include/efwi.h"
+    assert set(names) == set(N.SIGNATURES), "ctypes signatures out of sync with include/*.h"
     assert lib.efwi_version() >= 100
 
 
