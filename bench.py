@@ -177,6 +177,106 @@ def bench_batch(args, w):
     P.close()
 
 
+def unet_flops(dims):
+    """FLOPs of one training step (forward, input and weight gradients of every
+    3x3x3 convolution) of the U-Net on a volume of `dims` voxels."""
+    from efwi.unet import LAYERS
+    v = dims[0] * dims[1] * dims[2]
+    fl = 0
+    for name, cin, cout, lev in LAYERS:
+        n = v // 8 ** (lev - 1)
+        per = 2 * 27 * cin * cout * n
+        fl += per * (2 if name == "enc1a" else 3)  # no input gradient for the first layer
+    return fl
+
+
+def bench_learned(args):
+    """Learned preconditioner: training pairs from the plain method on random
+    models, training on the library's BF16 MFMA kernels, then inversions of
+    held-out models with plain L-BFGS and with the learned preconditioner."""
+    import numpy as np
+    import torch
+    from efwi import learned as L
+    from efwi.inversion import lbfgs_invert
+    from efwi.propagator import Propagator
+    log = lambda m: print(m, file=sys.stderr, flush=True)
+    dev = torch.device("cuda:0")
+    tdims = (96, 128, 128)
+    data, fdims, t_gen = L.generate(args.train_models, dims=tdims, npml=20, nt=600, seed=11, log=log)
+    log(f"[bench] {len(data)} training pairs at {tdims} in {t_gen:.1f} s")
+    net, losses, t_train = L.train(data, fdims, epochs=args.epochs, log=log)
+    # kernel throughput of a training step, timed with events on torch's stream
+    V = fdims[0] * fdims[1] * fdims[2]
+    mask = torch.ones(V, device=dev)
+    e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    e0.record()
+    nstep = 10
+    for k in range(nstep):
+        net.train_step(data[k][0], data[k][2], mask, V, fdims, lr=0.0, log_scaling=True)
+    e1.record()
+    torch.cuda.synchronize()
+    step_ms = e0.elapsed_time(e1) / nstep
+    tflops = unet_flops(fdims) / (step_ms * 1e-3) / 1e12
+    del data
+    torch.cuda.empty_cache()
+    # held-out inversions at the production size
+    edims, npml = (120, 200, 200), 20
+    rng = np.random.default_rng(2024)
+    results = []
+    for m in range(args.eval_models):
+        survey = L.surface_survey(*edims, npml, args.eval_nt, 12.5, 1e-3, 8.0, 8, rng)
+        vp, vs, rho, vp0 = L.random_model(rng, *edims, sigma=8.0)
+        row = {}
+        for method in ("plain", "learned"):
+            prop = Propagator(survey.grid, 0)
+            prob = L.make_problem(prop, survey, vp, vs, rho, dev)
+            v0 = L._padded(vp0, npml, dev)
+            if method == "plain":
+                _, hist = lbfgs_invert(prob, v0, args.iters, log=log)
+                counts = None
+            else:
+                _, hist, counts = L.learned_invert(prob, v0, args.iters, net, log=log)
+            prop.close()
+            row[method] = {"f": hist.f, "t": hist.t, "nfev": hist.nfev[-1], "counts": counts}
+            log(f"[bench] model {m} {method}: misfit {hist.f[0]:.4e} -> {hist.f[-1]:.4e} in {hist.t[-1]:.1f} s, "
+                f"{hist.nfev[-1]} evaluations {counts or ''}")
+        target = row["plain"]["f"][-1]
+        reach = next((t for f, t in zip(row["learned"]["f"], row["learned"]["t"]) if f <= target), None)
+        row["time_ratio"] = None if reach is None else reach / row["plain"]["t"][-1]
+        results.append(row)
+    ratios = [r["time_ratio"] for r in results]
+    reached = [x for x in ratios if x is not None]
+    out = {
+        "metric": "wall time of the learned-preconditioned inversion to the plain L-BFGS final misfit, "
+                  "relative to plain L-BFGS",
+        "value": round(float(np.mean(reached)), 4) if len(reached) == len(ratios) else None,
+        "unit": "ratio",
+        "n_gpus": 1, "steps": args.iters, "warmup": 0,
+        "ms_per_step": None,
+        "higher_is_better": False, "scaling": "weak", "vs_baseline": None, "dtype": "bf16 (network), fp32 (stencil)",
+        "data": "synthetic",
+        "config": {"workload": f"cfg5: U-Net (3 levels, 16 base channels, 3x3x3, BF16 MFMA, {net.nparams()} "
+                               f"parameters) trained on {args.train_models} random layered+anomaly models at "
+                               f"{tdims[0]}x{tdims[1]}x{tdims[2]} ({args.epochs} epochs); {args.eval_models} "
+                               f"held-out models at {edims[0]}x{edims[1]}x{edims[2]} + {npml} PML, 8 shots, "
+                               f"nt={args.eval_nt}, {args.iters} iterations each method",
+                   "time_ratios": ratios,
+                   "final_misfit_ratio_learned_over_plain": [r["learned"]["f"][-1] / r["plain"]["f"][-1] for r in results],
+                   "plain": [{"f_rel": [round(x / r["plain"]["f"][0], 5) for x in r["plain"]["f"]],
+                              "seconds": round(r["plain"]["t"][-1], 2), "nfev": r["plain"]["nfev"]} for r in results],
+                   "learned": [{"f_rel": [round(x / r["learned"]["f"][0], 5) for x in r["learned"]["f"]],
+                                "seconds": round(r["learned"]["t"][-1], 2), "nfev": r["learned"]["nfev"],
+                                "steps": r["learned"]["counts"]} for r in results],
+                   "training": {"pairs_s": round(t_gen, 1), "train_s": round(t_train, 1),
+                                "loss_first": losses[0], "loss_last_epoch": float(np.mean(losses[-args.train_models:]))}},
+        "roofline": {"bound": "mfma", "kernel": "U-Net training step (conv3 forward/input/weight gradients)",
+                     "achieved": round(tflops, 2), "peak": 2500.0, "unit": "TFLOP/s",
+                     "frac": round(tflops / 2500.0, 5), "traffic": None, "step_ms": round(step_ms, 2)},
+        "cpu_baseline": None,
+    }
+    print(json.dumps(out), flush=True)
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -185,6 +285,11 @@ def main():
     ap.add_argument("--workload", default="cfg2")
     ap.add_argument("--nt", type=int, default=None, help="override the number of time steps")
     ap.add_argument("--cpu-steps", type=int, default=100)
+    ap.add_argument("--train-models", type=int, default=256, help="cfg5: training models")
+    ap.add_argument("--epochs", type=int, default=6, help="cfg5: training epochs")
+    ap.add_argument("--eval-models", type=int, default=2, help="cfg5: held-out models")
+    ap.add_argument("--eval-nt", type=int, default=800, help="cfg5: time steps of the held-out inversions")
+    ap.add_argument("--iters", type=int, default=20, help="cfg5: inversion iterations")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-profile", action="store_true", help="skip the per-kernel event timing")
     ap.add_argument("--no-autotune", action="store_true", help="keep the default launch shapes")
@@ -215,6 +320,8 @@ def main():
     from efwi import synth
     from efwi.propagator import Propagator
 
+    if args.workload == "cfg5":
+        return bench_learned(args)
     w = synth.WORKLOADS[args.workload](**({} if args.nt is None else {"nt": args.nt}))
     g = w.grid
     if len(w.shots) > 1 and g.memory_mode != 1:

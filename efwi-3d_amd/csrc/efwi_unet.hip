@@ -293,6 +293,20 @@ __global__ void k_head_bwd(const uint16_t* __restrict__ act, int ld, int c, cons
   }
 }
 
+// multiplicative output: u[v] = g[v] exp(clamp(o[v], -4, 6)), g = channel 0
+// of the input; gradient do[v] = du[v] u[v] inside the clamp, 0 outside
+constexpr float kGateLo = -4.f, kGateHi = 6.f;
+__global__ void k_gate(const uint16_t* __restrict__ feat, int ld, const float* __restrict__ o, float* __restrict__ u,
+                       long nvox) {
+  for (long v = (long)blockIdx.x * blockDim.x + threadIdx.x; v < nvox; v += (long)gridDim.x * blockDim.x)
+    u[v] = bf2f(feat[v * ld]) * expf(fminf(fmaxf(o[v], kGateLo), kGateHi));
+}
+__global__ void k_gate_bwd(const float* __restrict__ o, const float* __restrict__ u, const float* __restrict__ du,
+                           float* __restrict__ dout, long nvox) {
+  for (long v = (long)blockIdx.x * blockDim.x + threadIdx.x; v < nvox; v += (long)gridDim.x * blockDim.x)
+    dout[v] = (o[v] > kGateLo && o[v] < kGateHi) ? du[v] * u[v] : 0.f;
+}
+
 // masked mean-square loss: dout = 2 (out - target) m / sum(m); loss += sum m (out - target)^2 / sum(m)
 __global__ void k_mse(const float* __restrict__ out, const float* __restrict__ target, const float* __restrict__ m,
                       long nvox, float inv_count, float* __restrict__ dout, double* __restrict__ loss) {
@@ -499,6 +513,20 @@ int efwi_nn_head_bwd(const uint16_t* act, int ld, int c, const float* w, const f
   hipLaunchKernelGGL(k_zero, dim3(1), dim3(64), 0, s, reinterpret_cast<unsigned*>(dw), c);
   hipLaunchKernelGGL(k_zero, dim3(1), dim3(64), 0, s, reinterpret_cast<unsigned*>(db), 1);
   hipLaunchKernelGGL(k_head_bwd, dim3(grid_for(nvox, 64, 2048)), dim3(64), 0, s, act, ld, c, w, dout, dact, ldd, dw, db, nvox);
+  return nn_check(s);
+}
+
+int efwi_nn_gate(const uint16_t* feat, int ld, const float* o, float* u, long nvox, void* stream) {
+  if (!feat || !o || !u || ld < 1) return EFWI_NN_ERR_ARG;
+  hipStream_t s = (hipStream_t)stream;
+  hipLaunchKernelGGL(k_gate, dim3(grid_for(nvox, 256)), dim3(256), 0, s, feat, ld, o, u, nvox);
+  return nn_check(s);
+}
+
+int efwi_nn_gate_bwd(const float* o, const float* u, const float* du, float* dout, long nvox, void* stream) {
+  if (!o || !u || !du || !dout) return EFWI_NN_ERR_ARG;
+  hipStream_t s = (hipStream_t)stream;
+  hipLaunchKernelGGL(k_gate_bwd, dim3(grid_for(nvox, 256)), dim3(256), 0, s, o, u, du, dout, nvox);
   return nn_check(s);
 }
 

@@ -89,6 +89,8 @@ SIGNATURES = {
     "efwi_nn_add": (C.c_int, [_vp, C.c_int, _vp, C.c_int, C.c_int, C.c_long, _vp, C.c_int, _vp]),
     "efwi_nn_head": (C.c_int, [_vp, C.c_int, C.c_int, _vp, _vp, _vp, C.c_long, _vp]),
     "efwi_nn_head_bwd": (C.c_int, [_vp, C.c_int, C.c_int, _vp, _vp, _vp, C.c_int, _vp, _vp, C.c_long, _vp]),
+    "efwi_nn_gate": (C.c_int, [_vp, C.c_int, _vp, _vp, C.c_long, _vp]),
+    "efwi_nn_gate_bwd": (C.c_int, [_vp, _vp, _vp, _vp, C.c_long, _vp]),
     "efwi_nn_mse": (C.c_int, [_vp, _vp, _vp, C.c_long, C.c_double, _vp, _vp, _vp]),
     "efwi_nn_adam": (C.c_int, [_vp, _vp, _vp, _vp, C.c_long, C.c_float, C.c_float, C.c_float, C.c_float,
                                C.c_int, _vp, _vp]),

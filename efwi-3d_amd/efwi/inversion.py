@@ -95,12 +95,13 @@ class History:
 
 
 def lbfgs_invert(problem: Problem, vp0: torch.Tensor, niter: int, first_step: float = 20.0,
-                 history: int = 30, dot=default_dot, step_hook=None):
+                 history: int = 30, dot=default_dot, step_hook=None, log=None):
     """Projected L-BFGS with Armijo backtracking. The first direction is the
     steepest descent scaled to a largest update of `first_step` m/s; later
     iterations try the unit step. `step_hook(it, vp, g, d)` may return an
     alternative trial model (a learned step, say); it is kept only when it
-    lowers the misfit, otherwise the L-BFGS step is taken.
+    lowers the misfit, otherwise the L-BFGS step is taken (a hook with a
+    `record(accepted)` method is told which).
 
     Returns (vp, History) with the misfit after each iteration and the
     wall time since the start (synchronised)."""
@@ -119,6 +120,8 @@ def lbfgs_invert(problem: Problem, vp0: torch.Tensor, niter: int, first_step: fl
             if alt is not None:
                 alt = problem.project(alt)
                 fa, ga = problem.evaluate(alt)
+                if hasattr(step_hook, "record"):
+                    step_hook.record(fa < f)
                 if fa < f:
                     H.update(alt - vp, ga - g)
                     vp, f, g = alt, fa, ga
@@ -150,4 +153,6 @@ def lbfgs_invert(problem: Problem, vp0: torch.Tensor, niter: int, first_step: fl
         sync()
         hist.f.append(f); hist.t.append(time.perf_counter() - t0)
         hist.nfev.append(problem.nfev); hist.kind.append("lbfgs")
+        if log:
+            log(f"[lbfgs] iteration {it + 1}: misfit {f:.6e}, {problem.nfev} evaluations, {hist.t[-1]:.1f} s")
     return vp, hist

@@ -48,15 +48,24 @@ class LBFGS:
             self.s.pop(0); self.y.pop(0); self.rho.pop(0)
         return True
 
-    def direction(self, g: torch.Tensor) -> torch.Tensor:
-        """-H g for the current inverse-Hessian approximation H."""
+    def direction(self, g: torch.Tensor, precond=None) -> torch.Tensor:
+        """-H g for the current inverse-Hessian approximation H. With
+        `precond` (a map q -> H0 q, positive definite in effect), the initial
+        approximation is gamma H0 with gamma = s.y / y.H0 y (Nocedal & Wright,
+        eq. 7.20 generalised); without it H0 = I."""
         q = g.clone()
         alpha = []
         for s, y, rho in zip(reversed(self.s), reversed(self.y), reversed(self.rho)):
             a = rho * self.dot(s, q)
             q.add_(y, alpha=-a)
             alpha.append(a)
-        if self.s:
+        if precond is not None:
+            q = precond(q)
+            if self.s:
+                yh = self.dot(self.y[-1], precond(self.y[-1]))
+                if yh > 0:
+                    q.mul_(self.dot(self.s[-1], self.y[-1]) / yh)
+        elif self.s:
             gamma = self.dot(self.s[-1], self.y[-1]) / self.dot(self.y[-1], self.y[-1])
             q.mul_(gamma)
         for (s, y, rho), a in zip(zip(self.s, self.y, self.rho), reversed(alpha)):

@@ -188,6 +188,20 @@ class Propagator:
                     "efwi_get_gradient")
         return tuple(out)
 
+    def set_illumination(self, on: bool = True):
+        """Accumulate the source illumination sum_t |v|^2 of every shot's forward
+        field from now on (cleared by zero_gradient)."""
+        with torch.cuda.device(self.device):
+            N.check(N.lib().efwi_set_illumination(self._h, int(bool(on)), self._stream()), "efwi_set_illumination")
+
+    def illumination(self):
+        """The accumulated source illumination (padded grid, float32 CUDA tensor)."""
+        g = self.grid
+        out = torch.empty(g.shape, dtype=torch.float32, device=f"cuda:{self.device}")
+        with torch.cuda.device(self.device):
+            N.check(N.lib().efwi_get_illumination(self._h, N.vptr(out), 0, self._stream()), "efwi_get_illumination")
+        return out
+
     # -- shots ------------------------------------------------------------
     def run_shot(self, shot: ShotSpec, calc_id: int, obs=None, want_syn=False, want_res=False):
         """Forward (+ misfit, + backward for calc_id 1) of one shot.

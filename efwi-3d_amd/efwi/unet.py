@@ -13,6 +13,9 @@ Architecture (3 levels, 16 base channels, 3x3x3 convolutions with ReLU,
         pool -> bot_a 32->64, bot_b 64->64 -> up -> cat2[0:64]          |
                   dec2a 96->32, dec2b 32->32 -> up -> cat1[0:32] -------+
                                     dec1a 48->16, dec1b 16->16 -> head 16->1 (FP32)
+    update = g exp(clamp(head, -4, 6))   (g = input channel 0: a learned,
+                            spatially varying scaling of the gradient, as a
+                            preconditioner is)
 
 Weights and activations are BF16 with FP32 accumulation; FP32 master weights
 and Adam state; activation gradients BF16, weight gradients FP32. The
@@ -63,7 +66,8 @@ class UNet3D:
                 w[:, :, IN_CHANNELS:] = 0.0
             self.w[name] = w.to(self.dev).contiguous()
             self.b[name] = torch.zeros(cout, device=self.dev)
-        self.wh = (torch.randn(16, generator=g) * math.sqrt(1.0 / 16)).to(self.dev)
+        # small head: the initial scaling exp(o) is close to 1 (the plain gradient)
+        self.wh = (torch.randn(16, generator=g) * 0.01).to(self.dev)
         self.bh = torch.zeros(1, device=self.dev)
         self._refresh_bf16()
         self.m1 = {k: torch.zeros_like(v) for k, v in self._params().items()}
@@ -109,7 +113,8 @@ class UNet3D:
         bf = lambda n, c: torch.zeros(n, c, dtype=torch.bfloat16, device=self.dev)
         B = {"a1": bf(v1, 16), "cat1": bf(v1, 48), "p1": bf(v2, 16), "a2": bf(v2, 32), "cat2": bf(v2, 96),
              "p2": bf(v3, 32), "a3": bf(v3, 64), "b3": bf(v3, 64), "d2a": bf(v2, 32), "d2b": bf(v2, 32),
-             "d1a": bf(v1, 16), "d1b": bf(v1, 16), "out": torch.zeros(v1, device=self.dev),
+             "d1a": bf(v1, 16), "d1b": bf(v1, 16), "o": torch.zeros(v1, device=self.dev),
+             "out": torch.zeros(v1, device=self.dev), "do": torch.zeros(v1, device=self.dev),
              # gradients
              "g1": bf(v1, 16), "g1b": bf(v1, 16), "gcat1": bf(v1, 48), "gcat2": bf(v2, 96), "g2": bf(v2, 32),
              "g2b": bf(v2, 32), "gp1": bf(v2, 16), "gp2": bf(v3, 32), "g3": bf(v3, 64), "g3b": bf(v3, 64),
@@ -150,7 +155,7 @@ class UNet3D:
 
     # ---- forward / backward
     def forward(self, feat, dims):
-        """feat: [V, 16] bfloat16 (channels-last, V = nz*nx*ny); returns the FP32 output [V]."""
+        """feat: [V, 16] bfloat16 (channels-last, V = nz*nx*ny); returns the FP32 update [V]."""
         B = self._buffers(dims)
         s = self._stream()
         d1, d2 = dims, tuple(x // 2 for x in dims)
@@ -170,18 +175,28 @@ class UNet3D:
         nn_check(self.lib.efwi_nn_up2(_p(B["d2b"]), 32, _p(B["cat1"]), 48, 32, *d1, s), "up1")
         self._conv("dec1a", B["cat1"], 48, B["d1a"], 16, d1)
         self._conv("dec1b", B["d1a"], 16, B["d1b"], 16, d1)
-        nn_check(self.lib.efwi_nn_head(_p(B["d1b"]), 16, 16, _p(self.wh), _p(self.bh), _p(B["out"]), v1, s), "head")
+        nn_check(self.lib.efwi_nn_head(_p(B["d1b"]), 16, 16, _p(self.wh), _p(self.bh), _p(B["o"]), v1, s), "head")
+        nn_check(self.lib.efwi_nn_gate(_p(feat), 16, _p(B["o"]), _p(B["out"]), v1, s), "gate")
         self._feat, self._dims = feat, dims
         return B["out"]
 
-    def backward(self, dout):
-        """Gradients of every parameter (self.grads) from dL/dout [V] (FP32)."""
+    def scaling_log(self, dims):
+        """The head output o of the last forward (FP32 [nz, nx, ny]): the
+        network's log-scaling of the gradient before the clamp to [-4, 6]."""
+        return self._buffers(dims)["o"].reshape(dims)
+
+    def backward(self, dout, through_gate=True):
+        """Gradients of every parameter (self.grads) from dL/dupdate [V] (FP32),
+        or from dL/do of the head output when through_gate is False."""
         B = self._buffers(self._dims)
         s = self._stream()
         d1 = self._dims
         d2, d3 = tuple(x // 2 for x in d1), tuple(x // 4 for x in d1)
         v1 = d1[0] * d1[1] * d1[2]
         v2 = v1 // 8
+        if through_gate:
+            nn_check(self.lib.efwi_nn_gate_bwd(_p(B["o"]), _p(B["out"]), _p(dout), _p(B["do"]), v1, s), "gate bwd")
+            dout = B["do"]
         nn_check(self.lib.efwi_nn_head_bwd(_p(B["d1b"]), 16, 16, _p(self.wh), _p(dout), _p(B["g1b"]), 16,
                                            _p(self.grads["head.w"]), _p(self.grads["head.b"]), v1, s), "head bwd")
         # level 1 decoder
@@ -232,10 +247,16 @@ class UNet3D:
                                            lr, b1, b2, eps, self.steps, None if w16 is None else _p(w16), s),
                      f"adam {k}")
 
-    def train_step(self, feat, target, mask, count, dims, lr=1e-3):
+    def train_step(self, feat, target, mask, count, dims, lr=1e-3, log_scaling=False):
+        """One Adam step on the mean-square error of the update (target: the
+        update), or of the head output (log_scaling: target = the log of the
+        per-voxel scaling update / gradient)."""
         out = self.forward(feat, dims)
-        loss, dout = self.loss(out, target, mask, count)
-        self.backward(dout)
+        if log_scaling:
+            loss, dout = self.loss(self._buffers(dims)["o"], target, mask, count)
+        else:
+            loss, dout = self.loss(out, target, mask, count)
+        self.backward(dout, through_gate=not log_scaling)
         self.adam(lr)
         return loss
 

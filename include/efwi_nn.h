@@ -75,6 +75,12 @@ int efwi_nn_head(const uint16_t* act, int ld, int c, const float* w, const float
 int efwi_nn_head_bwd(const uint16_t* act, int ld, int c, const float* w, const float* dout, uint16_t* dact, int ldd,
                      float* dw, float* db, long nvox, void* stream);
 
+/* Multiplicative output: u[v] = g[v] exp(clamp(o[v], -4, 6)) with g =
+ * channel 0 of the network input feat (BF16, stride ld) and o the head's FP32
+ * output; its gradient dout[v] = du[v] u[v] where -4 < o[v] < 6, else 0. */
+int efwi_nn_gate(const uint16_t* feat, int ld, const float* o, float* u, long nvox, void* stream);
+int efwi_nn_gate_bwd(const float* o, const float* u, const float* du, float* dout, long nvox, void* stream);
+
 /* Masked mean-square loss over `count` voxels: *loss = sum m (out - t)^2 /
  * count (FP64, device), dout = 2 m (out - t) / count. */
 int efwi_nn_mse(const float* out, const float* target, const float* mask, long nvox, double count, float* dout,

@@ -115,6 +115,19 @@ def head_bwd(act, w, dout):
     return dact, dw, dout.sum()
 
 
+GATE_LO, GATE_HI = -4.0, 6.0
+
+
+def gate(feat, o):
+    """u = g exp(clamp(o, -4, 6)), g = channel 0 of the network input."""
+    return np.asarray(feat, np.float64)[..., 0] * np.exp(np.clip(np.asarray(o, np.float64), GATE_LO, GATE_HI))
+
+
+def gate_bwd(o, u, du):
+    o = np.asarray(o, np.float64)
+    return np.where((o > GATE_LO) & (o < GATE_HI), np.asarray(du, np.float64) * np.asarray(u, np.float64), 0.0)
+
+
 def mse(out, target, mask, count):
     d = (np.asarray(out, np.float64) - target) * mask
     return float((d * d).sum() / count), 2.0 * d / count

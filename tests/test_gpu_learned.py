@@ -1,0 +1,72 @@
+"""What the trained network gains over the method without it, on the GPU.
+
+A small version of the learned-preconditioner workflow: training pairs from
+the plain method on 48 random layered-plus-anomaly models (48x64x64 physical,
+300 steps), 24 epochs on the library's BF16 MFMA kernels, then 8-iteration
+inversions of held-out models (4 shots) with plain L-BFGS and with L-BFGS
+whose initial inverse Hessian is the network (the plain step as fallback).
+The exact pseudo-Hessian scaling that the network was trained to reproduce is
+run alongside for reference. Reported: final misfit over the starting misfit
+for each method. Asserted: the learned preconditioner is accepted as the
+step and ends no worse than plain L-BFGS on average (measured on MI355X: 2-3 %
+lower; the exact scaling 5-20 % lower).
+"""
+import numpy as np
+import pytest
+import torch
+
+pytestmark = pytest.mark.gpu
+
+
+def test_learned_preconditioner_gain():
+    from efwi import learned as L
+    from efwi.inversion import lbfgs_invert
+    from efwi.propagator import Propagator
+
+    class Exact:
+        def __init__(self, net, problem):
+            self.problem, self.s = problem, None
+
+        def refresh(self, g, illum):
+            h = illum.clamp_min(0).pow(L.ILLUM_POWER)
+            self.s = self.problem.mask / (h + L.ILLUM_EPS * h.max())
+
+        def __call__(self, q):
+            return q * self.s
+
+    dims, npml, nt = (48, 64, 64), 12, 300
+    print(f"[gpu] learned preconditioner: 48 training models at {dims}", flush=True)
+    data, fdims, _ = L.generate(48, dims=dims, npml=npml, nt=nt, seed=1)
+    net, losses, _ = L.train(data, fdims, epochs=24)
+    assert np.mean(losses[-48:]) < np.mean(losses[:48])
+    rng = np.random.default_rng(99)
+    survey = L.surface_survey(*dims, npml, nt, 10.0, 1e-3, 10.0, 4, rng)
+    dev = torch.device("cuda:0")
+    rows = []
+    for m in range(2):
+        vp, vs, rho, vp0 = L.random_model(rng, *dims)
+        row = {}
+        for method in ("plain", "exact", "learned"):
+            prop = Propagator(survey.grid, 0)
+            prob = L.make_problem(prop, survey, vp, vs, rho, dev)
+            v0 = L._padded(vp0, npml, dev)
+            if method == "plain":
+                _, hist = lbfgs_invert(prob, v0, 8)
+            elif method == "exact":
+                orig = L.Preconditioner
+                L.Preconditioner = Exact
+                try:
+                    _, hist, _ = L.learned_invert(prob, v0, 8, None)
+                finally:
+                    L.Preconditioner = orig
+            else:
+                _, hist, counts = L.learned_invert(prob, v0, 8, net)
+                assert counts["learned"] >= 4, counts
+            prop.close()
+            row[method] = hist.f[-1] / hist.f[0]
+        rows.append(row)
+        print(f"[gpu] held-out model {m}: final/initial misfit "
+              + ", ".join(f"{k} {v:.4f}" for k, v in row.items()), flush=True)
+    gain = np.mean([r["learned"] / r["plain"] for r in rows])
+    print(f"[gpu] learned / plain final misfit: {gain:.4f}", flush=True)
+    assert gain <= 1.0

@@ -168,6 +168,14 @@ def test_pool_up_add_head_loss_adam_pack():
     dw, db = torch.zeros(16, device=DEV), torch.zeros(1, device=DEV)
     nn_check(lib.efwi_nn_head_bwd(_p(actd), c, c, _p(wd), _p(dout), _p(dact), c, _p(dw), _p(db), V, _s()), "head bwd")
     rda, rdw, rdb = O.head_bwd(act, w, dout.cpu().numpy())
+    # multiplicative output and its gradient
+    ug = torch.zeros(V, device=DEV)
+    nn_check(lib.efwi_nn_gate(_p(xd), c, _p(out), _p(ug), V, _s()), "gate")
+    assert np.allclose(ug.cpu().numpy(), O.gate(x.reshape(V, c), out.cpu().numpy()), rtol=1e-5, atol=1e-6)
+    dgo = torch.zeros(V, device=DEV)
+    nn_check(lib.efwi_nn_gate_bwd(_p(out), _p(ug), _p(dout), _p(dgo), V, _s()), "gate bwd")
+    assert np.allclose(dgo.cpu().numpy(), O.gate_bwd(out.cpu().numpy(), ug.cpu().numpy(), dout.cpu().numpy()),
+                       rtol=1e-6, atol=1e-12)
     assert _ulps_ok(_host(dact), rda).all()
     assert np.allclose(dw.cpu().numpy(), rdw, rtol=1e-4, atol=1e-7)
     assert np.isclose(float(db), rdb, rtol=1e-4, atol=1e-7)
@@ -222,11 +230,12 @@ def _oracle_network(net, feat, dims, target, mask, count):
     cat1 = np.concatenate([O.up2(d2b), e1], axis=3)
     d1a = conv("dec1a", cat1)
     d1b = conv("dec1b", d1a)
-    out = O.head(d1b, wh, bh).astype(np.float32)
+    o = O.head(d1b, wh, bh).astype(np.float32)
+    out = O.gate(x, o).astype(np.float32)
     _, dout = O.mse(out, target.reshape(dims), mask.reshape(dims), count)
-    dout = dout.astype(np.float32)
+    do = O.gate_bwd(o, out, dout.astype(np.float32)).astype(np.float32)
     G = {}
-    g1b, G["head.w"], _ = O.head_bwd(d1b, wh, dout)
+    g1b, G["head.w"], _ = O.head_bwd(d1b, wh, do)
     g1b = f(g1b)
 
     def layer(n, gpre, src, mask_src):
@@ -258,9 +267,11 @@ def test_unet_forward_and_weight_gradients_match_oracle():
     dims = (8, 12, 16)
     V = dims[0] * dims[1] * dims[2]
     net = UNet3D(0, seed=1)
-    # nonzero biases so that every ReLU mask has both signs
+    # nonzero biases so that every ReLU mask has both signs, and a head of
+    # trained size so that the gate's scaling varies
     for k in net.b:
         net.b[k].copy_(torch.from_numpy(rng.uniform(-0.05, 0.1, net.b[k].numel()).astype(np.float32)))
+    net.wh.copy_(torch.from_numpy(rng.standard_normal(16).astype(np.float32) * 0.25))
     f0 = np.zeros((V, 16), np.float32)
     f0[:, :2] = rng.standard_normal((V, 2))
     feat = _dev(f0)
@@ -284,9 +295,9 @@ def test_unet_forward_and_weight_gradients_match_oracle():
 
 
 def test_unet_training_lowers_the_loss():
-    """A short training run on the GPU's own kernels: learn a pointwise map of
-    the two input channels (the pseudo-Hessian scaling g / illumination) on
-    random smooth volumes; the loss must fall well below its start."""
+    """A short training run on the GPU's own kernels: learn the pseudo-Hessian
+    scaling g / (sqrt(illumination) + eps) of the two input channels on random
+    smooth volumes; the loss must fall well below its start."""
     from scipy.ndimage import gaussian_filter
     from efwi.unet import UNet3D
     rng = np.random.default_rng(2)
@@ -299,7 +310,7 @@ def test_unet_training_lowers_the_loss():
         il = np.exp(gaussian_filter(rng.standard_normal(dims), 3.0) * 2)
         g /= np.abs(g).max()
         il /= il.max()
-        u = -g / (il + 0.05)
+        u = g / (np.sqrt(il) + 0.05)
         u /= np.abs(u).max()
         f = np.zeros((V, 16), np.float32)
         f[:, 0] = g.ravel()
@@ -310,6 +321,6 @@ def test_unet_training_lowers_the_loss():
     for it in range(150):
         feat, t = data[it % 4]
         losses.append(float(net.train_step(feat, t, mask, V, dims, lr=3e-3)))
-    print(f"[gpu] U-Net training loss {losses[0]:.4f} -> {np.mean(losses[-8:]):.4f}", flush=True)
+    print(f"[gpu] U-Net training loss {losses[0]:.3e} -> {np.mean(losses[-8:]):.3e}", flush=True)
     assert np.isfinite(losses).all()
     assert np.mean(losses[-8:]) < 0.3 * np.mean(losses[:4])

@@ -93,6 +93,29 @@ def test_head_and_loss_gradients():
     assert np.isclose(dact[idx], fd, rtol=1e-5, atol=1e-9)
 
 
+def test_gate_gradient():
+    rng = np.random.default_rng(6)
+    feat = _rand(rng, 3, 4, 5, 16)
+    o = 0.3 * _rand(rng, 3, 4, 5)
+    t = _rand(rng, 3, 4, 5)
+    m = np.ones((3, 4, 5))
+    u = O.gate(feat, o)
+    assert np.allclose(u, feat[..., 0] * np.exp(o))
+    _, du = O.mse(u, t, m, m.sum())
+    do = O.gate_bwd(o, u, du)
+    h = 1e-6
+    for idx in ((0, 0, 0), (2, 3, 4), (1, 2, 1)):
+        op, om = o.copy(), o.copy()
+        op[idx] += h
+        om[idx] -= h
+        fd = (O.mse(O.gate(feat, op), t, m, m.sum())[0] - O.mse(O.gate(feat, om), t, m, m.sum())[0]) / (2 * h)
+        assert np.isclose(do[idx], fd, rtol=1e-6, atol=1e-10)
+    # outside the clamp the output is constant in o
+    big = np.full_like(o, 9.0)
+    assert np.allclose(O.gate(feat, big), feat[..., 0] * np.exp(6.0))
+    assert not O.gate_bwd(big, O.gate(feat, big), du).any()
+
+
 def test_adam_first_step_is_sign_times_lr():
     g = np.array([0.5, -2.0, 1e-3])
     w, m1, m2 = O.adam(np.zeros(3), g, np.zeros(3), np.zeros(3), 1, 0.01, eps=0.0)
