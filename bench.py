@@ -212,7 +212,8 @@ def bench_learned(args):
     e0.record()
     nstep = 10
     for k in range(nstep):
-        net.train_step(data[k][0], data[k][2], mask, V, fdims, lr=0.0, log_scaling=True)
+        feat, _, logs = data[k % len(data)]
+        net.train_step(feat, logs, mask, V, fdims, lr=0.0, log_scaling=True)
     e1.record()
     torch.cuda.synchronize()
     step_ms = e0.elapsed_time(e1) / nstep
@@ -249,7 +250,7 @@ def bench_learned(args):
     out = {
         "metric": "wall time of the learned-preconditioned inversion to the plain L-BFGS final misfit, "
                   "relative to plain L-BFGS",
-        "value": round(float(np.mean(reached)), 4) if len(reached) == len(ratios) else None,
+        "value": round(float(np.mean(reached)), 4) if ratios and len(reached) == len(ratios) else None,
         "unit": "ratio",
         "n_gpus": 1, "steps": args.iters, "warmup": 0,
         "ms_per_step": None,

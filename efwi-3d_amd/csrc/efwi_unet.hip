@@ -163,22 +163,35 @@ __global__ __launch_bounds__(64) void k_wgrad(const uint16_t* __restrict__ dpre,
     for (int j = 0; j < 4; j++) out[(long)(mb * 16 + kq + j) * kc + b * 16 + i] = acc[b][j];
 }
 
+// out[k] += sum over chunks [blockIdx.y * kChunkGroup, +kChunkGroup) of part[chunk][k]
+// (out cleared beforehand; one atomic per element per chunk group)
+constexpr int kChunkGroup = 32;
 __global__ void k_sum_chunks(const float* __restrict__ part, long width, int nchunks, float* __restrict__ out) {
   const long k = (long)blockIdx.x * blockDim.x + threadIdx.x;
   if (k >= width) return;
+  const int c0 = blockIdx.y * kChunkGroup, c1 = min(nchunks, c0 + kChunkGroup);
   float s = 0.f;
-  for (int c = 0; c < nchunks; c++) s += part[(long)c * width + k];
-  out[k] = s;
+  for (int c = c0; c < c1; c++) s += part[(long)c * width + k];
+  atomicAdd(out + k, s);
 }
 
-// out[c] += sum_v x[v][c] (bias gradients); blockDim 256, c < 256 / stride groups
+// out[c] += sum_v x[v][c] (bias gradients; out cleared beforehand). blockDim
+// 256 = (256 / c voxel lanes) x c channels; the lanes are reduced in LDS and
+// each block adds one value per channel.
 __global__ void k_chan_sum(const uint16_t* __restrict__ x, int ld, int c, long nvox, float* __restrict__ out) {
-  const int lanes = 256 / c;  // voxel lanes per channel
+  __shared__ float red[256];
+  const int lanes = 256 / c;
   const int ch = threadIdx.x % c, vl = threadIdx.x / c;
-  if (vl >= lanes) return;
   float s = 0.f;
-  for (long v = (long)blockIdx.x * lanes + vl; v < nvox; v += (long)gridDim.x * lanes) s += bf2f(x[v * ld + ch]);
-  atomicAdd(out + ch, s);
+  if (vl < lanes)
+    for (long v = (long)blockIdx.x * lanes + vl; v < nvox; v += (long)gridDim.x * lanes) s += bf2f(x[v * ld + ch]);
+  red[threadIdx.x] = s;
+  __syncthreads();
+  if (threadIdx.x < c) {
+    float t = 0.f;
+    for (int q = 0; q < lanes; q++) t += red[q * c + threadIdx.x];
+    atomicAdd(out + threadIdx.x, t);
+  }
 }
 
 // 2x2x2 average pooling; (nz, nx, ny) are the fine dimensions (even)
@@ -445,7 +458,10 @@ int efwi_nn_wgrad(const uint16_t* dpre, int ldp, int mc, const uint16_t* src, in
     default: hipLaunchKernelGGL((k_wgrad<6>), grid, dim3(64), 0, s, dpre, ldp, mc, src, lds, kc, scratch, V, per); break;
   }
   const long width = 27L * mc * kc;
-  hipLaunchKernelGGL(k_sum_chunks, dim3(grid_for(width, 256, 1L << 30)), dim3(256), 0, s, scratch, width, chunks, dw);
+  hipLaunchKernelGGL(k_zero, dim3((unsigned)((width + 255) / 256)), dim3(256), 0, s, reinterpret_cast<unsigned*>(dw),
+                     (int)width);
+  hipLaunchKernelGGL(k_sum_chunks, dim3(grid_for(width, 256, 1L << 30), (chunks + kChunkGroup - 1) / kChunkGroup),
+                     dim3(256), 0, s, scratch, width, chunks, dw);
   return nn_check(s);
 }
 
@@ -453,7 +469,7 @@ int efwi_nn_chan_sum(const uint16_t* x, int ld, int c, long nvox, float* out, vo
   if (!x || !out || c <= 0 || c > 256 || 256 % c) return EFWI_NN_ERR_ARG;
   hipStream_t s = (hipStream_t)stream;
   hipLaunchKernelGGL(k_zero, dim3((c + 255) / 256), dim3(256), 0, s, reinterpret_cast<unsigned*>(out), c);
-  hipLaunchKernelGGL(k_chan_sum, dim3(grid_for(nvox * c, 256, 2048)), dim3(256), 0, s, x, ld, c, nvox, out);
+  hipLaunchKernelGGL(k_chan_sum, dim3(grid_for(nvox * c, 256, 256)), dim3(256), 0, s, x, ld, c, nvox, out);
   return nn_check(s);
 }
 
