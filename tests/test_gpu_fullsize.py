@@ -76,3 +76,38 @@ def test_dot64_matches_numpy(n):
     ref = float(np.dot(a.astype(np.float64), b.astype(np.float64)))
     # each product of two floats is exact in FP64; only the summation order differs
     assert abs(got - ref) <= 1e-12 * max(1.0, float(np.sum(np.abs(a.astype(np.float64) * b))))
+
+
+def test_cfg2_first_300_steps_match_oracle_summary():
+    """The production grid (200x200x120 physical + 20 C-PML cells, padded
+    160x240x240, 10,000 receivers) for its first 300 steps against the fp32
+    oracle's summary (tests/golden/cfg2_n300_summary.npz, written by
+    make_cfg2_summary.py): seismograms, residuals, misfit and the gradients'
+    per-plane sums. The GPU path is bit-identical to the oracle's fp32 mode,
+    so the selected traces must be equal and every summary within 1e-5
+    relative L2 (in 300 steps the anomalies' reflections have not reached the
+    surface: the misfit is small, and the comparison is of the arithmetic)."""
+    import sys
+    from efwi import synth
+    from efwi.propagator import Propagator
+    sys.path.insert(0, os.path.join(HERE, "golden"))
+    from make_cfg2_summary import NT, summarize
+    w = synth.cfg2(nt=NT)
+    print(f"[gpu] cfg2 grid={w.grid.shape} nt={w.grid.nt} nrec={len(w.shots[0].rec)}", flush=True)
+    with Propagator(w.grid) as P:
+        P.set_model(*(torch.from_numpy(a) for a in w.true))
+        _, obs, _ = P.run_shot(w.shots[0], 2, None, want_syn=True)
+        P.set_model(*(torch.from_numpy(a) for a in w.model))
+        P.zero_gradient()
+        j, syn, res = P.run_shot(w.shots[0], 1, obs, want_syn=True, want_res=True)
+        g = [x.cpu().numpy() for x in P.gradient()]
+        torch.cuda.synchronize()
+    got = summarize(syn.cpu().numpy(), res.cpu().numpy(), j, *g)
+    got["obs_trace_energy"] = np.sum(np.asarray(obs.cpu().numpy(), np.float64) ** 2, axis=2)
+    ref = np.load(os.path.join(HERE, "golden", "cfg2_n300_summary.npz"))
+    assert abs(got["misfit64"] - ref["misfit64"]) <= 1e-9 * abs(ref["misfit64"])
+    for k in ref.files:
+        e = _rel(got[k], ref[k])
+        print(f"{k}: rel-L2 {e:.3e}")
+        assert e <= TOL, k
+    assert np.array_equal(got["traces"], ref["traces"])
