@@ -234,12 +234,13 @@ int Ctx::run_batch(const efwi_shot* shots, int n, int calc_id, const float* cons
 int Ctx::batch_gradient(int shot, float* gl, float* gm, float* gr, int on_host, hipStream_t s) {
   if (shot < 0 || shot >= batch.n) return fail(EFWI_ERR_ARG, "no such shot in the batch");
   HIP_TRY(hipSetDevice(cfg.device));
-  const size_t fb = (size_t)N * sizeof(float), fsf = G.fs / sizeof(float);
+  const size_t fsf = G.fs / sizeof(float);
   const hipMemcpyKind k = on_host ? hipMemcpyDeviceToHost : hipMemcpyDeviceToDevice;
   const float* m = batch.F[shot].m;
-  if (gl) HIP_TRY(hipMemcpyAsync(gl, m + M_GLAM * fsf, fb, k, s));
-  if (gm) HIP_TRY(hipMemcpyAsync(gm, m + M_GMU * fsf, fb, k, s));
-  if (gr) HIP_TRY(hipMemcpyAsync(gr, m + M_GRHO * fsf, fb, k, s));
+  int rc;
+  if (gl && (rc = copy_out(gl, m + M_GLAM * fsf, k, s))) return rc;
+  if (gm && (rc = copy_out(gm, m + M_GMU * fsf, k, s))) return rc;
+  if (gr && (rc = copy_out(gr, m + M_GRHO * fsf, k, s))) return rc;
   if (on_host) HIP_TRY(hipStreamSynchronize(s));
   return EFWI_OK;
 }

@@ -215,8 +215,10 @@ int Ctx::init() {
   if (c.device < 0 || c.device >= ndev) return fail(EFWI_ERR_ARG, "device ordinal out of range");
   HIP_TRY(hipSetDevice(c.device));
   G.nz = c.nz; G.nx = c.nx; G.ny = c.ny; G.npml = c.npml;
-  G.py = c.ny;
-  G.pxy = (long)c.nx * c.ny;
+  // rows padded to whole 128-byte lines: a wavefront's 64-cell y segment then
+  // covers exactly two lines (the padding cells are never computed or read)
+  G.py = (c.ny + 31) / 32 * 32;
+  G.pxy = (long)c.nx * G.py;
   G.dz = c.dz; G.dx = c.dx; G.dy = c.dy; G.dt = c.dt;
   N = (long)c.nz * G.pxy;
   // field groups: fields fs bytes apart (256-byte aligned), each group below 4 GiB
@@ -437,21 +439,29 @@ int Ctx::checkpoint(int k, const Wave& Wf, const Psi& Pf, bool save, hipStream_t
   return EFWI_OK;
 }
 
+int Ctx::copy_in(float* field, const float* user, hipMemcpyKind k, hipStream_t s) {
+  const size_t row = (size_t)cfg.ny * sizeof(float);
+  HIP_TRY(hipMemcpy2DAsync(field, (size_t)G.py * sizeof(float), user, row, row, (size_t)cfg.nz * cfg.nx, k, s));
+  return EFWI_OK;
+}
+
+int Ctx::copy_out(float* user, const float* field, hipMemcpyKind k, hipStream_t s) {
+  const size_t row = (size_t)cfg.ny * sizeof(float);
+  HIP_TRY(hipMemcpy2DAsync(user, row, field, (size_t)G.py * sizeof(float), row, (size_t)cfg.nz * cfg.nx, k, s));
+  return EFWI_OK;
+}
+
 int Ctx::set_model(const float* l, const float* m, const float* rho, int on_host, hipStream_t s) {
   HIP_TRY(hipSetDevice(cfg.device));
-  const size_t fb = (size_t)N * sizeof(float);
   const hipMemcpyKind k = on_host ? hipMemcpyHostToDevice : hipMemcpyDeviceToDevice;
-  HIP_TRY(hipMemcpyAsync(lam, l, fb, k, s));
-  HIP_TRY(hipMemcpyAsync(mu, m, fb, k, s));
-  float* rho_dev = const_cast<float*>(rho);
-  if (on_host) {
-    // stage rho through the gradient buffer, which is zeroed before use
-    HIP_TRY(hipMemcpyAsync(Gr.grho, rho, fb, hipMemcpyHostToDevice, s));
-    rho_dev = Gr.grho;
-  }
-  launch_byc(G, rho_dev, byc, s);
+  int rc;
+  if ((rc = copy_in(lam, l, k, s)) || (rc = copy_in(mu, m, k, s))) return rc;
+  // rho is staged in the row layout through the gradient buffer, then the
+  // buffer is cleared again
+  if ((rc = copy_in(Gr.grho, rho, k, s))) return rc;
+  launch_byc(G, Gr.grho, byc, s);
   HIP_TRY(hipGetLastError());
-  if (on_host) launch_fill(Gr.grho, N, 0.0f, s);
+  launch_fill(Gr.grho, N, 0.0f, s);
   model_set = true;
   return EFWI_OK;
 }
@@ -700,11 +710,11 @@ int Ctx::autotune(int reps, hipStream_t s, double* report) {
 
 int Ctx::get_gradient(float* gl, float* gm, float* gr, int on_host, hipStream_t s) {
   HIP_TRY(hipSetDevice(cfg.device));
-  const size_t fb = (size_t)N * sizeof(float);
   const hipMemcpyKind k = on_host ? hipMemcpyDeviceToHost : hipMemcpyDeviceToDevice;
-  if (gl) HIP_TRY(hipMemcpyAsync(gl, Gr.glam, fb, k, s));
-  if (gm) HIP_TRY(hipMemcpyAsync(gm, Gr.gmu, fb, k, s));
-  if (gr) HIP_TRY(hipMemcpyAsync(gr, Gr.grho, fb, k, s));
+  int rc;
+  if (gl && (rc = copy_out(gl, Gr.glam, k, s))) return rc;
+  if (gm && (rc = copy_out(gm, Gr.gmu, k, s))) return rc;
+  if (gr && (rc = copy_out(gr, Gr.grho, k, s))) return rc;
   if (on_host) HIP_TRY(hipStreamSynchronize(s));
   return EFWI_OK;
 }
@@ -782,8 +792,8 @@ int efwi_get_illumination(efwi_ctx* ctx, float* out, int on_host, void* stream) 
   if (!c->illum) return fail(EFWI_ERR_STATE, "illumination is off (efwi_set_illumination)");
   HIP_TRY(hipSetDevice(c->cfg.device));
   hipStream_t s = (hipStream_t)stream;
-  HIP_TRY(hipMemcpyAsync(out, c->illum, sizeof(float) * (size_t)c->N,
-                         on_host ? hipMemcpyDeviceToHost : hipMemcpyDeviceToDevice, s));
+  int rc = c->copy_out(out, c->illum, on_host ? hipMemcpyDeviceToHost : hipMemcpyDeviceToDevice, s);
+  if (rc) return rc;
   if (on_host) HIP_TRY(hipStreamSynchronize(s));
   return EFWI_OK;
 }

@@ -117,6 +117,9 @@ struct Ctx {
 
   ~Ctx();
   int alloc(void** p, size_t bytes);
+  // user volume [nz][nx][ny] (contiguous) <-> device field (rows G.py apart)
+  int copy_in(float* field, const float* user, hipMemcpyKind k, hipStream_t s);
+  int copy_out(float* user, const float* field, hipMemcpyKind k, hipStream_t s);
   int init();
   int plan_memory();
   int ensure_ring();
