@@ -51,36 +51,50 @@ __device__ __forceinline__ void tap(int t, int& dz, int& dx, int& dy) {
   dy = t % 3 - 1;
 }
 
-// One wavefront per 16-voxel row segment; NB output blocks of 16 channels
-// starting at block blockIdx.y * NB. MODE 0 forward, 1 dgrad.
-template <int NB, int MODE>
+// One wavefront per row segment of 16 MB voxels along y (MB row blocks of the
+// MFMA's A operand), NB output blocks of 16 channels starting at block
+// blockIdx.y * NB: every B fragment (weights) is used MB times, every A
+// fragment NB times. MODE 0 forward, 1 dgrad.
+template <int NB, int MB, int MODE>
 __global__ __launch_bounds__(256) void k_conv3(const uint16_t* __restrict__ src, int lds, int kc,
                                                const uint16_t* __restrict__ w, const float* __restrict__ bias,
                                                uint16_t* __restrict__ dst, int ldd, int nc,
                                                const uint16_t* __restrict__ mask, int ldm, Vol V, int relu) {
   const int lane = threadIdx.x & 63;
   const long tile = (long)blockIdx.x * 4 + (threadIdx.x >> 6);
-  const int tpr = (V.ny + 15) / 16;
+  const int tpr = (V.ny + 16 * MB - 1) / (16 * MB);
   const long rows = (long)V.nz * V.nx;
   if (tile >= rows * tpr) return;  // wavefront-uniform
   const int row = (int)(tile / tpr);
-  const int y0 = (int)(tile - (long)row * tpr) * 16;
+  const int y0 = (int)(tile - (long)row * tpr) * 16 * MB;
   const int z = row / V.nx, x = row - z * V.nx;
   const int i = lane & 15, kq = (lane >> 4) * 4;
   const int nb0 = blockIdx.y * NB, nbt = nc / 16;
-  f4 acc[NB];
+  f4 acc[MB][NB];
 #pragma unroll
-  for (int b = 0; b < NB; b++) acc[b] = f4{0.f, 0.f, 0.f, 0.f};
+  for (int m = 0; m < MB; m++)
+#pragma unroll
+    for (int b = 0; b < NB; b++) acc[m][b] = f4{0.f, 0.f, 0.f, 0.f};
   const int sgn = MODE == 0 ? 1 : -1;
   for (int t = 0; t < 27; t++) {
     int dz, dx, dy;
     tap(t, dz, dx, dy);
-    const int zs = z + sgn * dz, xs = x + sgn * dx, ys = y0 + i + sgn * dy;
-    const bool ok = V.in(zs, xs, ys);
-    const long vs = ok ? V.vox(zs, xs, ys) : 0;
+    const int zs = z + sgn * dz, xs = x + sgn * dx;
+    bool ok[MB];
+    long vs[MB];
+#pragma unroll
+    for (int m = 0; m < MB; m++) {
+      const int ys = y0 + 16 * m + i + sgn * dy;
+      ok[m] = V.in(zs, xs, ys);
+      vs[m] = ok[m] ? V.vox(zs, xs, ys) : 0;
+    }
     for (int kb = 0; kb < kc; kb += 16) {
-      s4 a = s4{0, 0, 0, 0};
-      if (ok) a = *reinterpret_cast<const s4*>(src + vs * lds + kb + kq);
+      s4 a[MB];
+#pragma unroll
+      for (int m = 0; m < MB; m++) {
+        a[m] = s4{0, 0, 0, 0};
+        if (ok[m]) a[m] = *reinterpret_cast<const s4*>(src + vs[m] * lds + kb + kq);
+      }
 #pragma unroll
       for (int b = 0; b < NB; b++) {
         if (nb0 + b >= nbt) break;
@@ -92,7 +106,8 @@ __global__ __launch_bounds__(256) void k_conv3(const uint16_t* __restrict__ src,
 #pragma unroll
           for (int j = 0; j < 4; j++) bw[j] = (short)w[((long)t * kc + kb + kq + j) * nc + n];  // w[t][k][n]
         }
-        acc[b] = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(a, bw, acc[b], 0, 0, 0);
+#pragma unroll
+        for (int m = 0; m < MB; m++) acc[m][b] = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(a[m], bw, acc[m][b], 0, 0, 0);
       }
     }
   }
@@ -102,15 +117,17 @@ __global__ __launch_bounds__(256) void k_conv3(const uint16_t* __restrict__ src,
     const int n = (nb0 + b) * 16 + i;
     const float bn = bias ? bias[n] : 0.f;
 #pragma unroll
-    for (int j = 0; j < 4; j++) {
-      const int y = y0 + kq + j;
-      if (y >= V.ny) continue;
-      const long v = V.vox(z, x, y);
-      float r = acc[b][j] + bn;
-      if (relu) r = fmaxf(r, 0.f);
-      if (mask && !(bf2f(mask[v * ldm + n]) > 0.f)) r = 0.f;
-      dst[v * ldd + n] = f2bf(r);
-    }
+    for (int m = 0; m < MB; m++)
+#pragma unroll
+      for (int j = 0; j < 4; j++) {
+        const int y = y0 + 16 * m + kq + j;
+        if (y >= V.ny) continue;
+        const long v = V.vox(z, x, y);
+        float r = acc[m][b][j] + bn;
+        if (relu) r = fmaxf(r, 0.f);
+        if (mask && !(bf2f(mask[v * ldm + n]) > 0.f)) r = 0.f;
+        dst[v * ldd + n] = f2bf(r);
+      }
   }
 }
 
@@ -415,20 +432,26 @@ int efwi_nn_conv3(const uint16_t* src, int lds, int kc, const uint16_t* w, const
     return EFWI_NN_ERR_ARG;
   hipStream_t s = (hipStream_t)stream;
   const Vol V{nz, nx, ny};
-  const long tiles = (long)nz * nx * ((ny + 15) / 16);
+  // row segments of 64 voxels (4 MFMA row blocks) where the rows are long enough
+  const int MBv = ny >= 48 ? 4 : 1;
+  const long tiles = (long)nz * nx * ((ny + 16 * MBv - 1) / (16 * MBv));
   const int nbt = nc / 16;
   const int NB = nbt >= 4 ? 4 : nbt;  // output blocks per wavefront
   const dim3 grid((unsigned)((tiles + 3) / 4), (unsigned)((nbt + NB - 1) / NB));
-#define EFWI_NN_CONV(B)                                                                                          \
-  if (mode == 0)                                                                                               \
-    hipLaunchKernelGGL((k_conv3<B, 0>), grid, dim3(256), 0, s, src, lds, kc, w, bias, dst, ldd, nc, mask, ldm, V, relu); \
-  else                                                                                                         \
-    hipLaunchKernelGGL((k_conv3<B, 1>), grid, dim3(256), 0, s, src, lds, kc, w, bias, dst, ldd, nc, mask, ldm, V, relu)
-  if (NB == 1) EFWI_NN_CONV(1);
-  else if (NB == 2) EFWI_NN_CONV(2);
-  else if (NB == 3) EFWI_NN_CONV(3);
-  else EFWI_NN_CONV(4);
+#define EFWI_NN_CONV_M(B, M)                                                                                       \
+  if (mode == 0)                                                                                                 \
+    hipLaunchKernelGGL((k_conv3<B, M, 0>), grid, dim3(256), 0, s, src, lds, kc, w, bias, dst, ldd, nc, mask, ldm, V, relu); \
+  else                                                                                                           \
+    hipLaunchKernelGGL((k_conv3<B, M, 1>), grid, dim3(256), 0, s, src, lds, kc, w, bias, dst, ldd, nc, mask, ldm, V, relu)
+#define EFWI_NN_CONV(B)                \
+  if (MBv == 4) { EFWI_NN_CONV_M(B, 4); } \
+  else { EFWI_NN_CONV_M(B, 1); }
+  if (NB == 1) { EFWI_NN_CONV(1); }
+  else if (NB == 2) { EFWI_NN_CONV(2); }
+  else if (NB == 3) { EFWI_NN_CONV(3); }
+  else { EFWI_NN_CONV(4); }
 #undef EFWI_NN_CONV
+#undef EFWI_NN_CONV_M
   return nn_check(s);
 }
 
