@@ -217,7 +217,7 @@ __device__ __forceinline__ void sweep_block(const Sweep& S, const Geom& G, int b
   if (row >= rows) return;
   Cell c;
   c.x = S.x0 + xcd * S.rows_base + min(xcd, S.rows_rem) + row;
-  c.y = yb * 64 + (int)threadIdx.x;
+  c.y = (S.yb0 + yb) * 64 + (int)threadIdx.x;
   if (c.y < S.y0 || c.y > S.y1) return;
   const unsigned sx = 4u * (unsigned)G.py, sz = 4u * (unsigned)G.pxy;
   const int za = S.z0 + zc * S.Z;
@@ -763,7 +763,7 @@ __device__ __forceinline__ void march(const Sweep& Sw, const Geom& G, const Grou
   const int row = __builtin_amdgcn_readfirstlane((int)threadIdx.y);  // blockDim.x == 64
   const int lane = (int)threadIdx.x;
   const int x_base = Sw.x0 + xcd * Sw.rows_base + min(xcd, Sw.rows_rem) + rg * R;
-  const int y_base = yb * 64;
+  const int y_base = (Sw.yb0 + yb) * 64;
   Cell c;
   c.x = x_base + row;
   c.y = y_base + lane;
@@ -1040,7 +1040,8 @@ Sweep make_sweep(int z0, int z1, int x0, int x1, int y0, int y1, Tile t) {
   const int rows_max = S.rows_base + (S.rows_rem ? 1 : 0);
   S.rg_max = (rows_max + S.R - 1) / S.R;
   S.y0 = y0; S.y1 = y1;
-  S.nyb = (y1 + 64) / 64;  // segments of 64 aligned at y = 0
+  S.yb0 = y0 / 64;  // segments of 64 aligned at y = 0, from the one holding y0
+  S.nyb = y1 / 64 - S.yb0 + 1;
   S.nzc = (S.nz + S.Z - 1) / S.Z;
   S.blocks = 8L * S.rg_max * S.nyb * S.nzc;
   return S;
