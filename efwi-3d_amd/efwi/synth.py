@@ -137,10 +137,12 @@ def cfg2(nt=3000, scale=1.0):
                    [(5, nx // 2, ny // 2)], rec)
 
 
-def small(n=32, npml=6, nt=120, nrec_side=8, seed=3, formulation=0, h=10.0, dt=1e-3, f0=15.0):
-    """A small heterogeneous case for parity tests (seeded)."""
+def small(n=32, npml=6, nt=120, nrec_side=8, seed=3, formulation=0, h=10.0, dt=1e-3, f0=15.0, ny=None):
+    """A small heterogeneous case for parity tests (seeded): n^3 cells, or
+    n x n x ny with a longer y axis."""
     rng = np.random.default_rng(seed)
-    nz = nx = ny = n
+    nz = nx = n
+    ny = n if ny is None else ny
     vps = np.sort(rng.uniform(2000.0, 3200.0, size=3))
     interfaces = np.sort(rng.choice(np.arange(4, nz - 4), size=2, replace=False))
     vp_b = layered(nz, nx, ny, vps, interfaces)
@@ -149,7 +151,8 @@ def small(n=32, npml=6, nt=120, nrec_side=8, seed=3, formulation=0, h=10.0, dt=1
     rho_b = 310.0 * vp_b ** 0.25
     vp_a = vp_b * (1.0 + 0.05 * gaussian(nz, nx, ny, (nz // 2, nx // 2, ny // 2), n / 10.0))
     xs = np.linspace(1, nx - 2, nrec_side).astype(int)
-    gx, gy = np.meshgrid(xs, xs, indexing="ij")
+    ys = np.linspace(1, ny - 2, nrec_side).astype(int)
+    gx, gy = np.meshgrid(xs, ys, indexing="ij")
     rec = np.stack([np.full(gx.size, 2), gx.ravel(), gy.ravel()], axis=1)
     return _finish("small", nz, nx, ny, npml, h, dt, nt, f0, vp_b, vs_b, rho_b, vp_a, vs_b, rho_b,
                    [(3, nx // 2, ny // 2)], rec, formulation)
