@@ -530,14 +530,29 @@ int Ctx::prepare_shot(const efwi_shot* sh, int calc_id) {
 
 // One forward time step on (Wf, Pf): stress + source, velocity, and (record)
 // the receiver gather of step it + 1.
-void Ctx::forward_step(const Groups& Ff, const Wave& Wf, const Source& Sf, int it, bool record, hipStream_t s) {
+void Ctx::forward_step(const Groups& Ff, const Wave& Wf, const Source& Sf, int it, bool record, hipStream_t s,
+                       long save) {
+  // save >= 0: ring slot of the state this step produces, written by the
+  // stencil kernels themselves where they can (a source added after the stress
+  // kernel leaves the stress fields to the separate pass)
+  RingFold rf{};
+  rf.B = B;
+  rf.slot = save >= 0 ? ring + save * 9 * B.n : nullptr;
   prof.begin(K_STRESS_FWD, s);
-  launch_stress_fwd(tiles[K_STRESS_FWD], G, Ff, C, Sf, it, s);
+  const bool fs = launch_stress_fwd(tiles[K_STRESS_FWD], G, Ff, C, Sf, it, s, src_stencil ? &rf : nullptr);
   prof.end(s);
   if (!src_stencil) launch_add_source(Wf.szz, Wf.sxx, Wf.syy, src_cell, stf_dev, it, G.dt, +1, s);
   prof.begin(K_VELOCITY_FWD, s);
-  launch_velocity_fwd(tiles[K_VELOCITY_FWD], cfg.formulation, G, Ff, C, s);
+  const bool fv = launch_velocity_fwd(tiles[K_VELOCITY_FWD], cfg.formulation, G, Ff, C, s, &rf);
   prof.end(s);
+  if (save >= 0 && !(fs && fv)) {
+    Fields9 F9;
+    fields9(Wf, F9);
+    prof.begin(K_RING, s);
+    if (!fv) launch_ring(B, G, F9, ring, save, F_VZ, F_VY + 1, 0, s);
+    if (!fs) launch_ring(B, G, F9, ring, save, F_SZZ, F_WAVE_COUNT, 0, s);
+    prof.end(s);
+  }
   if (record) launch_record(Wf, rec_dev, nrec, it + 1, syn_t, s);
 }
 
@@ -571,13 +586,16 @@ int Ctx::run_shot(const efwi_shot* sh, int calc_id, const float* obs, float* syn
     const int seg = it / seg_len;
     if (grad && cfg.memory_mode == 1 && it % seg_len == 0 && seg < last_seg)
       if ((rc = checkpoint(seg, W, P, true, s))) return rc;
-    // the ring slots of the last segment are filled by this pass directly
-    if (grad && seg == last_seg) {
+    // The ring slots of the last segment are filled by this pass directly:
+    // slot it - a holds the state before step it (a: the segment's first
+    // step), the first by a pass of its own, the others by the step before.
+    const int a = last_seg * seg_len;
+    if (grad && it == a) {
       prof.begin(K_RING, s);
-      launch_ring(B, G, F9, ring, it - seg * seg_len, 0, 9, 0, s);
+      launch_ring(B, G, F9, ring, 0, 0, 9, 0, s);
       prof.end(s);
     }
-    forward_step(F, W, Sf, it, true, s);
+    forward_step(F, W, Sf, it, true, s, grad && it >= a && it + 1 <= nt - 2 ? it + 1 - a : -1);
     if (illum_on) launch_illum(W.vz, W.vx, W.vy, illum, N, s);
   }
   HIP_TRY(hipGetLastError());
@@ -619,25 +637,28 @@ int Ctx::run_shot(const efwi_shot* sh, int calc_id, const float* obs, float* syn
       if (seg < last_seg) {
         // replay the segment from its checkpoint to rebuild its ring
         if ((rc = checkpoint(seg, W2, P2, false, s))) return rc;
-        for (int it = a; it <= b; it++) {
-          prof.begin(K_RING, s);
-          launch_ring(B, G, F9r, ring, it - a, 0, 9, 0, s);
-          prof.end(s);
-          forward_step(F2, W2, Sf, it, false, s);
-        }
+        prof.begin(K_RING, s);
+        launch_ring(B, G, F9r, ring, 0, 0, 9, 0, s);
+        prof.end(s);
+        for (int it = a; it <= b; it++) forward_step(F2, W2, Sf, it, false, s, it < b ? it + 1 - a : -1);
       }
       for (int it = b; it >= a; it--) {
+        // the restores of slot it - a, folded into the kernels before them
+        // where those can take them
         const long slot = it - a;
+        RingFold rf{};
+        rf.B = B;
+        rf.slot = ring + slot * 9 * B.n;
         prof.begin(K_VELOCITY_BWD, s);
-        launch_velocity_bwd(tiles[K_VELOCITY_BWD], form, G, F, C, s);
+        const bool fv = launch_velocity_bwd(tiles[K_VELOCITY_BWD], form, G, F, C, s, &rf);
         prof.end(s);
-        launch_ring(B, G, F9, ring, slot, 0, 3, 1, s);
+        if (!fv) launch_ring(B, G, F9, ring, slot, F_VZ, F_VY + 1, 1, s);
         if (!src_interior) launch_add_source(W.szz, W.sxx, W.syy, src_cell, stf_dev, it, G.dt, -1, s);
         launch_inject(A, rec_dev, nrec, it, res_t, s);
         prof.begin(K_STRESS_BWD, s);
-        launch_stress_bwd(tiles[K_STRESS_BWD], G, F, C, S, it, s);
+        const bool fs = launch_stress_bwd(tiles[K_STRESS_BWD], G, F, C, S, it, s, &rf);
         prof.end(s);
-        launch_ring(B, G, F9, ring, slot, 3, 9, 1, s);
+        if (!fs) launch_ring(B, G, F9, ring, slot, F_SZZ, F_WAVE_COUNT, 1, s);
       }
     }
     HIP_TRY(hipGetLastError());

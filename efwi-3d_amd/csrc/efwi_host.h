@@ -126,7 +126,8 @@ struct Ctx {
   int ensure_checkpoints();
   size_t slab_floats(int axis) const;
   int checkpoint(int k, const Wave& Wf, const Psi& Pf, bool save, hipStream_t s);
-  void forward_step(const Groups& Ff, const Wave& Wf, const Source& Sf, int it, bool record, hipStream_t s);
+  void forward_step(const Groups& Ff, const Wave& Wf, const Source& Sf, int it, bool record, hipStream_t s,
+                    long save = -1);
   int set_model(const float* l, const float* m, const float* rho, int on_host, hipStream_t s);
   int zero_gradient(hipStream_t s);
   int prepare_shot(const efwi_shot* sh, int calc_id);

@@ -38,12 +38,27 @@ Sweep make_sweep(int z0, int z1, int x0, int x1, int y0, int y1, Tile t);
 // stencil kernels (efwi_stencil.hip). F.w is the wavefield a forward kernel
 // advances (the forward state or its replay copy); the backward kernels
 // reverse F.w and advance the adjoint F.a.
-void launch_stress_fwd(Tile t, const Geom& G, const Groups& F, const Prof& C, const Source& S, int it,
-                       hipStream_t s);
-void launch_velocity_fwd(Tile t, int form, const Geom& G, const Groups& F, const Prof& C, hipStream_t s);
-void launch_velocity_bwd(Tile t, int form, const Geom& G, const Groups& F, const Prof& C, hipStream_t s);
-void launch_stress_bwd(Tile t, const Geom& G, const Groups& F, const Prof& C, const Source& S, int it,
-                       hipStream_t s);
+//
+// A RingFold with a slot folds the boundary ring into these kernels: the
+// forward kernels also write the new F.w values of the band cells to the slot
+// (the save of the state they produce), the backward kernels leave the slot's
+// values in the band cells of the F.w fields they update (the restore that
+// follows them). Only the flat sweep folds, and only when the band lies inside
+// its cell box; the launchers return whether they did, and otherwise the
+// caller moves the ring with launch_ring.
+struct RingFold {
+  Band B;
+  float* slot;  // ring + slot index * 9 * B.n, or null: no ring traffic
+};
+bool ring_foldable(const Geom& G, const Band& B);
+bool launch_stress_fwd(Tile t, const Geom& G, const Groups& F, const Prof& C, const Source& S, int it,
+                       hipStream_t s, const RingFold* ring = nullptr);
+bool launch_velocity_fwd(Tile t, int form, const Geom& G, const Groups& F, const Prof& C, hipStream_t s,
+                         const RingFold* ring = nullptr);
+bool launch_velocity_bwd(Tile t, int form, const Geom& G, const Groups& F, const Prof& C, hipStream_t s,
+                         const RingFold* ring = nullptr);
+bool launch_stress_bwd(Tile t, const Geom& G, const Groups& F, const Prof& C, const Source& S, int it,
+                       hipStream_t s, const RingFold* ring = nullptr);
 // adjoint-only steps (no reverse reconstruction): the first backward step
 void launch_velocity_adj(int form, const Geom& G, const Groups& F, const Prof& C, hipStream_t s);
 void launch_stress_adj(const Geom& G, const Groups& F, const Prof& C, hipStream_t s);

@@ -624,24 +624,93 @@ __device__ __forceinline__ FlatAcc flat_acc(const Geom& G, const Groups& F, cons
                  group(F.m, M_COUNT, G.fs), c.o, {c.zn, c.xn, c.yn}};
 }
 
-__global__ __launch_bounds__(256) EFWI_FWD_ATTR void k_stress_fwd(Sweep Sw, Geom G, Groups F, Prof C, Source S, int it) {
+// Index of cell (z, x, y) in one field of a ring slot, or -1 outside the band:
+// the inverse of the band's enumeration (z slabs, then x slabs, then y slabs,
+// each row-major in (z, x, y); Src/utilities.cu:323-420 store the same cells).
+__device__ __forceinline__ int band_index(const Band& B, int z, int x, int y) {
+  if (z < B.lo[0] || z > B.hi[0] || x < B.lo[1] || x > B.hi[1] || y < B.lo[2] || y > B.hi[2]) return -1;
+  const int nxB = B.hi[1] - B.lo[1] + 1, nyB = B.hi[2] - B.lo[2] + 1;
+  const int nzM = B.hi[0] - B.lo[0] - 9, nxM = nxB - 10;
+  if (z <= B.lo[0] + 4 || z >= B.hi[0] - 4) {
+    const int zl = z >= B.hi[0] - 4 ? 5 + z - (B.hi[0] - 4) : z - B.lo[0];
+    return (zl * nxB + (x - B.lo[1])) * nyB + (y - B.lo[2]);
+  }
+  const int zm = z - B.lo[0] - 5;
+  if (x <= B.lo[1] + 4 || x >= B.hi[1] - 4) {
+    const int sl = x >= B.hi[1] - 4 ? 1 : 0;
+    const int xl = sl ? x - (B.hi[1] - 4) : x - B.lo[1];
+    return (int)B.nzs + ((sl * nzM + zm) * 5 + xl) * nyB + (y - B.lo[2]);
+  }
+  if (y <= B.lo[2] + 4 || y >= B.hi[2] - 4) {
+    const int sl = y >= B.hi[2] - 4 ? 1 : 0;
+    const int yl = sl ? y - (B.hi[2] - 4) : y - B.lo[2];
+    return (int)(B.nzs + B.nxs) + ((sl * nzM + zm) * nxM + (x - B.lo[1] - 5)) * 5 + yl;
+  }
+  return -1;
+}
+__device__ __forceinline__ int ring_index(const RingFold& R, const Cell& c) {
+  return R.slot ? band_index(R.B, c.z, c.x, c.y) : -1;
+}
+
+// Accessor of the forward kernels: a band cell's new wavefield values also go
+// to the ring slot (rk: the cell's index in a slot's field, or -1), so the
+// boundary save needs no pass of its own. The ring's field order is the
+// WaveField order.
+struct SaveAcc : FlatAcc {
+  float* rs;
+  long rn;
+  int rk;
+  __device__ __forceinline__ void st(int gi, int k, float v) const {
+    FlatAcc::st(gi, k, v);
+    if (gi == GW && rk >= 0) rs[(long)k * rn + rk] = v;
+  }
+};
+__device__ __forceinline__ SaveAcc save_acc(const Geom& G, const Groups& F, const RingFold& R, const Cell& c) {
+  return SaveAcc{flat_acc(G, F, c), R.slot, R.B.n, ring_index(R, c)};
+}
+
+// Restore folded into a backward kernel: a band cell loads the slot's values
+// of wavefield fields K0..K1-1 before its update (so the loads overlap the
+// cell's own) and stores them after it, as the reverse reconstruction's
+// restore after that kernel would leave them.
+template <int K0, int K1>
+struct RingRestore {
+  int rk;
+  float v[K1 - K0];
+  __device__ __forceinline__ RingRestore(const RingFold& R, const Cell& c) : rk(ring_index(R, c)) {
+    if (rk >= 0) {
+#pragma unroll
+      for (int k = K0; k < K1; k++) v[k - K0] = R.slot[(long)k * R.B.n + rk];
+    }
+  }
+  __device__ __forceinline__ void store(const FlatAcc& acc) const {
+    if (rk >= 0) {
+#pragma unroll
+      for (int k = K0; k < K1; k++) acc.st(GW, k, v[k - K0]);
+    }
+  }
+};
+
+__global__ __launch_bounds__(256) EFWI_FWD_ATTR void k_stress_fwd(Sweep Sw, Geom G, Groups F, Prof C, Source S, int it,
+                                                                  RingFold Rf) {
   const unsigned so = source_off(S);
   const float amp = source_amp(S, G, it);
-  sweep(Sw, G, [&](const Cell& c) { stress_fwd_cell(G, flat_acc(G, F, c), C, c, so, amp); });
+  sweep(Sw, G, [&](const Cell& c) { stress_fwd_cell(G, save_acc(G, F, Rf, c), C, c, so, amp); });
 }
 
 template <int FORM>
-__global__ __launch_bounds__(256) EFWI_FWD_ATTR void k_velocity_fwd(Sweep Sw, Geom G, Groups F, Prof C) {
-  sweep(Sw, G, [&](const Cell& c) { velocity_fwd_cell<FORM>(G, flat_acc(G, F, c), C, c); });
+__global__ __launch_bounds__(256) EFWI_FWD_ATTR void k_velocity_fwd(Sweep Sw, Geom G, Groups F, Prof C, RingFold Rf) {
+  sweep(Sw, G, [&](const Cell& c) { velocity_fwd_cell<FORM>(G, save_acc(G, F, Rf, c), C, c); });
 }
 
 // Backward velocity step (RV = true): reverse velocity + density imaging
 // (interior), then the adjoint velocity ([2, n-3]); RV = false: adjoint only.
 template <int FORM, bool RV>
-__global__ __launch_bounds__(256) EFWI_BWD_ATTR void k_velocity_bwd(Sweep Sw, Geom G, Groups F, Prof C) {
+__global__ __launch_bounds__(256) EFWI_BWD_ATTR void k_velocity_bwd(Sweep Sw, Geom G, Groups F, Prof C, RingFold Rf) {
   sweep(Sw, G, [&](const Cell& c) {
     const FlatAcc acc = flat_acc(G, F, c);
     const Flags f = flags_of(G, c);
+    const RingRestore<F_VZ, F_VY + 1> rr(RV ? Rf : RingFold{}, c);
     if (f.xz) {
       if (f.y) velocity_bwd_cell<FORM, RV, true, true>(G, acc, C, c);
       else velocity_bwd_cell<FORM, RV, true, false>(G, acc, C, c);
@@ -649,6 +718,7 @@ __global__ __launch_bounds__(256) EFWI_BWD_ATTR void k_velocity_bwd(Sweep Sw, Ge
       if (f.y) velocity_bwd_cell<FORM, RV, false, true>(G, acc, C, c);
       else velocity_bwd_cell<FORM, RV, false, false>(G, acc, C, c);
     }
+    if (RV) rr.store(acc);
   });
 }
 
@@ -656,12 +726,14 @@ __global__ __launch_bounds__(256) EFWI_BWD_ATTR void k_velocity_bwd(Sweep Sw, Ge
 // lambda/mu imaging (interior), then the adjoint stress ([2, n-3]);
 // RV = false: adjoint only.
 template <bool RV>
-__global__ __launch_bounds__(256) EFWI_BWD_ATTR void k_stress_bwd(Sweep Sw, Geom G, Groups F, Prof C, Source S, int it) {
+__global__ __launch_bounds__(256) EFWI_BWD_ATTR void k_stress_bwd(Sweep Sw, Geom G, Groups F, Prof C, Source S, int it,
+                                                                  RingFold Rf) {
   const unsigned so = RV ? source_off(S) : 0xFFFFFFFFu;
   const float amp = RV ? source_amp(S, G, it) : 0.0f;
   sweep(Sw, G, [&](const Cell& c) {
     const FlatAcc acc = flat_acc(G, F, c);
     const Flags f = flags_of(G, c);
+    const RingRestore<F_SZZ, F_WAVE_COUNT> rr(RV ? Rf : RingFold{}, c);
     if (f.xz) {
       if (f.y) stress_bwd_cell<RV, true, true>(G, acc, C, c, so, amp);
       else stress_bwd_cell<RV, true, false>(G, acc, C, c, so, amp);
@@ -669,6 +741,7 @@ __global__ __launch_bounds__(256) EFWI_BWD_ATTR void k_stress_bwd(Sweep Sw, Geom
       if (f.y) stress_bwd_cell<RV, false, true>(G, acc, C, c, so, amp);
       else stress_bwd_cell<RV, false, false>(G, acc, C, c, so, amp);
     }
+    if (RV) rr.store(acc);
   });
 }
 
@@ -1071,16 +1144,35 @@ static inline Sweep sweep_march(const Geom& G, Tile t) {
     else hipLaunchKernelGGL((KERNEL<4>), grid_of(Sw), block_of(Sw), 0, s, __VA_ARGS__);          \
   } while (0)
 
-void launch_stress_fwd(Tile t, const Geom& G, const Groups& F, const Prof& C, const Source& S, int it,
-                       hipStream_t s) {
+// The band inside the flat sweep's box [2, n-3] and at least 10 cells wide on
+// every axis (so its slab groups are as band_index enumerates them).
+bool ring_foldable(const Geom& G, const Band& B) {
+  const int n[3] = {G.nz, G.nx, G.ny};
+  for (int ax = 0; ax < 3; ax++)
+    if (B.lo[ax] < 2 || B.hi[ax] > n[ax] - 3 || B.hi[ax] - B.lo[ax] + 1 < 10) return false;
+  return true;
+}
+// The fold a flat launch takes: the caller's, or none.
+static inline RingFold fold_for(const Geom& G, const RingFold* r) {
+  if (r && r->slot && ring_foldable(G, r->B)) return *r;
+  RingFold none{};
+  none.slot = nullptr;
+  return none;
+}
+
+bool launch_stress_fwd(Tile t, const Geom& G, const Groups& F, const Prof& C, const Source& S, int it,
+                       hipStream_t s, const RingFold* ring) {
   if (marching(t)) {
     EFWI_MARCH(k_stress_fwd_m, Sw, G, F, C, S, it);
-    return;
+    return false;
   }
   const Sweep Sw = sweep_full(G, t);
-  hipLaunchKernelGGL(k_stress_fwd, grid_of(Sw), block_of(Sw), 0, s, Sw, G, F, C, S, it);
+  const RingFold Rf = fold_for(G, ring);
+  hipLaunchKernelGGL(k_stress_fwd, grid_of(Sw), block_of(Sw), 0, s, Sw, G, F, C, S, it, Rf);
+  return Rf.slot != nullptr;
 }
-void launch_velocity_fwd(Tile t, int form, const Geom& G, const Groups& F, const Prof& C, hipStream_t s) {
+bool launch_velocity_fwd(Tile t, int form, const Geom& G, const Groups& F, const Prof& C, hipStream_t s,
+                         const RingFold* ring) {
   if (marching(t)) {
     const Sweep Sw = sweep_march(G, t);
     if (form == 1) {
@@ -1090,13 +1182,16 @@ void launch_velocity_fwd(Tile t, int form, const Geom& G, const Groups& F, const
       if (Sw.R == 8) hipLaunchKernelGGL((k_velocity_fwd_m<0, 8>), grid_of(Sw), block_of(Sw), 0, s, Sw, G, F, C);
       else hipLaunchKernelGGL((k_velocity_fwd_m<0, 4>), grid_of(Sw), block_of(Sw), 0, s, Sw, G, F, C);
     }
-    return;
+    return false;
   }
   const Sweep Sw = sweep_full(G, t);
-  if (form == 1) hipLaunchKernelGGL(k_velocity_fwd<1>, grid_of(Sw), block_of(Sw), 0, s, Sw, G, F, C);
-  else hipLaunchKernelGGL(k_velocity_fwd<0>, grid_of(Sw), block_of(Sw), 0, s, Sw, G, F, C);
+  const RingFold Rf = fold_for(G, ring);
+  if (form == 1) hipLaunchKernelGGL(k_velocity_fwd<1>, grid_of(Sw), block_of(Sw), 0, s, Sw, G, F, C, Rf);
+  else hipLaunchKernelGGL(k_velocity_fwd<0>, grid_of(Sw), block_of(Sw), 0, s, Sw, G, F, C, Rf);
+  return Rf.slot != nullptr;
 }
-void launch_velocity_bwd(Tile t, int form, const Geom& G, const Groups& F, const Prof& C, hipStream_t s) {
+bool launch_velocity_bwd(Tile t, int form, const Geom& G, const Groups& F, const Prof& C, hipStream_t s,
+                         const RingFold* ring) {
   if (marching(t)) {
     const Sweep Sw = sweep_march(G, t);
     if (form == 1) {
@@ -1106,32 +1201,37 @@ void launch_velocity_bwd(Tile t, int form, const Geom& G, const Groups& F, const
       if (Sw.R == 8) hipLaunchKernelGGL((k_velocity_bwd_m<0, 8>), grid_of(Sw), block_of(Sw), 0, s, Sw, G, F, C);
       else hipLaunchKernelGGL((k_velocity_bwd_m<0, 4>), grid_of(Sw), block_of(Sw), 0, s, Sw, G, F, C);
     }
-    return;
+    return false;
   }
   const Sweep Sw = sweep_full(G, t);
-  if (form == 1) hipLaunchKernelGGL((k_velocity_bwd<1, true>), grid_of(Sw), block_of(Sw), 0, s, Sw, G, F, C);
-  else hipLaunchKernelGGL((k_velocity_bwd<0, true>), grid_of(Sw), block_of(Sw), 0, s, Sw, G, F, C);
+  const RingFold Rf = fold_for(G, ring);
+  if (form == 1) hipLaunchKernelGGL((k_velocity_bwd<1, true>), grid_of(Sw), block_of(Sw), 0, s, Sw, G, F, C, Rf);
+  else hipLaunchKernelGGL((k_velocity_bwd<0, true>), grid_of(Sw), block_of(Sw), 0, s, Sw, G, F, C, Rf);
+  return Rf.slot != nullptr;
 }
-void launch_stress_bwd(Tile t, const Geom& G, const Groups& F, const Prof& C, const Source& S, int it,
-                       hipStream_t s) {
+bool launch_stress_bwd(Tile t, const Geom& G, const Groups& F, const Prof& C, const Source& S, int it,
+                       hipStream_t s, const RingFold* ring) {
   if (marching(t)) {
     EFWI_MARCH(k_stress_bwd_m, Sw, G, F, C, S, it);
-    return;
+    return false;
   }
   const Sweep Sw = sweep_full(G, t);
-  hipLaunchKernelGGL(k_stress_bwd<true>, grid_of(Sw), block_of(Sw), 0, s, Sw, G, F, C, S, it);
+  const RingFold Rf = fold_for(G, ring);
+  hipLaunchKernelGGL(k_stress_bwd<true>, grid_of(Sw), block_of(Sw), 0, s, Sw, G, F, C, S, it, Rf);
+  return Rf.slot != nullptr;
 }
 void launch_velocity_adj(int form, const Geom& G, const Groups& F, const Prof& C, hipStream_t s) {
   const Sweep Sw = sweep_full(G, Tile{1, 1});
-  if (form == 1) hipLaunchKernelGGL((k_velocity_bwd<1, false>), grid_of(Sw), block_of(Sw), 0, s, Sw, G, F, C);
-  else hipLaunchKernelGGL((k_velocity_bwd<0, false>), grid_of(Sw), block_of(Sw), 0, s, Sw, G, F, C);
+  const RingFold none = fold_for(G, nullptr);
+  if (form == 1) hipLaunchKernelGGL((k_velocity_bwd<1, false>), grid_of(Sw), block_of(Sw), 0, s, Sw, G, F, C, none);
+  else hipLaunchKernelGGL((k_velocity_bwd<0, false>), grid_of(Sw), block_of(Sw), 0, s, Sw, G, F, C, none);
 }
 void launch_stress_adj(const Geom& G, const Groups& F, const Prof& C, hipStream_t s) {
   const Sweep Sw = sweep_full(G, Tile{1, 1});
   Source none;
   none.cell = -1;
   none.stf = nullptr;
-  hipLaunchKernelGGL(k_stress_bwd<false>, grid_of(Sw), block_of(Sw), 0, s, Sw, G, F, C, none, 0);
+  hipLaunchKernelGGL(k_stress_bwd<false>, grid_of(Sw), block_of(Sw), 0, s, Sw, G, F, C, none, 0, fold_for(G, nullptr));
 }
 
 // Batched launches (flat tiles only; a zdepth >= 16 shape falls back to its
