@@ -148,13 +148,22 @@ int Ctx::run_batch(const efwi_shot* shots, int n, int calc_id, const float* cons
   const std::string key = std::to_string(n) + "|" + std::to_string(max_nrec) + "|" + tiles_key + "|" +
                           std::to_string(batch.take) + "|" + std::to_string(batch.static_pct) + "|";
   // ---- forward loop (Src/libCUFD.cu:312-354 per shot): ring save of the
-  // state before step it, stress + source, velocity, record of step it + 1
+  // state before step it, stress + source, velocity, record of step it + 1.
+  // With the ring folded, the stencil kernels of step it save the state they
+  // produce (slot it + 1) and slot 0 is saved once before the loop.
+  const bool fold = grad && ring_foldable(G, B);
+  BatchRing save{};
+  save.B = B;
+  save.mode = 1;
+  save.last = nt - 2;
+  BatchRing restore = save;
+  restore.mode = 2;
   auto fwd_step = [&](hipStream_t st) {
-    if (grad) launch_ring_batch(B, G, batch.shots_dev, n, batch.it_dev, 0, 9, 0, st);
+    if (grad && !fold) launch_ring_batch(B, G, batch.shots_dev, n, batch.it_dev, 0, 9, 0, st);
     launch_fill(reinterpret_cast<float*>(batch.counters), 8 * kQueueStride, 0.0f, st);
-    launch_stress_fwd_batch(tiles[K_STRESS_FWD], G, C, L0, st);
+    launch_stress_fwd_batch(tiles[K_STRESS_FWD], G, C, L0, st, fold ? &save : nullptr);
     launch_fill(reinterpret_cast<float*>(batch.counters + 8 * kQueueStride), 8 * kQueueStride, 0.0f, st);
-    launch_velocity_fwd_batch(tiles[K_VELOCITY_FWD], form, G, C, L1, st);
+    launch_velocity_fwd_batch(tiles[K_VELOCITY_FWD], form, G, C, L1, st, fold ? &save : nullptr);
     launch_record_batch(batch.shots_dev, n, max_nrec, batch.it_dev, 1, st);
     launch_step_add(batch.it_dev, 1, st);
   };
@@ -181,6 +190,7 @@ int Ctx::run_batch(const efwi_shot* shots, int n, int calc_id, const float* cons
   };
   const int zero = 0;
   HIP_TRY(hipMemcpyAsync(batch.it_dev, &zero, sizeof(int), hipMemcpyHostToDevice, q));
+  if (fold) launch_ring_batch(B, G, batch.shots_dev, n, batch.it_dev, 0, 9, 0, q);
   if ((rc = run_steps(&batch.gf, &batch.key_f, key + (grad ? "g" : "f"), nt - 1, fwd_step))) return rc;
   HIP_TRY(hipGetLastError());
   for (int k = 0; k < n; k++)
@@ -214,12 +224,12 @@ int Ctx::run_batch(const efwi_shot* shots, int n, int calc_id, const float* cons
     }
     auto bwd_step = [&](hipStream_t st) {
       launch_fill(reinterpret_cast<float*>(batch.counters), 8 * kQueueStride, 0.0f, st);
-      launch_velocity_bwd_batch(tiles[K_VELOCITY_BWD], form, G, C, L0, st);
-      launch_ring_batch(B, G, batch.shots_dev, n, batch.it_dev, 0, 3, 1, st);
+      launch_velocity_bwd_batch(tiles[K_VELOCITY_BWD], form, G, C, L0, st, fold ? &restore : nullptr);
+      if (!fold) launch_ring_batch(B, G, batch.shots_dev, n, batch.it_dev, 0, 3, 1, st);
       launch_inject_batch(batch.shots_dev, n, max_nrec, batch.it_dev, st);
       launch_fill(reinterpret_cast<float*>(batch.counters + 8 * kQueueStride), 8 * kQueueStride, 0.0f, st);
-      launch_stress_bwd_batch(tiles[K_STRESS_BWD], G, C, L1, st);
-      launch_ring_batch(B, G, batch.shots_dev, n, batch.it_dev, 3, 9, 1, st);
+      launch_stress_bwd_batch(tiles[K_STRESS_BWD], G, C, L1, st, fold ? &restore : nullptr);
+      if (!fold) launch_ring_batch(B, G, batch.shots_dev, n, batch.it_dev, 3, 9, 1, st);
       launch_step_add(batch.it_dev, -1, st);
     };
     const int start = nt - 2;

@@ -64,11 +64,25 @@ void launch_velocity_adj(int form, const Geom& G, const Groups& F, const Prof& C
 void launch_stress_adj(const Geom& G, const Groups& F, const Prof& C, hipStream_t s);
 
 // batched persistent stencil kernels: one launch covers the (shot, tile) items
-// of all shots of L, with the flat sweep's tiles (t.zdepth < 16)
-void launch_stress_fwd_batch(Tile t, const Geom& G, const Prof& C, const BatchLaunch& L, hipStream_t s);
-void launch_velocity_fwd_batch(Tile t, int form, const Geom& G, const Prof& C, const BatchLaunch& L, hipStream_t s);
-void launch_velocity_bwd_batch(Tile t, int form, const Geom& G, const Prof& C, const BatchLaunch& L, hipStream_t s);
-void launch_stress_bwd_batch(Tile t, const Geom& G, const Prof& C, const BatchLaunch& L, hipStream_t s);
+// of all shots of L, with the flat sweep's tiles (t.zdepth < 16).
+// A BatchRing folds each shot's boundary ring (ShotDev::ring) into them, the
+// slot following the launch's device step it: mode 1 (forward kernels) saves
+// the state they produce into slot it + 1 while it + 1 <= last, mode 2
+// (backward kernels) restores slot it. Callers pass one only where
+// ring_foldable holds.
+struct BatchRing {
+  Band B;
+  int mode;  // 0: no ring traffic
+  int last;
+};
+void launch_stress_fwd_batch(Tile t, const Geom& G, const Prof& C, const BatchLaunch& L, hipStream_t s,
+                             const BatchRing* ring = nullptr);
+void launch_velocity_fwd_batch(Tile t, int form, const Geom& G, const Prof& C, const BatchLaunch& L, hipStream_t s,
+                               const BatchRing* ring = nullptr);
+void launch_velocity_bwd_batch(Tile t, int form, const Geom& G, const Prof& C, const BatchLaunch& L, hipStream_t s,
+                               const BatchRing* ring = nullptr);
+void launch_stress_bwd_batch(Tile t, const Geom& G, const Prof& C, const BatchLaunch& L, hipStream_t s,
+                             const BatchRing* ring = nullptr);
 // workgroups a persistent launch keeps resident per CU
 int device_cus(int device);
 

@@ -1044,11 +1044,16 @@ __device__ __forceinline__ long uniform_long(long x) {
 
 // KIND: 0 stress_fwd, 1 velocity_fwd, 2 velocity_bwd, 3 stress_bwd
 template <int KIND, int FORM>
-__global__ __launch_bounds__(256) void k_batch(Sweep Sw, Geom G, Prof C, BatchLaunch L, int per_xcd) {
+__global__ __launch_bounds__(256) void k_batch(Sweep Sw, Geom G, Prof C, BatchLaunch L, int per_xcd, BatchRing R) {
   const int it = device_step(L.it);
+  // ring slot of this step (the same index in every shot's ring), or none
+  const int slot = R.mode == 1 ? it + 1 : it;
+  const bool ring_on = R.mode == 2 || (R.mode == 1 && slot <= R.last);
   int have = -1;
   Groups F{};
   Source S{};
+  RingFold Rf{};
+  Rf.B = R.B;
   persistent_items(L, per_xcd, [&](int shot, int b) {
     if (shot != have) {  // consecutive items mostly belong to one shot
       const ShotDev& sd = L.shots[shot];
@@ -1058,18 +1063,20 @@ __global__ __launch_bounds__(256) void k_batch(Sweep Sw, Geom G, Prof C, BatchLa
       F.m = uniform_ptr(sd.F.m);
       S.cell = uniform_long(sd.cell);
       S.stf = uniform_ptr(sd.stf);
+      Rf.slot = ring_on ? uniform_ptr(sd.ring) + (long)slot * 9 * R.B.n : nullptr;
       have = shot;
     }
     if constexpr (KIND == 0) {
       const unsigned so = source_off(S);
       const float amp = source_amp(S, G, it);
-      sweep_block(Sw, G, b, [&](const Cell& c) { stress_fwd_cell(G, flat_acc(G, F, c), C, c, so, amp); });
+      sweep_block(Sw, G, b, [&](const Cell& c) { stress_fwd_cell(G, save_acc(G, F, Rf, c), C, c, so, amp); });
     } else if constexpr (KIND == 1) {
-      sweep_block(Sw, G, b, [&](const Cell& c) { velocity_fwd_cell<FORM>(G, flat_acc(G, F, c), C, c); });
+      sweep_block(Sw, G, b, [&](const Cell& c) { velocity_fwd_cell<FORM>(G, save_acc(G, F, Rf, c), C, c); });
     } else if constexpr (KIND == 2) {
       sweep_block(Sw, G, b, [&](const Cell& c) {
         const FlatAcc acc = flat_acc(G, F, c);
         const Flags f = flags_of(G, c);
+        const RingRestore<F_VZ, F_VY + 1> rr(Rf, c);
         if (f.xz) {
           if (f.y) velocity_bwd_cell<FORM, true, true, true>(G, acc, C, c);
           else velocity_bwd_cell<FORM, true, true, false>(G, acc, C, c);
@@ -1077,6 +1084,7 @@ __global__ __launch_bounds__(256) void k_batch(Sweep Sw, Geom G, Prof C, BatchLa
           if (f.y) velocity_bwd_cell<FORM, true, false, true>(G, acc, C, c);
           else velocity_bwd_cell<FORM, true, false, false>(G, acc, C, c);
         }
+        rr.store(acc);
       });
     } else {
       const unsigned so = source_off(S);
@@ -1084,6 +1092,7 @@ __global__ __launch_bounds__(256) void k_batch(Sweep Sw, Geom G, Prof C, BatchLa
       sweep_block(Sw, G, b, [&](const Cell& c) {
         const FlatAcc acc = flat_acc(G, F, c);
         const Flags f = flags_of(G, c);
+        const RingRestore<F_SZZ, F_WAVE_COUNT> rr(Rf, c);
         if (f.xz) {
           if (f.y) stress_bwd_cell<true, true, true>(G, acc, C, c, so, amp);
           else stress_bwd_cell<true, true, false>(G, acc, C, c, so, amp);
@@ -1091,6 +1100,7 @@ __global__ __launch_bounds__(256) void k_batch(Sweep Sw, Geom G, Prof C, BatchLa
           if (f.y) stress_bwd_cell<true, false, true>(G, acc, C, c, so, amp);
           else stress_bwd_cell<true, false, false>(G, acc, C, c, so, amp);
         }
+        rr.store(acc);
       });
     }
   });
@@ -1279,20 +1289,35 @@ static inline dim3 batch_plan(const Sweep& Sw, BatchLaunch& L, int per_cu, int s
     const int per_cu = resident_per_cu((KERNEL), (int)(blk.x * blk.y * blk.z), occ_cache);    \
     BatchLaunch Lk = L;                                                                       \
     const dim3 grid = batch_plan(Sw, Lk, per_cu, L.stat);                                             \
-    hipLaunchKernelGGL((KERNEL), grid, blk, 0, s, Sw, G, C, Lk, per_xcd);                     \
+    hipLaunchKernelGGL((KERNEL), grid, blk, 0, s, Sw, G, C, Lk, per_xcd, R);                  \
   } while (0)
-void launch_stress_fwd_batch(Tile t, const Geom& G, const Prof& C, const BatchLaunch& L, hipStream_t s) {
+// The caller's ring fold where it applies, else none.
+static inline BatchRing batch_ring(const Geom& G, const BatchRing* ring) {
+  if (ring && ring->mode && ring_foldable(G, ring->B)) return *ring;
+  BatchRing none{};
+  none.mode = 0;
+  return none;
+}
+void launch_stress_fwd_batch(Tile t, const Geom& G, const Prof& C, const BatchLaunch& L, hipStream_t s,
+                             const BatchRing* ring) {
+  const BatchRing R = batch_ring(G, ring);
   EFWI_BATCH((k_batch<0, 0>));
 }
-void launch_velocity_fwd_batch(Tile t, int form, const Geom& G, const Prof& C, const BatchLaunch& L, hipStream_t s) {
+void launch_velocity_fwd_batch(Tile t, int form, const Geom& G, const Prof& C, const BatchLaunch& L, hipStream_t s,
+                               const BatchRing* ring) {
+  const BatchRing R = batch_ring(G, ring);
   if (form == 1) EFWI_BATCH((k_batch<1, 1>));
   else EFWI_BATCH((k_batch<1, 0>));
 }
-void launch_velocity_bwd_batch(Tile t, int form, const Geom& G, const Prof& C, const BatchLaunch& L, hipStream_t s) {
+void launch_velocity_bwd_batch(Tile t, int form, const Geom& G, const Prof& C, const BatchLaunch& L, hipStream_t s,
+                               const BatchRing* ring) {
+  const BatchRing R = batch_ring(G, ring);
   if (form == 1) EFWI_BATCH((k_batch<2, 1>));
   else EFWI_BATCH((k_batch<2, 0>));
 }
-void launch_stress_bwd_batch(Tile t, const Geom& G, const Prof& C, const BatchLaunch& L, hipStream_t s) {
+void launch_stress_bwd_batch(Tile t, const Geom& G, const Prof& C, const BatchLaunch& L, hipStream_t s,
+                             const BatchRing* ring) {
+  const BatchRing R = batch_ring(G, ring);
   EFWI_BATCH((k_batch<3, 0>));
 }
 
