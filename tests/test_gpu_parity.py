@@ -102,6 +102,27 @@ def test_long_y_axis_is_bit_identical(orc, rows, zdepth):
         assert np.array_equal(a, b), name
 
 
+@pytest.mark.parametrize("n,npml,ny", [(24, 3, None), (26, 6, 71)])
+def test_thin_pml_and_ragged_rows_are_bit_identical(orc, n, npml, ny):
+    """A 3-cell C-PML (the boundary band reaches the outer two cells, outside
+    the stencil kernels' box, so every ring save and restore is a pass of its
+    own) and a y extent of 83 padded cells (rows padded to 96, the last
+    64-cell segment partly past the row's end, the band's far y slab in it).
+    Seismograms, residuals and gradients must equal the oracle's bit for bit."""
+    from efwi import synth
+    w = synth.small(n=n, npml=npml, nt=120, ny=ny)
+    ref_true = oracle_run(orc, w, model=w.true, calc_id=2)
+    obs = [s.astype(np.float32) for s in ref_true.syn]
+    j, syn, res, grads = _gpu_run(w, w.model, 1, obs)
+    ref = oracle_run(orc, w, model=w.model, obs=obs, calc_id=1)
+    assert abs(j - ref.misfit64) <= 1e-12 * abs(ref.misfit64)
+    assert np.array_equal(syn[0], ref.syn[0].astype(np.float32))
+    assert np.array_equal(res[0], ref.res[0].astype(np.float32))
+    for name, a, b in zip(("lambda", "mu", "rho"), grads, (ref.glam, ref.gmu, ref.grho)):
+        assert np.abs(b).max() > 0
+        assert np.array_equal(a, b.astype(np.float32)), name
+
+
 @pytest.mark.parametrize("src_z", [1, 5])
 def test_source_near_the_edge_is_bit_identical(orc, src_z):
     """Sources in the C-PML (padded z 5 of 6 C-PML cells: injected by the
