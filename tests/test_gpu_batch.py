@@ -61,6 +61,31 @@ def test_batch_equals_single_shots(three_shots, use_graph):
             assert np.array_equal(a, b), k
 
 
+def test_batch_thin_pml_equals_single_shots():
+    """A 3-cell C-PML: the boundary band leaves the stencil box, so the batched
+    ring saves and restores are passes of their own (not folded into the
+    batched kernels); still bit-identical to the shots run one by one."""
+    from efwi import synth
+    from efwi.propagator import Propagator, ShotSpec
+    w = synth.small(n=24, npml=3, nt=100)
+    base = w.shots[0]
+    rec = np.array([[5, x, y] for x in range(6, 24, 3) for y in range(6, 24, 4)], np.int32)
+    shots = [base, ShotSpec((9, 12, 15), base.stf, rec)]
+    obs = [o[1] for o in _single(w, shots, w.true, calc=2)]
+    ref = _single(w, shots, w.model, obs=obs)
+    with Propagator(w.grid) as P:
+        P.set_model(*(torch.from_numpy(a) for a in w.model))
+        mis, syns = P.run_batch(shots, 1, [torch.from_numpy(o) for o in obs], want_syn=True)
+        grads = [[x.cpu().numpy() for x in P.batch_gradient(k)] for k in range(len(shots))]
+        torch.cuda.synchronize()
+    for k, (j, syn, g) in enumerate(ref):
+        assert mis[k] == j, k
+        assert np.array_equal(syns[k].cpu().numpy(), syn), k
+        for a, b in zip(grads[k], g):
+            assert np.abs(b).max() > 0
+            assert np.array_equal(a, b), k
+
+
 def test_batch_forward_only(three_shots):
     from efwi.propagator import Propagator
     w, shots = three_shots
