@@ -227,6 +227,8 @@ def bench_learned(args):
     for m in range(args.eval_models):
         survey = L.surface_survey(*edims, npml, args.eval_nt, 12.5, 1e-3, 8.0, 8, rng)
         vp, vs, rho, vp0 = L.random_model(rng, *edims, sigma=8.0)
+        if m < args.eval_skip:  # drawn (the sequence stays the same) but not inverted
+            continue
         row = {}
         for method in ("plain", "learned"):
             prop = Propagator(survey.grid, 0)
@@ -258,8 +260,8 @@ def bench_learned(args):
         "data": "synthetic",
         "config": {"workload": f"cfg5: U-Net (3 levels, 16 base channels, 3x3x3, BF16 MFMA, {net.nparams()} "
                                f"parameters) trained on {args.train_models} random layered+anomaly models at "
-                               f"{tdims[0]}x{tdims[1]}x{tdims[2]} ({args.epochs} epochs); {args.eval_models} "
-                               f"held-out models at {edims[0]}x{edims[1]}x{edims[2]} + {npml} PML, 8 shots, "
+                               f"{tdims[0]}x{tdims[1]}x{tdims[2]} ({args.epochs} epochs); {args.eval_models - args.eval_skip} "
+                               f"held-out models (from model {args.eval_skip}) at {edims[0]}x{edims[1]}x{edims[2]} + {npml} PML, 8 shots, "
                                f"nt={args.eval_nt}, {args.iters} iterations each method",
                    "time_ratios": ratios,
                    "final_misfit_ratio_learned_over_plain": [r["learned"]["f"][-1] / r["plain"]["f"][-1] for r in results],
@@ -290,6 +292,8 @@ def main():
     ap.add_argument("--epochs", type=int, default=6, help="cfg5: training epochs")
     ap.add_argument("--eval-models", type=int, default=2, help="cfg5: held-out models")
     ap.add_argument("--eval-nt", type=int, default=800, help="cfg5: time steps of the held-out inversions")
+    ap.add_argument("--eval-skip", type=int, default=0,
+                    help="cfg5: draw but do not invert the first K held-out models (to split a long evaluation)")
     ap.add_argument("--iters", type=int, default=20, help="cfg5: inversion iterations")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-profile", action="store_true", help="skip the per-kernel event timing")
