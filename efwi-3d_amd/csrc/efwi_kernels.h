@@ -31,6 +31,8 @@ struct Tile {
 // the first rows_rem XCDs) x rows and walks it by chunks of Z planes.
 struct Sweep {
   int z0, nz, x0, rows_base, rows_rem, rg_max, y0, y1, yb0, nyb, R, Z, nzc;  // y segments yb0 .. yb0+nyb-1
+  int zrev;  // plane chunks last to first (consecutive kernels alternate, so that each starts where the
+             // previous one ended, on data the Infinity Cache still holds)
   long blocks;
 };
 Sweep make_sweep(int z0, int z1, int x0, int x1, int y0, int y1, Tile t);

@@ -207,8 +207,9 @@ template <typename Body>
 __device__ __forceinline__ void sweep_block(const Sweep& S, const Geom& G, int b, Body&& body) {
   const int xcd = b & 7, j = b >> 3;
   const int per_plane = S.rg_max * S.nyb;
-  const int zc = j / per_plane;
-  const int r = j - zc * per_plane;
+  const int zc0 = j / per_plane;
+  const int zc = S.zrev ? S.nzc - 1 - zc0 : zc0;
+  const int r = j - zc0 * per_plane;
   const int rg = r / S.nyb;
   const int yb = r - rg * S.nyb;
   const int rows = S.rows_base + (xcd < S.rows_rem ? 1 : 0);
@@ -1126,6 +1127,7 @@ Sweep make_sweep(int z0, int z1, int x0, int x1, int y0, int y1, Tile t) {
   S.yb0 = y0 / 64;  // segments of 64 aligned at y = 0, from the one holding y0
   S.nyb = y1 / 64 - S.yb0 + 1;
   S.nzc = (S.nz + S.Z - 1) / S.Z;
+  S.zrev = 0;
   S.blocks = 8L * S.rg_max * S.nyb * S.nzc;
   return S;
 }
@@ -1194,7 +1196,8 @@ bool launch_velocity_fwd(Tile t, int form, const Geom& G, const Groups& F, const
     }
     return false;
   }
-  const Sweep Sw = sweep_full(G, t);
+  Sweep Sw = sweep_full(G, t);
+  Sw.zrev = 1;  // the stress kernel before it ran first to last
   const RingFold Rf = fold_for(G, ring);
   if (form == 1) hipLaunchKernelGGL(k_velocity_fwd<1>, grid_of(Sw), block_of(Sw), 0, s, Sw, G, F, C, Rf);
   else hipLaunchKernelGGL(k_velocity_fwd<0>, grid_of(Sw), block_of(Sw), 0, s, Sw, G, F, C, Rf);
@@ -1225,7 +1228,8 @@ bool launch_stress_bwd(Tile t, const Geom& G, const Groups& F, const Prof& C, co
     EFWI_MARCH(k_stress_bwd_m, Sw, G, F, C, S, it);
     return false;
   }
-  const Sweep Sw = sweep_full(G, t);
+  Sweep Sw = sweep_full(G, t);
+  Sw.zrev = 1;  // the velocity kernel before it ran first to last
   const RingFold Rf = fold_for(G, ring);
   hipLaunchKernelGGL(k_stress_bwd<true>, grid_of(Sw), block_of(Sw), 0, s, Sw, G, F, C, S, it, Rf);
   return Rf.slot != nullptr;
@@ -1283,7 +1287,8 @@ static inline dim3 batch_plan(const Sweep& Sw, BatchLaunch& L, int per_cu, int s
 #define EFWI_BATCH(KERNEL)                                                                     \
   do {                                                                                        \
     static int occ_cache[8] = {0};                                                            \
-    const Sweep Sw = sweep_batch(G, t);                                                       \
+    Sweep Sw = sweep_batch(G, t);                                                             \
+    Sw.zrev = L.reverse; /* shots and planes last to first: start where the last launch ended */ \
     const int per_xcd = (int)(Sw.blocks / 8);                                                 \
     const dim3 blk = block_of(Sw);                                                            \
     const int per_cu = resident_per_cu((KERNEL), (int)(blk.x * blk.y * blk.z), occ_cache);    \
