@@ -829,8 +829,9 @@ __device__ __forceinline__ void march(const Sweep& Sw, const Geom& G, const Grou
   const int b = blockIdx.x;
   const int xcd = b & 7, j = b >> 3;
   const int per_plane = Sw.rg_max * Sw.nyb;
-  const int zc = j / per_plane;
-  const int r = j - zc * per_plane;
+  const int zc0 = j / per_plane;
+  const int zc = Sw.zrev ? Sw.nzc - 1 - zc0 : zc0;
+  const int r = j - zc0 * per_plane;
   const int rg = r / Sw.nyb;
   const int yb = r - rg * Sw.nyb;
   const int rows = Sw.rows_base + (xcd < Sw.rows_rem ? 1 : 0);
@@ -1149,9 +1150,10 @@ static inline Sweep sweep_march(const Geom& G, Tile t) {
   S.blocks = 8L * S.rg_max * S.nyb * S.nzc;
   return S;
 }
-#define EFWI_MARCH(KERNEL, ...)                                                                   \
+#define EFWI_MARCH(KERNEL, ZREV, ...)                                                             \
   do {                                                                                            \
-    const Sweep Sw = sweep_march(G, t);                                                           \
+    Sweep Sw = sweep_march(G, t);                                                                 \
+    Sw.zrev = (ZREV);                                                                             \
     if (Sw.R == 8) hipLaunchKernelGGL((KERNEL<8>), grid_of(Sw), block_of(Sw), 0, s, __VA_ARGS__); \
     else hipLaunchKernelGGL((KERNEL<4>), grid_of(Sw), block_of(Sw), 0, s, __VA_ARGS__);          \
   } while (0)
@@ -1175,7 +1177,7 @@ static inline RingFold fold_for(const Geom& G, const RingFold* r) {
 bool launch_stress_fwd(Tile t, const Geom& G, const Groups& F, const Prof& C, const Source& S, int it,
                        hipStream_t s, const RingFold* ring) {
   if (marching(t)) {
-    EFWI_MARCH(k_stress_fwd_m, Sw, G, F, C, S, it);
+    EFWI_MARCH(k_stress_fwd_m, 0, Sw, G, F, C, S, it);
     return false;
   }
   const Sweep Sw = sweep_full(G, t);
@@ -1186,7 +1188,8 @@ bool launch_stress_fwd(Tile t, const Geom& G, const Groups& F, const Prof& C, co
 bool launch_velocity_fwd(Tile t, int form, const Geom& G, const Groups& F, const Prof& C, hipStream_t s,
                          const RingFold* ring) {
   if (marching(t)) {
-    const Sweep Sw = sweep_march(G, t);
+    Sweep Sw = sweep_march(G, t);
+    Sw.zrev = 1;
     if (form == 1) {
       if (Sw.R == 8) hipLaunchKernelGGL((k_velocity_fwd_m<1, 8>), grid_of(Sw), block_of(Sw), 0, s, Sw, G, F, C);
       else hipLaunchKernelGGL((k_velocity_fwd_m<1, 4>), grid_of(Sw), block_of(Sw), 0, s, Sw, G, F, C);
@@ -1225,7 +1228,7 @@ bool launch_velocity_bwd(Tile t, int form, const Geom& G, const Groups& F, const
 bool launch_stress_bwd(Tile t, const Geom& G, const Groups& F, const Prof& C, const Source& S, int it,
                        hipStream_t s, const RingFold* ring) {
   if (marching(t)) {
-    EFWI_MARCH(k_stress_bwd_m, Sw, G, F, C, S, it);
+    EFWI_MARCH(k_stress_bwd_m, 1, Sw, G, F, C, S, it);
     return false;
   }
   Sweep Sw = sweep_full(G, t);
