@@ -72,6 +72,18 @@ def hbm_traffic(kernel, grid, workload):
     return 1024.0 * (k["fetch_kib"] * t["fetch_scale"] + k["write_kib"] * t["write_scale"])
 
 
+def cpu_model():
+    """The host CPU's model name (/proc/cpuinfo), for the baseline record."""
+    try:
+        with open("/proc/cpuinfo") as f:
+            for line in f:
+                if line.startswith("model name"):
+                    return line.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return "unknown"
+
+
 def cpu_baseline(w, nsteps):
     """The CPU oracle (fp32, OpenMP over all host threads) on the first
     `nsteps` time steps of the same workload (fwd + adjoint + gradient)."""
@@ -86,7 +98,7 @@ def cpu_baseline(w, nsteps):
     r = oracle.run(og, osh, *w.model, obs=obs, calc_id=1, precision="fp32")
     cores = int(os.environ.get("OMP_NUM_THREADS", "0")) or len(os.sched_getaffinity(0))
     gcell = g.ncells * nsteps / r.seconds / 1e9
-    return {"value": gcell, "unit": "Gcell-updates/s", "cores": cores, "kind": "port",
+    return {"value": gcell, "unit": "Gcell-updates/s", "cores": cores, "cpu_model": cpu_model(), "kind": "port",
             "sample": f"{w.name} grid {g.nz}x{g.nx}x{g.ny}, first {nsteps} of {g.nt} time steps, "
                       f"fwd+adj+grad, 1 shot, fp32 oracle, {r.seconds:.1f} s"}
 

@@ -41,6 +41,38 @@ __global__ void k_byc(Geom G, const float* __restrict__ rho, float* __restrict__
   }
 }
 
+// Largest squared P velocity (lambda + 2 mu) / rho over the grid, per block
+// (velInit, Src/utilities.cu:100-113, evaluates it in double on every cell;
+// compCourantNumber, :186-202, takes the maximum). Non-finite quotients and
+// the row padding are skipped; the per-block maxima go to `partials`.
+__global__ __launch_bounds__(256) void k_vp2max(Geom G, const float* __restrict__ lam, const float* __restrict__ mu,
+                                                const float* __restrict__ rho, double* __restrict__ partials) {
+  const long n = (long)G.nz * G.pxy;
+  double m = 0.0;
+  for (long i = (long)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (long)gridDim.x * blockDim.x) {
+    const int y = (int)(i % G.py);
+    if (y >= G.ny) continue;
+    const double v2 = ((double)lam[i] + 2.0 * (double)mu[i]) / (double)rho[i];
+    if (v2 > m && v2 < 1e300) m = v2;
+  }
+  for (int off = 32; off > 0; off >>= 1) m = fmax(m, __shfl_down(m, off, 64));
+  __shared__ double part[4];
+  const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
+  if (lane == 0) part[wid] = m;
+  __syncthreads();
+  if (threadIdx.x == 0) partials[blockIdx.x] = fmax(fmax(part[0], part[1]), fmax(part[2], part[3]));
+}
+__global__ __launch_bounds__(256) void k_max_partials(const double* __restrict__ p, int n, double* out) {
+  double m = 0.0;
+  for (int k = threadIdx.x; k < n; k += blockDim.x) m = fmax(m, p[k]);
+  for (int off = 32; off > 0; off >>= 1) m = fmax(m, __shfl_down(m, off, 64));
+  __shared__ double part[4];
+  const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
+  if (lane == 0) part[wid] = m;
+  __syncthreads();
+  if (threadIdx.x == 0) *out = fmax(fmax(part[0], part[1]), fmax(part[2], part[3]));
+}
+
 // Src/utilities.cu:442-479: +/- 1500^2 * stf[it] * dt at the source cell
 // (the 9^3 amplitude cube is exactly 1 at its centre and 0 elsewhere).
 __global__ void k_add_source(float* szz, float* sxx, float* syy, long cell, const float* stf, int it,
@@ -311,6 +343,13 @@ int launch_dot64(const float* a, const float* b, long n, double* partials, int m
   const int nb = blocks_for(n, 256, max_blocks);
   hipLaunchKernelGGL(k_dot64, dim3(nb), dim3(256), 0, s, a, b, n, partials);
   return nb;
+}
+
+void launch_vp2max(const Geom& G, const float* lam, const float* mu, const float* rho, double* partials,
+                   int max_blocks, double* out, hipStream_t s) {
+  const int nb = blocks_for((long)G.nz * G.pxy, 256, max_blocks);
+  hipLaunchKernelGGL(k_vp2max, dim3(nb), dim3(256), 0, s, G, lam, mu, rho, partials);
+  hipLaunchKernelGGL(k_max_partials, dim3(1), dim3(256), 0, s, partials, nb, out);
 }
 
 void launch_sum_partials(const double* partials, int n, double* out, hipStream_t s) {

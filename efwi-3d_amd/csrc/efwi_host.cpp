@@ -7,6 +7,7 @@
 // the reference driver (Src/libCUFD.cu:220-613) on one HIP stream.
 #include <hip/hip_runtime.h>
 
+#include <algorithm>
 #include <cmath>
 #include <cstdio>
 #include <cstring>
@@ -83,6 +84,19 @@ static float taper_w(float t, float t0, float t1, float t2, float t3) {
   if (t >= t1 && t < t2) return 1.0f;
   if (t >= t2 && t < t3) return (float)std::cos(PI / 2.0 * (t - t2) / (t3 - t2));
   return 0.0f;
+}
+
+// Courant number of the 3-D 4th-order staggered scheme, vmax dt sqrt(3)
+// (9/8 + 1/24) / min(dz, dx, dy): the scheme is unstable above 1. The
+// reference's guard (compCourantNumber, Src/utilities.cu:186-202, called at
+// Src/libCUFD.cu:91) uses the 2-D form sqrt(2) / min(dz, dx), which never
+// exceeds this one, so every model it refuses is refused here too.
+double courant_number(double vmax, double dt, double dz, double dx, double dy) {
+  const double hmin = std::min(dz, std::min(dx, dy));
+  return vmax * dt * std::sqrt(3.0) * (9.0 / 8.0 + 1.0 / 24.0) / hmin;
+}
+double courant_number_2d(double vmax, double dt, double dz, double dx) {
+  return vmax * dt * std::sqrt(2.0) * (1.0 / 24.0 + 9.0 / 8.0) / std::min(dz, dx);
 }
 
 float taper_plain_w2(int it, int nt, float dt, float ratio) {
@@ -459,6 +473,25 @@ int Ctx::set_model(const float* l, const float* m, const float* rho, int on_host
   // rho is staged in the row layout through the gradient buffer, then the
   // buffer is cleared again
   if ((rc = copy_in(Gr.grho, rho, k, s))) return rc;
+  model_set = false;
+  // stability guard (Src/libCUFD.cu:91): refuse a model whose Courant number exceeds 1
+  launch_vp2max(G, lam, mu, Gr.grho, partials, kMaxPartials, msum, s);
+  HIP_TRY(hipGetLastError());
+  double v2 = 0.0;
+  HIP_TRY(hipMemcpyAsync(&v2, msum, sizeof(double), hipMemcpyDeviceToHost, s));
+  HIP_TRY(hipStreamSynchronize(s));
+  const double vmax = std::sqrt(v2);
+  const double cn = courant_number(vmax, cfg.dt, cfg.dz, cfg.dx, cfg.dy);
+  if (!(cn <= 1.0)) {
+    launch_fill(Gr.grho, N, 0.0f, s);
+    char msg[256];
+    snprintf(msg, sizeof msg,
+             "unstable model: Courant number %.4f > 1 (vp max %.2f, dt %g, h min %g; the reference's 2-D "
+             "form gives %.4f)",
+             cn, vmax, (double)cfg.dt, (double)std::min(cfg.dz, std::min(cfg.dx, cfg.dy)),
+             courant_number_2d(vmax, cfg.dt, cfg.dz, cfg.dx));
+    return fail(EFWI_ERR_ARG, msg);
+  }
   launch_byc(G, Gr.grho, byc, s);
   HIP_TRY(hipGetLastError());
   launch_fill(Gr.grho, N, 0.0f, s);
@@ -892,6 +925,12 @@ long long efwi_device_bytes(const efwi_ctx* ctx) {
 int efwi_cpml_profile(int n, int npml, float h, float f0, float dt, float* out) {
   if (n <= 0 || !out) return fail(EFWI_ERR_ARG, "bad argument");
   efwi::cpml_profile_1d(n, npml, h, f0, dt, out, out + n, out + 2 * n, out + 3 * n, out + 4 * n, out + 5 * n);
+  return EFWI_OK;
+}
+
+int efwi_courant_number(double vp_max, float dt, float dz, float dx, float dy, double* out) {
+  if (!out || !(dt > 0) || !(dz > 0) || !(dx > 0) || !(dy > 0)) return fail(EFWI_ERR_ARG, "bad argument");
+  *out = efwi::courant_number(vp_max, dt, dz, dx, dy);
   return EFWI_OK;
 }
 

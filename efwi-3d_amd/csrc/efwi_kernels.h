@@ -106,6 +106,10 @@ void launch_ring(const Band& B, const Geom& G, const Fields9& F, float* ring, lo
 int launch_misfit(const float* obs_t, const float* syn_t, float* res_t, const float* fac,
                   int per_trace, int nt, int nrec, double* partials, int max_blocks, hipStream_t s);
 void launch_sum_partials(const double* partials, int n, double* out, hipStream_t s);
+// max over the grid of (lambda + 2 mu) / rho (row padding skipped), through
+// per-block partials (at most max_blocks), into *out (device)
+void launch_vp2max(const Geom& G, const float* lam, const float* mu, const float* rho, double* partials,
+                   int max_blocks, double* out, hipStream_t s);
 // FP64 dot product of float vectors; returns the number of partials written.
 int launch_dot64(const float* a, const float* b, long n, double* partials, int max_blocks, hipStream_t s);
 // batched per-step aux kernels of the shot table (time step read on the device)

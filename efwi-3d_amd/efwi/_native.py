@@ -15,8 +15,7 @@ import os
 import torch  # noqa: F401  (must precede loading libefwi.so)
 
 _HERE = os.path.dirname(os.path.abspath(__file__))
-# EFWI_LIBRARY selects another build of the same library (e.g. a tuning variant)
-LIB_PATH = os.environ.get("EFWI_LIBRARY") or os.path.join(_HERE, "libefwi.so")
+LIB_PATH = os.path.join(_HERE, "libefwi.so")
 
 OK = 0
 KERNELS = ("stress_fwd", "velocity_fwd", "stress_bwd", "velocity_bwd", "velocity_adj", "stress_adj",
@@ -69,6 +68,7 @@ SIGNATURES = {
     "efwi_profile_read": (C.c_int, [_vp, C.POINTER(C.c_double), C.POINTER(C.c_longlong), C.c_int]),
     "efwi_cpml_profile": (C.c_int, [C.c_int, C.c_int, C.c_float, C.c_float, C.c_float, _fp]),
     "efwi_taper_w2": (C.c_int, [C.c_int, C.c_float, C.c_float, _fp]),
+    "efwi_courant_number": (C.c_int, [C.c_double, C.c_float, C.c_float, C.c_float, C.c_float, _dp]),
     "efwi_cufd": (C.c_int, [_fp, _fp, _fp, _fp, _fp, _fp, _fp, _fp, C.c_int, C.c_int, C.c_int, _ip,
                             C.c_char_p]),
     "efwi_set_illumination": (C.c_int, [_vp, C.c_int, _vp]),

@@ -79,7 +79,11 @@ void efwi_destroy(efwi_ctx* ctx);
 /* Model in the op's units (lambda, mu in the Mu/Lambda units of FWI_3d, rho
  * in g/cc), [nz][nx][ny]. Copies lambda and mu and derives the buoyancy
  * (Src/Model.cu:15-80). on_host = 1 for host pointers. Replaces the model
- * half of cufd's set-up (Src/libCUFD.cu:67-88). */
+ * half of cufd's set-up (Src/libCUFD.cu:67-88). Returns EFWI_ERR_ARG, and
+ * leaves the context without a model, when the Courant number of the model
+ * (efwi_courant_number of the largest sqrt((lambda + 2 mu) / rho)) exceeds 1,
+ * where the reference exits (Src/libCUFD.cu:91, Src/utilities.cu:186-202).
+ * Synchronises `stream` (the guard reads the maximum back). */
 int efwi_set_model(efwi_ctx* ctx, const float* lambda, const float* mu, const float* rho,
                    int on_host, void* stream);
 
@@ -177,6 +181,11 @@ long long efwi_device_bytes(const efwi_ctx* ctx);
  * rows K, a, b, K_half, a_half, b_half (Src/utilities.cu:204-320), and the
  * squared time taper w(t)^2 (Src/utilities.cu:578-614). */
 int efwi_cpml_profile(int n, int npml, float h, float f0, float dt, float* out6n);
+/* Courant number of the 3-D scheme, vp_max dt sqrt(3) (9/8 + 1/24) /
+ * min(dz, dx, dy); efwi_set_model refuses models above 1. The reference's
+ * compCourantNumber (Src/utilities.cu:186-202) uses sqrt(2) / min(dz, dx),
+ * which is never larger. */
+int efwi_courant_number(double vp_max, float dt, float dz, float dx, float dy, double* out);
 int efwi_taper_w2(int nt, float dt, float ratio, float* out);
 
 /* The reference's native entry, cufd (Src/libCUFD.h:6-9, Src/libCUFD.cu:37),

@@ -55,6 +55,37 @@ def test_library_taper_equals_oracle(orc, nt, dt, ratio):
     assert np.array_equal(out, orc.taper_w2(nt, dt, ratio))
 
 
+def _courant(vmax, dt, dz, dx, dy):
+    from efwi import _native as N
+    out = C.c_double(0.0)
+    N.check(N.lib().efwi_courant_number(vmax, dt, dz, dx, dy, C.byref(out)))
+    return out.value
+
+
+def test_courant_number_closed_form_and_configs():
+    """efwi_set_model's stability guard (Src/utilities.cu:186-202 with the 3-D
+    factor sqrt(3) in place of sqrt(2)): the closed form, its scaling in vmax,
+    dt and the smallest step, and the numbers of the five workloads (stable:
+    0.78, 0.70 and 0.68 for 64^3/10 m, 120x200x200/12.5 m and 160x320x320/10 m)."""
+    k = np.sqrt(3.0) * (9.0 / 8.0 + 1.0 / 24.0)
+    assert _courant(3000.0, 1e-3, 10.0, 10.0, 10.0) == pytest.approx(3000 * float(np.float32(1e-3)) * k / 10, rel=1e-12)
+    # the smallest of the three steps sets it, whichever axis it is on
+    for steps in ((5.0, 10.0, 10.0), (10.0, 5.0, 10.0), (10.0, 10.0, 5.0)):
+        assert _courant(3000.0, 1e-3, *steps) == pytest.approx(2 * _courant(3000.0, 1e-3, 10, 10, 10), rel=1e-12)
+    # never below the reference's 2-D form sqrt(2)(9/8 + 1/24) / min(dz, dx)
+    two_d = 3000.0 * float(np.float32(1e-3)) * np.sqrt(2.0) * (1 / 24 + 9 / 8) / 10.0
+    assert _courant(3000.0, 1e-3, 10.0, 10.0, 10.0) > two_d
+    # the workloads: vp max (anomaly included) against h and dt
+    assert _courant(3500 * 1.10, 1e-3, 10, 10, 10) == pytest.approx(0.778, abs=2e-3)
+    assert _courant(4000 * 1.08, 1e-3, 12.5, 12.5, 12.5) == pytest.approx(0.698, abs=2e-3)
+    assert _courant(4200, 8e-4, 10, 10, 10) == pytest.approx(0.679, abs=2e-3)
+    # the stability edge: vp dt / h = 1 / (sqrt(3) (9/8 + 1/24))
+    edge = 10.0 / (float(np.float32(1e-3)) * k)
+    assert _courant(edge * (1 - 1e-6), 1e-3, 10, 10, 10) < 1.0 < _courant(edge * (1 + 1e-6), 1e-3, 10, 10, 10)
+    from efwi import _native as N
+    assert N.lib().efwi_courant_number(3000.0, 0.0, 10.0, 10.0, 10.0, C.byref(C.c_double())) == -1
+
+
 def test_cufd_reports_errors_instead_of_exiting(tmp_path):
     from efwi import _native as N
     lib = N.lib()

@@ -114,3 +114,27 @@ def test_precision_modes_agree(orc):
                 assert max(e.values()) < bar, e
             else:
                 assert e["syn"] < 1e-5, e
+        # fp32 (the GPU's arithmetic) against the reference's literal
+        # arithmetic, whose 2.0 * mu literals promote the normal-stress
+        # updates and several adjoint terms to FP64 (Src/stress.cu:57-59,
+        # 112-114,132, Src/stress_adj.cu:131-140, Src/velocity_adj.cu:72,100,129)
+        lit = {k: rel_l2(getattr(res["fp32"], k), getattr(res["fp32_literal"], k)) for k in ("glam", "gmu", "grho")}
+        lit["syn"] = rel_l2(res["fp32"].syn[0], res["fp32_literal"].syn[0])
+        lit["res"] = rel_l2(res["fp32"].res[0], res["fp32_literal"].res[0])
+        print(f"{label}: fp32 vs fp32_literal:", {k: f"{v:.2e}" for k, v in lit.items()})
+        assert lit["syn"] < 1e-5, lit
+        if bar is not None:
+            # the pipeline's own float error: within the 1e-5 bar everywhere
+            assert max(lit.values()) < bar, lit
+        else:
+            # a residual from a nearby model is a small difference of large
+            # seismograms (|r| / |syn| ~ 1e-4 here): each arithmetic's rounding
+            # is amplified alike, and the two float arithmetics differ from each
+            # other by no more than they differ from the FP64 run
+            for k in ("glam", "gmu", "grho", "res"):
+                e32 = rel_l2(getattr(res["fp32"], k) if k != "res" else res["fp32"].res[0],
+                             getattr(res["fp64"], k) if k != "res" else res["fp64"].res[0])
+                elit = rel_l2(getattr(res["fp32_literal"], k) if k != "res" else res["fp32_literal"].res[0],
+                              getattr(res["fp64"], k) if k != "res" else res["fp64"].res[0])
+                assert lit[k] <= 1.5 * (e32 + elit), (k, lit[k], e32, elit)
+                assert e32 <= 4.0 * elit, (k, e32, elit)

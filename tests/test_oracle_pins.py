@@ -425,3 +425,113 @@ def test_band_mask_is_box_minus_inner_box(orc):
     inner = np.zeros_like(ref)
     inner[p + 3:g.nz - p - 3, p + 3:g.nx - p - 3, p + 3:g.ny - p - 3] = True
     assert np.array_equal(got, box & ~inner)
+
+
+# ---------------------------------------------------------------------------
+# Source and receiver terms (Src/utilities.cu:442-495, Src/libCUFD.cu:319-351,
+# Src/Model.cu:65): zero and linear response in the source time function, and
+# the exact wavefield of an explosive point source in a homogeneous medium
+# (tests/analytic.py), which fixes the amplitude 1500^2 s dt, the buoyancy
+# 1/rho, the source and receiver cells and the it + 1 record slot.
+# ---------------------------------------------------------------------------
+from analytic import Explosion, components  # noqa: E402
+
+_EXPL = dict(n=72, npml=12, nt=260, h=5.0, dt=1e-3, f0=25.0, vp=2000.0, rho=2.0)
+_EXPL_SRC = (33, 38, 35)
+_EXPL_OFFS = [(0, 10, 0), (0, 0, 14), (8, 8, 0), (-10, 6, 6), (0, -18, 0), (12, 0, -9), (0, 20, 5),
+              (-15, -5, 0)]
+
+
+def _homogeneous(orc, src, offs, stf, rho_scale=1.0, precision="fp64", nt=None):
+    e = _EXPL
+    nt = nt or e["nt"]
+    n = e["n"]
+    g = orc.Grid(n, n, n, e["npml"], nt, e["h"], e["h"], e["h"], e["dt"], e["f0"])
+    vp, rho = e["vp"], e["rho"]
+    vs = vp / 1.73
+    lam, mu = (vp ** 2 - 2 * vs ** 2) * rho, vs ** 2 * rho
+    shp = (n, n, n)
+    rec = np.array([[_EXPL_SRC[0] + a, _EXPL_SRC[1] + b, _EXPL_SRC[2] + c] for a, b, c in offs])
+    sh = orc.Shots(src=np.array([src]), stf=np.asarray(stf, np.float32)[None, :nt], rec=[rec])
+    r = orc.run(g, sh, np.full(shp, lam, np.float32), np.full(shp, mu, np.float32),
+                np.full(shp, rho * rho_scale, np.float32), calc_id=2, precision=precision)
+    return r.syn[0]
+
+
+def _stf(nt=None):
+    from efwi import synth
+    e = _EXPL
+    s = synth.ricker_integrated(e["f0"], nt or e["nt"], e["dt"]).astype(np.float32)
+    s[0] = 0.0  # the 0.001 stf taper (Src/Src_Rec.cu:146) zeroes sample 0 only; keep it identity
+    return s
+
+
+def _explosion_fits(syn, stf):
+    e = _EXPL
+    A = Explosion(stf, e["dt"], e["rho"], e["vp"], e["h"])
+    return [(k, ci, A.fit(syn[ci, k], syn.shape[2], rel, axis)) for k, ci, rel, axis in components(_EXPL_OFFS)]
+
+
+def _explosion_ok(fits):
+    return all(abs(f["shift"]) <= 0.4 and 0.92 <= f["amp"] <= 1.08 and f["err"] <= 0.08 and f["err0"] <= 0.10
+               for _, _, f in fits)
+
+
+def test_zero_source_gives_zero_output(orc):
+    syn = _homogeneous(orc, _EXPL_SRC, _EXPL_OFFS[:3], np.zeros(_EXPL["nt"], np.float32), nt=80)
+    assert np.count_nonzero(syn) == 0
+
+
+def test_output_linear_in_source(orc):
+    nt = 120
+    s1 = _stf(nt)
+    rng = np.random.default_rng(5)
+    s2 = (rng.standard_normal(nt) * np.abs(s1).max()).astype(np.float32)
+    s2[0] = 0.0
+    a, b = 0.75, -1.5
+    y1 = _homogeneous(orc, _EXPL_SRC, _EXPL_OFFS[:4], s1, nt=nt)
+    y2 = _homogeneous(orc, _EXPL_SRC, _EXPL_OFFS[:4], s2, nt=nt)
+    y12 = _homogeneous(orc, _EXPL_SRC, _EXPL_OFFS[:4], (a * s1.astype(np.float64) + b * s2).astype(np.float32),
+                       nt=nt)
+    ref = a * y1 + b * y2
+    assert np.abs(y1).max() > 0
+    # the combined stf is rounded to float once: that is the only difference
+    assert rel_l2(y12, ref) <= 1e-6
+
+
+def test_point_source_matches_exact_explosion(orc):
+    """fp64 oracle against the closed-form explosion at 8 receivers 10-21 cells
+    away in every direction: best time shift within 0.4 samples of zero,
+    amplitude within 8 %, shape within 8 % (4th-order grid dispersion at
+    ~7 cells per shortest wavelength and the point source's discretisation)."""
+    stf = _stf()
+    fits = _explosion_fits(_homogeneous(orc, _EXPL_SRC, _EXPL_OFFS, stf), stf)
+    for k, ci, f in fits:
+        print(f"receiver {k} component {'xyz'[ci]}: shift {f['shift']:+.2f} amp {f['amp']:.3f} "
+              f"err {f['err']:.3f} err0 {f['err0']:.3f}")
+    assert len(fits) >= 12
+    assert _explosion_ok(fits)
+
+
+def test_explosion_check_catches_planted_mistakes(orc):
+    """The same check fails for each mistake it is meant to catch, planted in
+    the oracle's inputs or outputs: the amplitude 1500 instead of 1500^2, the
+    source one cell off, the record slot it instead of it + 1, and rho in
+    place of the buoyancy 1/rho."""
+    stf = _stf()
+    good = _homogeneous(orc, _EXPL_SRC, _EXPL_OFFS, stf)
+    assert _explosion_ok(_explosion_fits(good, stf))
+    # 1500 * s * dt
+    assert not _explosion_ok(_explosion_fits(good / 1500.0, stf))
+    # record slot it: every trace one sample earlier
+    early = np.zeros_like(good)
+    early[:, :, :-1] = good[:, :, 1:]
+    assert not _explosion_ok(_explosion_fits(early, stf))
+    # source one cell off along each axis
+    for ax in range(3):
+        src = list(_EXPL_SRC)
+        src[ax] += 1
+        assert not _explosion_ok(_explosion_fits(_homogeneous(orc, src, _EXPL_OFFS, stf), stf)), ax
+    # buoyancy rho instead of 1/rho: the model's rho becomes 1/rho
+    assert not _explosion_ok(_explosion_fits(_homogeneous(orc, _EXPL_SRC, _EXPL_OFFS, stf,
+                                                          rho_scale=1.0 / _EXPL["rho"] ** 2), stf))
