@@ -589,6 +589,50 @@ void Ctx::forward_step(const Groups& Ff, const Wave& Wf, const Source& Sf, int i
   if (record) launch_record(Wf, rec_dev, nrec, it + 1, syn_t, s);
 }
 
+// The second set of wavefield and C-PML arrays of the fused steps, zeroed
+// once (cells a step never writes stay zero in both sets).
+int Ctx::ensure_alt() {
+  if (Falt.w) return EFWI_OK;
+  const size_t fs = G.fs;
+  int rc;
+  if ((rc = alloc((void**)&Falt.w, fs * F_WAVE_COUNT)) || (rc = alloc((void**)&Falt.p, fs * P_PSI_COUNT))) return rc;
+  HIP_TRY(hipMemset(Falt.w, 0, fs * F_WAVE_COUNT));
+  HIP_TRY(hipMemset(Falt.p, 0, fs * P_PSI_COUNT));
+  HIP_TRY(hipDeviceSynchronize());
+  return EFWI_OK;
+}
+
+// After a fused step: the new state is in Falt; make it the current one.
+void Ctx::swap_state() {
+  std::swap(F.w, Falt.w);
+  std::swap(F.p, Falt.p);
+  const size_t fsf = G.fs / sizeof(float);
+  wave_of(F.w, fsf, W);
+  psi_of(F.p, fsf, P);
+}
+
+void Ctx::fused_forward_step(const Source& Sf, int it, bool record, hipStream_t s, long save) {
+  RingFold rf{};
+  rf.B = B;
+  rf.slot = save >= 0 ? ring + save * 9 * B.n : nullptr;
+  Groups Fo = F;
+  Fo.w = Falt.w;
+  Fo.p = Falt.p;
+  const Tile t = tiles[K_FWD_FUSED];
+  prof.begin(K_FWD_FUSED, s);
+  const bool folded = launch_fwd_fused(t.rows, t.zdepth, cfg.formulation, G, F, Fo, C, Sf, it, s, &rf);
+  prof.end(s);
+  swap_state();
+  if (save >= 0 && !folded) {
+    Fields9 F9;
+    fields9(W, F9);
+    prof.begin(K_RING, s);
+    launch_ring(B, G, F9, ring, save, 0, 9, 0, s);
+    prof.end(s);
+  }
+  if (record) launch_record(W, rec_dev, nrec, it + 1, syn_t, s);
+}
+
 int Ctx::run_shot(const efwi_shot* sh, int calc_id, const float* obs, float* syn, float* res,
                   double* misfit, hipStream_t s) {
   if (calc_id < 0 || calc_id > 2) return fail(EFWI_ERR_ARG, "invalid calc_id " + std::to_string(calc_id));
@@ -602,8 +646,9 @@ int Ctx::run_shot(const efwi_shot* sh, int calc_id, const float* obs, float* syn
   const bool grad = calc_id == EFWI_CALC_GRADIENT;
   if (grad && (rc = ensure_ring())) return rc;
   const int nt = cfg.nt, form = cfg.formulation;
+  const bool fused_fwd = tiles[K_FWD_FUSED].rows > 0 && src_stencil;
+  if (fused_fwd && (rc = ensure_alt())) return rc;
   Fields9 F9;
-  fields9(W, F9);
 
   zero_wave(W, N, s);
   zero_wave(A, N, s);
@@ -624,13 +669,17 @@ int Ctx::run_shot(const efwi_shot* sh, int calc_id, const float* obs, float* syn
     // step), the first by a pass of its own, the others by the step before.
     const int a = last_seg * seg_len;
     if (grad && it == a) {
+      fields9(W, F9);
       prof.begin(K_RING, s);
       launch_ring(B, G, F9, ring, 0, 0, 9, 0, s);
       prof.end(s);
     }
-    forward_step(F, W, Sf, it, true, s, grad && it >= a && it + 1 <= nt - 2 ? it + 1 - a : -1);
+    const long save = grad && it >= a && it + 1 <= nt - 2 ? it + 1 - a : -1;
+    if (fused_fwd) fused_forward_step(Sf, it, true, s, save);
+    else forward_step(F, W, Sf, it, true, s, save);
     if (illum_on) launch_illum(W.vz, W.vx, W.vy, illum, N, s);
   }
+  fields9(W, F9);
   HIP_TRY(hipGetLastError());
   if (syn && nrec > 0) launch_from_time_major(syn_t, syn, nrec, nt, s);
 
@@ -884,6 +933,15 @@ int efwi_set_tile(efwi_ctx* ctx, int kernel, int rows, int zdepth) {
   if (zdepth >= 16 && rows != 1 && rows != 2 && rows != 4 && rows != 8)
     return fail(EFWI_ERR_ARG, "rows must be 1, 2, 4 or 8 (z-marching sweep, zdepth >= 16)");
   reinterpret_cast<Ctx*>(ctx)->tiles[kernel] = efwi::Tile{rows, zdepth};
+  return EFWI_OK;
+}
+
+int efwi_set_fusion(efwi_ctx* ctx, int pass, int rows, int zchunk) {
+  if (!ctx) return fail(EFWI_ERR_ARG, "NULL context");
+  if (pass != 0) return fail(EFWI_ERR_ARG, "pass must be 0 (forward)");
+  if (rows != 0 && rows != 4 && rows != 8 && rows != 12) return fail(EFWI_ERR_ARG, "rows must be 0, 4, 8 or 12");
+  if (zchunk < 1 || zchunk > 4096) return fail(EFWI_ERR_ARG, "zchunk must be in [1, 4096]");
+  reinterpret_cast<Ctx*>(ctx)->tiles[efwi::K_FWD_FUSED] = efwi::Tile{rows, zchunk};
   return EFWI_OK;
 }
 

@@ -28,7 +28,7 @@ float taper_window_fac(int it, int nt, float dt, float ratio, float t0, float t3
 
 // Optional per-kernel timing with HIP events on the launch stream.
 enum KernelId { K_STRESS_FWD, K_VELOCITY_FWD, K_STRESS_BWD, K_VELOCITY_BWD, K_VELOCITY_ADJ, K_STRESS_ADJ,
-                K_RING, K_OTHER, K_COUNT };
+                K_RING, K_OTHER, K_FWD_FUSED, K_BWD_FUSED, K_COUNT };
 
 struct Profiler {
   bool on = false;
@@ -67,7 +67,11 @@ struct BatchState {
 struct Ctx {
   static constexpr int kMaxPartials = 4096;
   Profiler prof;
-  Tile tiles[K_COUNT] = {{1, 1}, {1, 1}, {1, 1}, {1, 1}, {1, 1}, {1, 1}, {1, 1}, {1, 1}};
+  Tile tiles[K_COUNT] = {{1, 1}, {1, 1}, {1, 1}, {1, 1}, {1, 1}, {1, 1}, {1, 1}, {1, 1}, {0, 16}, {0, 16}};
+  // fused steps (efwi_set_fusion): tiles[K_FWD_FUSED] = {rows, z planes per
+  // workgroup}, rows 0 = off. They advance the state from F into Falt (w and p
+  // groups), and the two sets are swapped after every step.
+  Groups Falt{};
   efwi_config cfg{};
   Geom G{};
   long N = 0;
@@ -128,6 +132,9 @@ struct Ctx {
   int checkpoint(int k, const Wave& Wf, const Psi& Pf, bool save, hipStream_t s);
   void forward_step(const Groups& Ff, const Wave& Wf, const Source& Sf, int it, bool record, hipStream_t s,
                     long save = -1);
+  int ensure_alt();
+  void swap_state();
+  void fused_forward_step(const Source& Sf, int it, bool record, hipStream_t s, long save = -1);
   int set_model(const float* l, const float* m, const float* rho, int on_host, hipStream_t s);
   int zero_gradient(hipStream_t s);
   int prepare_shot(const efwi_shot* sh, int calc_id);

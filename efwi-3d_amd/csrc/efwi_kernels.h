@@ -37,6 +37,13 @@ struct Sweep {
 };
 Sweep make_sweep(int z0, int z1, int x0, int x1, int y0, int y1, Tile t);
 
+// Fused forward step (stress + velocity of one time step in one launch):
+// workgroups of 64 x (R + 4) threads own R x rows of 60 y cells and Zc z
+// planes; tile t = (zc * nxt + xt) * nys + ysg, t = (b % 8) * per_xcd + b / 8.
+struct FusedSweep {
+  int nys, nxt, nzc, Zc, tiles, per_xcd, zrev;
+};
+
 // stencil kernels (efwi_stencil.hip). F.w is the wavefield a forward kernel
 // advances (the forward state or its replay copy); the backward kernels
 // reverse F.w and advance the adjoint F.a.
@@ -61,6 +68,11 @@ bool launch_velocity_bwd(Tile t, int form, const Geom& G, const Groups& F, const
                          const RingFold* ring = nullptr);
 bool launch_stress_bwd(Tile t, const Geom& G, const Groups& F, const Prof& C, const Source& S, int it,
                        hipStream_t s, const RingFold* ring = nullptr);
+// One fused forward step from the state in Fi (w and p groups) to Fo (Fi.m
+// holds the model); rows R in {4, 8, 12}, zc planes per workgroup. The source
+// must lie inside the stencil box. Returns whether the ring save was folded.
+bool launch_fwd_fused(int R, int zc, int form, const Geom& G, const Groups& Fi, const Groups& Fo, const Prof& C,
+                      const Source& S, int it, hipStream_t s, const RingFold* ring = nullptr);
 // adjoint-only steps (no reverse reconstruction): the first backward step
 void launch_velocity_adj(int form, const Geom& G, const Groups& F, const Prof& C, hipStream_t s);
 void launch_stress_adj(const Geom& G, const Groups& F, const Prof& C, hipStream_t s);

@@ -956,6 +956,187 @@ __global__ __launch_bounds__(64 * R) void k_stress_bwd_m(Sweep Sw, Geom G, Group
 }
 
 // ---------------------------------------------------------------------------
+// Fused forward step: the stress and the velocity update of one time step in
+// one pass over the grid, reading the state of step it from the "in" field
+// groups and writing the state of step it + 1 to the "out" groups (the caller
+// alternates the two sets). A workgroup of 64 x (R + 4) threads owns R x rows
+// of 60 y cells and a chunk of z planes; it marches z, computing the new
+// stress of plane z + 2 for its whole thread grid (the owned cells plus a
+// 2-cell ring around them, which neighbouring workgroups own and compute as
+// well) and then the new velocity of plane z for the owned cells from the new
+// stress: z neighbours from per-thread register queues, x and y neighbours from
+// an LDS tile of the plane. Because no workgroup writes what another reads,
+// the order of workgroups does not matter; every value is the flat kernels'
+// expression, so the results are bit-identical to them.
+// ---------------------------------------------------------------------------
+constexpr int kFusedY = 60;  // owned y cells per wavefront (lanes 2..61)
+
+// Stress accessor of the fused step: state of step it from the in-groups; new
+// stress values go to `sig` (and to the out-group for owned cells), new psi
+// values to the out-group for owned cells only.
+struct FusedSAcc {
+  FlatAcc f;
+  Grp wo, po;
+  float* sig;  // [6]: new szz, sxx, sxz, syy, sxy, syz (WaveField order from F_SZZ)
+  bool owned;
+  float* rs;
+  long rn;
+  int rk;
+  __device__ __forceinline__ float own(int gi, int k) const { return f.own(gi, k); }
+  __device__ __forceinline__ float at(int gi, int k, int ax, int d) const { return f.at(gi, k, ax, d); }
+  __device__ __forceinline__ void st(int gi, int k, float v) const {
+    if (gi == GW) {
+      sig[k - F_SZZ] = v;
+      if (owned) {
+        efwi::st(wo, k, f.o, v);
+        if (rk >= 0) rs[(long)k * rn + rk] = v;
+      }
+    } else if (owned) {
+      efwi::st(po, k, f.o, v);
+    }
+  }
+};
+
+// Velocity accessor of the fused step: the new stress from the register
+// queues (z neighbours) and the LDS tile (x and y neighbours), everything else
+// from the in-groups; results to the out-groups.
+template <int NR>
+struct FusedVAcc {
+  FlatAcc f;
+  Grp wo, po;
+  const float (*q)[5];  // z queues of szz, sxz, syz: planes z-2..z+2
+  const float* t;       // this cell in LDS field 0 of the current plane
+  float* rs;
+  long rn;
+  int rk;
+  __device__ __forceinline__ float own(int gi, int k) const { return f.own(gi, k); }
+  __device__ __forceinline__ float at(int gi, int k, int ax, int d) const {
+    if (gi != GW || k < F_SZZ) return f.at(gi, k, ax, d);
+    if (ax == AZ) return q[k == F_SZZ ? 0 : k == F_SXZ ? 1 : 2][2 + d];
+    // LDS fields: sxz, syz, sxx, sxy, syy
+    const int l = k == F_SXZ ? 0 : k == F_SYZ ? 1 : k == F_SXX ? 2 : k == F_SXY ? 3 : 4;
+    const float* p = t + l * NR * 64;
+    return ax == AX ? p[d * 64] : p[d];
+  }
+  __device__ __forceinline__ void st(int gi, int k, float v) const {
+    if (gi == GW) {
+      efwi::st(wo, k, f.o, v);
+      if (rk >= 0) rs[(long)k * rn + rk] = v;
+    } else {
+      efwi::st(po, k, f.o, v);
+    }
+  }
+};
+
+template <int FORM, int R>
+__global__ __launch_bounds__(64 * (R + 4)) void k_fwd_fused(FusedSweep S, Geom G, Groups Fi, Groups Fo, Prof C,
+                                                            Source Src, int it, RingFold Rf) {
+  constexpr int NR = R + 4;
+  __shared__ float lds[2][5 * NR * 64];
+  const int b = blockIdx.x;
+  const int tile = (b & 7) * S.per_xcd + (b >> 3);
+  if (tile >= S.tiles) return;  // whole workgroup: uniform
+  const int ysg = tile % S.nys;
+  const int xt = (tile / S.nys) % S.nxt;
+  const int zc0 = tile / (S.nys * S.nxt);
+  const int zc = S.zrev ? S.nzc - 1 - zc0 : zc0;
+  const int row = __builtin_amdgcn_readfirstlane((int)threadIdx.y);
+  const int lane = (int)threadIdx.x;
+  const int x = xt * R + row;                 // rows 2.. owned: x = 2 + xt R + (row - 2)
+  const int y = ysg * kFusedY + lane;         // lanes 2.. owned
+  const bool x_in = x >= 2 && x <= G.nx - 3;  // stress computed here
+  const bool y_in = y >= 2 && y <= G.ny - 3;
+  const bool own_xy = row >= 2 && row < R + 2 && x_in && lane >= 2 && lane < kFusedY + 2 && y_in;
+  const int za = 2 + zc * S.Zc, zb = min(za + S.Zc, G.nz - 2);  // owned planes [za, zb)
+  const Grp wi = group(Fi.w, F_WAVE_COUNT, G.fs), pi = group(Fi.p, P_PSI_COUNT, G.fs);
+  const Grp wo = group(Fo.w, F_WAVE_COUNT, G.fs), po = group(Fo.p, P_PSI_COUNT, G.fs);
+  const Grp gm = group(Fi.m, M_COUNT, G.fs);
+  const Grp ga = group(Fi.a, F_WAVE_COUNT, G.fs);
+  const unsigned so = source_off(Src);
+  const float amp = source_amp(Src, G, it);
+  const unsigned sx = 4u * (unsigned)G.py, sz = 4u * (unsigned)G.pxy;
+  // this thread's column (clamped into the row pitch for the halo lanes past it)
+  const int yc = min(y, G.py - 1);
+  const unsigned col = 4u * ((unsigned)min(x, G.nx - 1) * (unsigned)G.py + (unsigned)yc);
+  float q[3][5], dq[3][3];
+#pragma unroll
+  for (int i = 0; i < 3; i++) {
+#pragma unroll
+    for (int k = 0; k < 5; k++) q[i][k] = 0.0f;
+#pragma unroll
+    for (int k = 0; k < 3; k++) dq[i][k] = 0.0f;
+  }
+  float* const me0 = &lds[0][row * 64 + lane];
+  float* const me1 = &lds[1][row * 64 + lane];
+#pragma unroll 1
+  for (int zs = za - 2; zs < zb + 2; zs++) {
+    // ---- new stress of plane zs at this thread's column
+    float s6[6];
+    const unsigned o = (unsigned)zs * sz + col;
+    if (x_in && y_in && zs >= 2 && zs <= G.nz - 3) {
+      Cell c;
+      c.z = zs;
+      c.x = x;
+      c.y = y;
+      c.o = o;
+      c.zn = Nb{o - 2 * sz, o - sz, o + sz, o + 2 * sz};
+      c.xn = Nb{o - 2 * sx, o - sx, o + sx, o + 2 * sx};
+      c.yn = Nb{o - 8u, o - 4u, o + 4u, o + 8u};
+      const bool own = own_xy && zs >= za && zs < zb;
+      const FusedSAcc acc{FlatAcc{wi, ga, pi, gm, o, {c.zn, c.xn, c.yn}}, wo, po, s6, own, Rf.slot, Rf.B.n,
+                          own ? ring_index(Rf, c) : -1};
+      stress_fwd_cell(G, acc, C, c, so, amp);
+    } else if (zs >= 0 && zs < G.nz && y < G.ny) {
+      // outside the stencil box: the state there never changes
+#pragma unroll
+      for (int k = 0; k < 6; k++) s6[k] = ld(wi, F_SZZ + k, o);
+    } else {
+#pragma unroll
+      for (int k = 0; k < 6; k++) s6[k] = 0.0f;
+    }
+    // ---- queues: szz, sxz, syz five planes deep, sxx, sxy, syy three
+#pragma unroll
+    for (int i = 0; i < 3; i++) {
+#pragma unroll
+      for (int k = 0; k < 4; k++) q[i][k] = q[i][k + 1];
+#pragma unroll
+      for (int k = 0; k < 2; k++) dq[i][k] = dq[i][k + 1];
+    }
+    q[0][4] = s6[F_SZZ - F_SZZ];
+    q[1][4] = s6[F_SXZ - F_SZZ];
+    q[2][4] = s6[F_SYZ - F_SZZ];
+    dq[0][2] = s6[F_SXX - F_SZZ];
+    dq[1][2] = s6[F_SXY - F_SZZ];
+    dq[2][2] = s6[F_SYY - F_SZZ];
+    // ---- new velocity of plane zv = zs - 2
+    const int zv = zs - 2;
+    if (zv >= za) {
+      float* const me = (zv & 1) ? me1 : me0;
+      me[0 * NR * 64] = q[1][2];
+      me[1 * NR * 64] = q[2][2];
+      me[2 * NR * 64] = dq[0][0];
+      me[3 * NR * 64] = dq[1][0];
+      me[4 * NR * 64] = dq[2][0];
+      __syncthreads();
+      if (own_xy) {
+        const unsigned ov = (unsigned)zv * sz + col;
+        Cell c;
+        c.z = zv;
+        c.x = x;
+        c.y = y;
+        c.o = ov;
+        c.zn = Nb{ov - 2 * sz, ov - sz, ov + sz, ov + 2 * sz};
+        c.xn = Nb{ov - 2 * sx, ov - sx, ov + sx, ov + 2 * sx};
+        c.yn = Nb{ov - 8u, ov - 4u, ov + 4u, ov + 8u};
+        const FusedVAcc<NR> acc{FlatAcc{wi, ga, pi, gm, ov, {c.zn, c.xn, c.yn}}, wo, po, q, me, Rf.slot,
+                                Rf.B.n, ring_index(Rf, c)};
+        velocity_fwd_cell<FORM>(G, acc, C, c);
+      }
+    }
+  }
+}
+
+// ---------------------------------------------------------------------------
 // Batched shots: one persistent launch per stencil step covers every (shot,
 // tile) item of the batch. Items are the flat sweep's workgroup tiles; those of
 // XCD strip q (tile index b with b % 8 == q) of every shot form queue q. A
@@ -1237,6 +1418,29 @@ bool launch_stress_bwd(Tile t, const Geom& G, const Groups& F, const Prof& C, co
   hipLaunchKernelGGL(k_stress_bwd<true>, grid_of(Sw), block_of(Sw), 0, s, Sw, G, F, C, S, it, Rf);
   return Rf.slot != nullptr;
 }
+bool launch_fwd_fused(int R, int zc, int form, const Geom& G, const Groups& Fi, const Groups& Fo, const Prof& C,
+                      const Source& S, int it, hipStream_t s, const RingFold* ring) {
+  R = R >= 12 ? 12 : R >= 8 ? 8 : 4;
+  FusedSweep Sw;
+  Sw.Zc = std::max(1, zc);
+  Sw.nys = (G.ny - 4 + kFusedY - 1) / kFusedY;
+  Sw.nxt = (G.nx - 4 + R - 1) / R;
+  Sw.nzc = (G.nz - 4 + Sw.Zc - 1) / Sw.Zc;
+  Sw.tiles = Sw.nys * Sw.nxt * Sw.nzc;
+  Sw.per_xcd = (Sw.tiles + 7) / 8;
+  Sw.zrev = 0;
+  const RingFold Rf = fold_for(G, ring);
+  const dim3 grid((unsigned)(8 * Sw.per_xcd)), block(64, R + 4);
+#define EFWI_FUSED(RR)                                                                                     \
+  if (form == 1) hipLaunchKernelGGL((k_fwd_fused<1, RR>), grid, block, 0, s, Sw, G, Fi, Fo, C, S, it, Rf); \
+  else hipLaunchKernelGGL((k_fwd_fused<0, RR>), grid, block, 0, s, Sw, G, Fi, Fo, C, S, it, Rf)
+  if (R == 12) { EFWI_FUSED(12); }
+  else if (R == 8) { EFWI_FUSED(8); }
+  else { EFWI_FUSED(4); }
+#undef EFWI_FUSED
+  return Rf.slot != nullptr;
+}
+
 void launch_velocity_adj(int form, const Geom& G, const Groups& F, const Prof& C, hipStream_t s) {
   const Sweep Sw = sweep_full(G, Tile{1, 1});
   const RingFold none = fold_for(G, nullptr);

@@ -19,7 +19,7 @@ LIB_PATH = os.path.join(_HERE, "libefwi.so")
 
 OK = 0
 KERNELS = ("stress_fwd", "velocity_fwd", "stress_bwd", "velocity_bwd", "velocity_adj", "stress_adj",
-           "ring", "other")
+           "ring", "other", "fwd_fused", "bwd_fused")
 CALC_MISFIT, CALC_GRADIENT, CALC_SYNTHETIC = 0, 1, 2
 TUNED_KERNELS = ("stress_fwd", "velocity_fwd", "stress_bwd", "velocity_bwd")
 # efwi_autotune's candidates, in its report order: flat sweep (zdepth < 16), then z-marching
@@ -62,6 +62,7 @@ SIGNATURES = {
     "efwi_device_bytes": (C.c_longlong, [_vp]),
     "efwi_dot64": (C.c_int, [_vp, _vp, C.c_longlong, _dp, _vp]),
     "efwi_set_tile": (C.c_int, [_vp, C.c_int, C.c_int, C.c_int]),
+    "efwi_set_fusion": (C.c_int, [_vp, C.c_int, C.c_int, C.c_int]),
     "efwi_get_tile": (C.c_int, [_vp, C.c_int, _ip, _ip]),
     "efwi_autotune": (C.c_int, [_vp, C.c_int, _dp, _vp]),
     "efwi_profile": (C.c_int, [_vp, C.c_int]),

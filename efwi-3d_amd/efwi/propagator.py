@@ -137,6 +137,11 @@ class Propagator:
         """Launch shape of one time-loop kernel (results do not depend on it)."""
         N.check(N.lib().efwi_set_tile(self._h, N.KERNELS.index(kernel), int(rows), int(zdepth)), "efwi_set_tile")
 
+    def set_fusion(self, rows: int, zchunk: int = 16, pass_: int = 0):
+        """Fused forward step (stress + velocity in one launch; rows 4, 8 or
+        12, 0 = two kernels). Results do not depend on it."""
+        N.check(N.lib().efwi_set_fusion(self._h, int(pass_), int(rows), int(zchunk)), "efwi_set_fusion")
+
     def tiles(self) -> dict:
         out = {}
         for k in N.TUNED_KERNELS:

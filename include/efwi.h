@@ -122,8 +122,9 @@ int efwi_get_illumination(efwi_ctx* ctx, float* out, int on_host, void* stream);
  * efwi_profile_read fills EFWI_NKERNEL totals in milliseconds and launch
  * counts, indexed stress_fwd, velocity_fwd, stress_bwd (reverse stress +
  * adjoint stress), velocity_bwd (reverse velocity + adjoint velocity),
- * velocity_adj, stress_adj (first backward step only), ring, other. */
-#define EFWI_NKERNEL 8
+ * velocity_adj, stress_adj (first backward step only), ring, other,
+ * fwd_fused, bwd_fused (the fused steps of efwi_set_fusion). */
+#define EFWI_NKERNEL 10
 int efwi_profile(efwi_ctx* ctx, int enable);
 int efwi_profile_read(efwi_ctx* ctx, double* ms, long long* launches, int reset);
 
@@ -140,6 +141,18 @@ int efwi_profile_read(efwi_ctx* ctx, double* ms, long long* launches, int reset)
  * average milliseconds in that order. No reference counterpart (the
  * reference launches fixed 32 x 1 x 32 blocks, Src/libCUFD.cu:93-97). */
 int efwi_set_tile(efwi_ctx* ctx, int kernel, int rows, int zdepth);
+
+/* Fused time steps. pass 0: the forward step's stress and velocity updates in
+ * one launch that marches z over workgroups of 64 x (rows + 4) threads, each
+ * owning `rows` x rows (4, 8 or 12) of 60 y cells and `zchunk` z planes and
+ * recomputing the new stress of a 2-cell ring around them, so that the new
+ * velocity reads the new stress from registers and LDS instead of memory.
+ * The state then alternates between two sets of wavefield and C-PML arrays
+ * (allocated on first use, ~27 fields). rows 0 turns it off (the two-kernel
+ * step). Results are bit-identical either way; sources outside the stencil
+ * box use the two-kernel step. No reference counterpart (the reference
+ * launches each kernel separately, Src/libCUFD.cu:312-354). */
+int efwi_set_fusion(efwi_ctx* ctx, int pass, int rows, int zchunk);
 int efwi_get_tile(const efwi_ctx* ctx, int kernel, int* rows, int* zdepth);
 int efwi_autotune(efwi_ctx* ctx, int reps, double* report, void* stream);
 

@@ -21,10 +21,11 @@ def test_unstable_model_is_refused_and_stable_one_runs():
     print(f"[gpu] guard: grid {g.shape}, vp max {vp:.1f}, Courant {vp * k:.3f}", flush=True)
     assert vp * k < 1.0
     # one cell fast enough to push the Courant number to 1.05
-    scale = (1.05 / (vp * k)) ** 2
+    c = (g.nz // 2, g.nx // 2, g.ny // 2)
+    vp_c = np.sqrt((float(lam[c]) + 2 * float(mu[c])) / float(rho[c]))
+    scale = (1.05 / (vp_c * k)) ** 2
     bad_lam = lam.copy()
     bad_mu = mu.copy()
-    c = (g.nz // 2, g.nx // 2, g.ny // 2)
     bad_lam[c] *= scale
     bad_mu[c] *= scale
     with Propagator(g) as P:
@@ -33,7 +34,7 @@ def test_unstable_model_is_refused_and_stable_one_runs():
         with pytest.raises(N.EfwiError, match="efwi_set_model was not called"):
             P.run_shot(w.shots[0], 2, None, want_syn=True)
         # the same cell just below the edge is accepted and runs
-        scale2 = (0.99 / (vp * k)) ** 2
+        scale2 = (0.99 / (vp_c * k)) ** 2
         bad_lam[c] = lam[c] * scale2
         bad_mu[c] = mu[c] * scale2
         P.set_model(torch.from_numpy(bad_lam), torch.from_numpy(bad_mu), torch.from_numpy(rho))
