@@ -36,6 +36,7 @@
 #include <cstdint>
 #include <cstdio>
 #include <cstring>
+#include <algorithm>
 #include <chrono>
 #include <memory>
 #include <vector>
@@ -206,6 +207,13 @@ struct EngineBase {
   virtual void ring_save(int slot) = 0;
   virtual void ring_restore(int slot, int k0, int k1) = 0;
   virtual void inject(const int* rec, long nrec, int it, const double* rx) = 0;
+  // checkpoint mode: the forward state (nine wavefield fields and the 18
+  // memory arrays) to and from raw storage, and the band of this engine's
+  // state into `dst`'s ring (an engine of the same precision)
+  virtual size_t state_bytes() const = 0;
+  virtual void save_state(char* out) const = 0;
+  virtual void load_state(const char* in) = 0;
+  virtual void ring_save_into(EngineBase* dst, int slot) const = 0;
 };
 
 // Order of the nine fields in a ring slot (the order of Src/Boundary.cu:59-96).
@@ -647,6 +655,31 @@ struct Engine : EngineBase {
       for (size_t b = 0; b < nb; b++) dst[b] = f[band_[b]];
     }
   }
+  static bool state_field(int f) { return f <= SYZ || (f >= M_DVZ_DZ && f <= M_DSYZ_DZ); }
+  size_t state_bytes() const override { return (size_t)27 * N * sizeof(R); }
+  void save_state(char* out) const override {
+    for (int f = 0; f < NFIELD; f++)
+      if (state_field(f)) {
+        std::memcpy(out, F[f].data(), N * sizeof(R));
+        out += N * sizeof(R);
+      }
+  }
+  void load_state(const char* in) override {
+    for (int f = 0; f < NFIELD; f++)
+      if (state_field(f)) {
+        std::memcpy(F[f].data(), in, N * sizeof(R));
+        in += N * sizeof(R);
+      }
+  }
+  void ring_save_into(EngineBase* dst, int slot) const override {
+    auto* d = static_cast<Engine*>(dst);
+    const size_t nb = d->band_.size();
+    for (int k = 0; k < 9; k++) {
+      const R* f = F[kRingField[k]].data();
+      R* out = d->ring_.data() + ((size_t)slot * 9 + k) * nb;
+      for (size_t b = 0; b < nb; b++) out[b] = f[d->band_[b]];
+    }
+  }
   // to_bnd (Src/Boundary.cu:97-138): k0..k1 selects the stress (0..6) or
   // velocity (6..9) part.
   void ring_restore(int slot, int k0, int k1) override {
@@ -803,6 +836,15 @@ struct orc_survey {
 //   misfit32 out, the reference's float accumulation of per-component fp32 block sums
 //   glam/gmu/grho out [N] (calc_id 1)
 //   seconds  out, wall time of the time loops
+// Checkpoint mode of orc_run (0 = the full ring of all nt - 1 steps, the
+// reference's scheme): with K > 0 the forward pass keeps the state every K
+// steps, and the backward pass replays each segment from its checkpoint in a
+// second engine to rebuild that segment's ring. The replayed band values are
+// the forward pass's own values, so the results equal the full ring's bit for
+// bit with K ring slots instead of nt - 1.
+static int g_ckpt_every = 0;
+void orc_set_checkpoint(int every) { g_ckpt_every = every > 0 ? every : 0; }
+
 int orc_run(const orc_grid* gg, const orc_survey* sv, int precision, int formulation,
             int calc_id, int cpml_off, const float* lam, const float* mu, const float* rho,
             const float* obs, double* syn, double* res, double* misfit64, float* misfit32,
@@ -843,6 +885,18 @@ int orc_run(const orc_grid* gg, const orc_survey* sv, int precision, int formula
     band_mask(g, mask);
     for (long i = 0; i < N; i++) if (mask[i]) band.push_back(i);
   }
+  const int K = with_adj && g_ckpt_every > 0 && g_ckpt_every < nt - 1 ? g_ckpt_every : 0;
+  std::unique_ptr<EngineBase> E2;
+  std::vector<std::vector<char>> ckpts;
+  if (K) {
+    E2.reset(make_engine(g, precision, formulation));
+    if (cpml_off) E2->set_cpml_off();
+    std::vector<double> tmp(N);
+    for (int f : {LAM, MU, BYC}) {
+      E->get(f, tmp.data());
+      E2->set(f, tmp.data());
+    }
+  }
   float h_l2 = 0.0f;
   double j64 = 0.0;
   auto t_start = std::chrono::steady_clock::now();
@@ -860,11 +914,17 @@ int orc_run(const orc_grid* gg, const orc_survey* sv, int precision, int formula
     for (int it = 0; it < nt; it++) stf[it] = sv->stf[(long)s * nt + it] * w2_src[it];
     const int zs = sv->src[3 * s], xs = sv->src[3 * s + 1], ys = sv->src[3 * s + 2];
 
-    if (with_adj) E->ring_init(band, nt > 1 ? nt - 1 : 1);
+    if (with_adj) E->ring_init(band, K ? K : (nt > 1 ? nt - 1 : 1));
+    const int nseg = K ? (nt - 1 + K - 1) / K : 1;
+    if (K) ckpts.assign(nseg, std::vector<char>());
 
     auto t0 = std::chrono::steady_clock::now();
     for (int it = 0; it <= nt - 2; it++) {
-      if (with_adj) E->ring_save(it);
+      if (K && it % K == 0) {
+        ckpts[it / K].resize(E->state_bytes());
+        E->save_state(ckpts[it / K].data());
+      }
+      if (with_adj && !K) E->ring_save(it);
       E->kernel(0);
       E->add_source(zs, xs, ys, stf[it], +1);
       E->kernel(1);
@@ -927,15 +987,28 @@ int orc_run(const orc_grid* gg, const orc_survey* sv, int precision, int formula
     E->kernel(4);
     E->inject(rec, nrec, nt - 1, rx);
     E->kernel(5);
-    for (int it = nt - 2; it >= 0; it--) {
-      E->kernel(3);
-      E->ring_restore(it, 6, 9);
-      E->add_source(zs, xs, ys, stf[it], -1);
-      E->kernel(2);
-      E->ring_restore(it, 0, 6);
-      E->kernel(4);
-      E->inject(rec, nrec, it, rx);
-      E->kernel(5);
+    for (int seg = nseg - 1; seg >= 0; seg--) {
+      const int a = K ? seg * K : 0, b = K ? std::min(nt - 2, a + K - 1) : nt - 2;
+      if (K) {
+        // rebuild this segment's ring: slot it - a holds the state before step it
+        E2->load_state(ckpts[seg].data());
+        for (int it = a; it <= b; it++) {
+          E2->ring_save_into(E.get(), it - a);
+          E2->kernel(0);
+          E2->add_source(zs, xs, ys, stf[it], +1);
+          E2->kernel(1);
+        }
+      }
+      for (int it = b; it >= a; it--) {
+        E->kernel(3);
+        E->ring_restore(it - a, 6, 9);
+        E->add_source(zs, xs, ys, stf[it], -1);
+        E->kernel(2);
+        E->ring_restore(it - a, 0, 6);
+        E->kernel(4);
+        E->inject(rec, nrec, it, rx);
+        E->kernel(5);
+      }
     }
     t1 = std::chrono::steady_clock::now();
     t_loops += std::chrono::duration<double>(t1 - t0).count();

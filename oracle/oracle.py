@@ -85,6 +85,7 @@ def lib():
         L.orc_fp32_block_sumsq.restype = C.c_float
         L.orc_fp32_block_sumsq.argtypes = [fp, C.c_long]
         L.orc_cpml_profile.argtypes = [C.c_int, C.c_int, C.c_float, C.c_float, C.c_float, fp]
+        L.orc_set_checkpoint.argtypes = [C.c_int]
         L.orc_run.restype = C.c_int
         L.orc_run.argtypes = [C.POINTER(_Grid), C.POINTER(_Survey), C.c_int, C.c_int, C.c_int,
                               C.c_int, fp, fp, fp, fp, dp, dp, dp, fp, dp, dp, dp, dp]
@@ -218,11 +219,14 @@ class Result:
 
 
 def run(g: Grid, shots: Shots, lam, mu, rho, obs=None, calc_id=1, precision="fp32",
-        formulation="reference", cpml_off=False, nthreads=None) -> Result:
+        formulation="reference", cpml_off=False, nthreads=None, checkpoint_every=0) -> Result:
     """Run the cufd pipeline (Src/libCUFD.cu:37-799) for all shots in order.
 
     ``obs`` is a list of per-shot [3][nrec][nt] arrays (required for calc_id 0/1).
     ``rho`` is in the units the op receives (g/cc for FWIFunction).
+    ``checkpoint_every`` > 0 keeps the boundary ring of that many steps only,
+    rebuilt per segment from checkpoints of the forward state (same results
+    bit for bit; for grids whose full ring exceeds host memory).
     """
     if nthreads is not None:
         os.environ["OMP_NUM_THREADS"] = str(nthreads)
@@ -270,6 +274,7 @@ def run(g: Grid, shots: Shots, lam, mu, rho, obs=None, calc_id=1, precision="fp3
         gl = gm = gr = None
         gp = [None, None, None]
     gc = g.c()
+    lib().orc_set_checkpoint(int(checkpoint_every))
     rc = lib().orc_run(C.byref(gc), C.byref(sv), PRECISION[precision], FORMULATION[formulation],
                        calc_id, int(cpml_off), _ptr(lam, C.c_float), _ptr(mu, C.c_float),
                        _ptr(rho, C.c_float), _ptr(obs_flat, C.c_float), _ptr(syn, C.c_double),

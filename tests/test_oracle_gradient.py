@@ -138,3 +138,25 @@ def test_precision_modes_agree(orc):
                               getattr(res["fp64"], k) if k != "res" else res["fp64"].res[0])
                 assert lit[k] <= 1.5 * (e32 + elit), (k, lit[k], e32, elit)
                 assert e32 <= 4.0 * elit, (k, e32, elit)
+
+
+@pytest.mark.parametrize("every", [1, 7, 50])
+def test_checkpointed_oracle_equals_full_ring(orc, every):
+    """The oracle's checkpoint mode (segments replayed from saved forward
+    states to rebuild their part of the boundary ring) gives the full ring's
+    seismograms, residuals and gradients bit for bit; it lets the oracle run
+    the 3000-step production grid, whose full ring exceeds host memory."""
+    from efwi import synth
+    from util import oracle_run
+    w = synth.small(n=24, npml=5, nt=90)
+    obs = [s.astype(np.float32) for s in oracle_run(orc, w, model=w.true, calc_id=2).syn]
+    ring = oracle_run(orc, w, obs=obs, calc_id=1)
+    g = w.grid
+    og = orc.Grid(g.nz, g.nx, g.ny, g.npml, g.nt, g.dz, g.dx, g.dy, g.dt, g.f0)
+    from util import oracle_shots
+    ck = orc.run(og, oracle_shots(orc, w.shots), *w.model, obs=obs, calc_id=1, checkpoint_every=every)
+    assert np.abs(ring.glam).max() > 0
+    assert ck.misfit64 == ring.misfit64
+    assert np.array_equal(ck.syn[0], ring.syn[0]) and np.array_equal(ck.res[0], ring.res[0])
+    for a, b in ((ck.glam, ring.glam), (ck.gmu, ring.gmu), (ck.grho, ring.grho)):
+        assert np.array_equal(a, b)
