@@ -21,6 +21,7 @@
 
 #include <algorithm>
 #include <cstdint>
+#include <mutex>
 
 #include "../../include/efwi_nn.h"
 
@@ -461,13 +462,60 @@ long efwi_nn_wgrad_scratch(int mc, int kc, int nz, int nx, int ny, int per) {
   return chunks * 27L * mc * kc;
 }
 
+static bool wgrad_args_ok(const uint16_t* dpre, int ldp, int mc, const uint16_t* src, int lds, int kc, float* dw,
+                          int nz, int nx, int ny, int per) {
+  return dpre && src && dw && mc % 16 == 0 && kc % 16 == 0 && mc > 0 && kc > 0 && kc <= 96 && per > 0 &&
+         ldp % 16 == 0 && lds % 16 == 0 && ldp >= mc && lds >= kc && nz > 0 && nx > 0 && ny > 0;
+}
+static int wgrad_launch(const uint16_t* dpre, int ldp, int mc, const uint16_t* src, int lds, int kc, float* scratch,
+                        float* dw, int nz, int nx, int ny, int per, hipStream_t s);
+
 int efwi_nn_wgrad(const uint16_t* dpre, int ldp, int mc, const uint16_t* src, int lds, int kc, float* scratch,
                   long scratch_floats, float* dw, int nz, int nx, int ny, int per, void* stream) {
-  if (!dpre || !src || !scratch || !dw || mc % 16 || kc % 16 || mc <= 0 || kc <= 0 || kc > 96 || per <= 0 ||
-      ldp % 16 || lds % 16 || ldp < mc || lds < kc)
-    return EFWI_NN_ERR_ARG;
+  if (!scratch || !wgrad_args_ok(dpre, ldp, mc, src, lds, kc, dw, nz, nx, ny, per)) return EFWI_NN_ERR_ARG;
   if (efwi_nn_wgrad_scratch(mc, kc, nz, nx, ny, per) > scratch_floats) return EFWI_NN_ERR_ARG;
-  hipStream_t s = (hipStream_t)stream;
+  return wgrad_launch(dpre, ldp, mc, src, lds, kc, scratch, dw, nz, nx, ny, per, (hipStream_t)stream);
+}
+
+// The partial sums of efwi_nn_wgrad_noalloc: one array per GPU, for the
+// process's lifetime. Host code uses the device address that
+// hipGetSymbolAddress returns for the current device (kept per device), never
+// the array's name.
+static __device__ __align__(256) float scratch[EFWI_NN_WGRAD_STATIC_FLOATS];
+static constexpr int kMaxDevices = 64;
+static float* g_scratch_dev[kMaxDevices];
+static std::mutex g_scratch_mu;
+
+static float* static_scratch(int* err) {
+  int dev = 0;
+  if (hipGetDevice(&dev) != hipSuccess || dev < 0 || dev >= kMaxDevices) {
+    *err = EFWI_NN_ERR_HIP;
+    return nullptr;
+  }
+  std::lock_guard<std::mutex> lock(g_scratch_mu);
+  if (!g_scratch_dev[dev]) {
+    void* ptr = nullptr;
+    if (hipGetSymbolAddress(&ptr, HIP_SYMBOL(scratch)) != hipSuccess || !ptr) {
+      *err = EFWI_NN_ERR_HIP;
+      return nullptr;
+    }
+    g_scratch_dev[dev] = static_cast<float*>(ptr);
+  }
+  return g_scratch_dev[dev];
+}
+
+int efwi_nn_wgrad_noalloc(const uint16_t* dpre, int ldp, int mc, const uint16_t* src, int lds, int kc, float* dw,
+                          int nz, int nx, int ny, int per, void* stream) {
+  if (!wgrad_args_ok(dpre, ldp, mc, src, lds, kc, dw, nz, nx, ny, per)) return EFWI_NN_ERR_ARG;
+  if (efwi_nn_wgrad_scratch(mc, kc, nz, nx, ny, per) > EFWI_NN_WGRAD_STATIC_FLOATS) return EFWI_NN_ERR_SCRATCH;
+  int err = 0;
+  float* ptr = static_scratch(&err);
+  if (!ptr) return err;
+  return wgrad_launch(dpre, ldp, mc, src, lds, kc, ptr, dw, nz, nx, ny, per, (hipStream_t)stream);
+}
+
+static int wgrad_launch(const uint16_t* dpre, int ldp, int mc, const uint16_t* src, int lds, int kc, float* scratch,
+                        float* dw, int nz, int nx, int ny, int per, hipStream_t s) {
   const Vol V{nz, nx, ny};
   const long tiles = (long)nz * nx * ((ny + 15) / 16);
   const int chunks = (int)((tiles + per - 1) / per);

@@ -80,6 +80,8 @@ SIGNATURES = {
     "efwi_nn_wgrad_scratch": (C.c_long, [C.c_int, C.c_int, C.c_int, C.c_int, C.c_int, C.c_int]),
     "efwi_nn_wgrad": (C.c_int, [_vp, C.c_int, C.c_int, _vp, C.c_int, C.c_int, _vp, C.c_long, _vp,
                                 C.c_int, C.c_int, C.c_int, C.c_int, _vp]),
+    "efwi_nn_wgrad_noalloc": (C.c_int, [_vp, C.c_int, C.c_int, _vp, C.c_int, C.c_int, _vp, C.c_int, C.c_int,
+                                        C.c_int, C.c_int, _vp]),
     "efwi_nn_chan_sum": (C.c_int, [_vp, C.c_int, C.c_int, C.c_long, _vp, _vp]),
     "efwi_nn_pool2": (C.c_int, [_vp, C.c_int, _vp, C.c_int, C.c_int, C.c_int, C.c_int, C.c_int, _vp]),
     "efwi_nn_pool2_bwd": (C.c_int, [_vp, C.c_int, _vp, C.c_int, C.c_int, C.c_int, C.c_int, C.c_int, C.c_int,

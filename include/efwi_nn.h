@@ -24,6 +24,7 @@ extern "C" {
 #define EFWI_NN_OK 0
 #define EFWI_NN_ERR_ARG 1 /* invalid sizes, strides or pointers */
 #define EFWI_NN_ERR_HIP 2 /* a kernel launch failed */
+#define EFWI_NN_ERR_SCRATCH 3 /* efwi_nn_wgrad_noalloc: partial sums beyond EFWI_NN_WGRAD_STATIC_FLOATS */
 
 /* 3x3x3 convolution with zero padding, BF16 in and out, FP32 accumulation on
  * the matrix cores (v_mfma_f32_16x16x16_bf16).
@@ -46,6 +47,19 @@ int efwi_nn_conv3(const uint16_t* src, int lds, int kc, const uint16_t* w, const
 long efwi_nn_wgrad_scratch(int mc, int kc, int nz, int nx, int ny, int per);
 int efwi_nn_wgrad(const uint16_t* dpre, int ldp, int mc, const uint16_t* src, int lds, int kc, float* scratch,
                   long scratch_floats, float* dw, int nz, int nx, int ny, int per, void* stream);
+
+/* efwi_nn_wgrad without a workspace argument and without any allocation:
+ * the partial sums live in a static device array of
+ * EFWI_NN_WGRAD_STATIC_FLOATS floats (16 MiB) that each GPU holds for the
+ * process's lifetime. Calls whose efwi_nn_wgrad_scratch(mc, kc, nz, nx, ny,
+ * per) exceeds it return EFWI_NN_ERR_SCRATCH (raise `per`). Every call on a
+ * GPU uses that GPU's one array: calls that overlap on one GPU (concurrent
+ * streams or threads) race. Results equal efwi_nn_wgrad's bit for bit. No
+ * counterpart in the reference; for callers that cannot allocate device
+ * memory (code under graph capture, training loops that forbid it). */
+#define EFWI_NN_WGRAD_STATIC_FLOATS (4L << 20)
+int efwi_nn_wgrad_noalloc(const uint16_t* dpre, int ldp, int mc, const uint16_t* src, int lds, int kc, float* dw,
+                          int nz, int nx, int ny, int per, void* stream);
 
 /* out[c] = sum_v x[v][c] (bias gradients), c divides 256. */
 int efwi_nn_chan_sum(const uint16_t* x, int ld, int c, long nvox, float* out, void* stream);
