@@ -104,9 +104,11 @@ def cpu_baseline(w, nsteps):
 
 
 def bench_batch(args, w):
-    """Batched shots (cfg3): all shots in one run_batch (persistent (shot, tile)
-    launches, one captured hipGraph per time step); the same shots one after
-    another through run_shot for comparison."""
+    """Batched shots (cfg3): all shots through one run_batch call, timed in
+    each of its modes (the automatic choice; persistent (shot, tile) launches
+    over groups of shots with one captured hipGraph per time step; one shot
+    after another on the single-shot kernels), the best one reported; the same
+    shots through run_shot one after another for comparison."""
     import torch
     from efwi.propagator import Propagator
     g = w.grid
@@ -115,35 +117,38 @@ def bench_batch(args, w):
           f"{max(s.nrec for s in w.shots)}", file=sys.stderr, flush=True)
     P = Propagator(g, 0)
     P.set_model(*(torch.from_numpy(a) for a in w.true))
+    P.batch_group(-1)
     _, obs = P.run_batch(w.shots, 2)
     P.set_model(*(torch.from_numpy(a) for a in w.model))
     if not args.no_autotune:
         tune = P.autotune(reps=3)
         print("[bench] autotune:", {k: min(v, key=v.get) for k, v in tune.items()}, file=sys.stderr, flush=True)
-    # autotune the batch: launch shape, shots per batch and queue split, one
-    # timed pass each (the batched launches run flat tiles)
     tuned = P.tiles()
     kinds = ("stress_fwd", "velocity_fwd", "stress_bwd", "velocity_bwd")
 
     def timed(cfg):
-        tile, size, take, static_pct = cfg
+        group, tile, take = cfg
         for k in kinds:
-            P.set_tile(k, *tile)
-        P.batch_config(take, static_pct)
+            P.set_tile(k, *(tuned[k] if tile is None else tile))
+        P.batch_group(group)
+        P.batch_config(take, 0)
         torch.cuda.synchronize()
         t0 = time.perf_counter()
-        for k in range(0, n, size):
-            P.run_batch(w.shots[k:k + size], 1, obs[k:k + size])
+        P.run_batch(w.shots, 1, obs)
         torch.cuda.synchronize()
         return time.perf_counter() - t0
+
+    def label(c):
+        mode = {0: "auto", -1: "single-shot kernels"}.get(c[0], f"persistent groups of {c[0]}")
+        return mode + ("" if c[1] is None else f", tile {c[1][0]}x{c[1][1]}, take {c[2]}")
+
     trials = {}
-    cands = [(tile, n, take, 0) for tile in ((2, 1), (4, 1), (2, 2), (4, 2)) for take in (1, 2, 4)]
-    cands += [((4, 2), n, 2, 50), ((4, 2), max(1, n // 4), 2, 0), ((4, 2), 1, 2, 90)]
+    cands = [(0, None, 2), (-1, None, 2)] + [(grp, tile, take) for grp in (n, max(1, n // 4))
+                                              for tile in ((2, 1), (2, 2)) for take in (1, 4)]
     for cfg in cands:
         timed(cfg)  # warm-up (graph capture)
         trials[cfg] = timed(cfg)
-        print(f"[bench] tile {cfg[0]} batch {cfg[1]} take {cfg[2]} static {cfg[3]}%: "
-              f"{n / trials[cfg]:.3f} shots/s", file=sys.stderr, flush=True)
+        print(f"[bench] {label(cfg)}: {n / trials[cfg]:.3f} shots/s", file=sys.stderr, flush=True)
     best = min(trials, key=trials.get)
     for _ in range(args.warmup):
         timed(best)
@@ -154,7 +159,8 @@ def bench_batch(args, w):
     dt_step = (time.perf_counter() - t0) / args.steps
     for k, tile in tuned.items():
         P.set_tile(k, *tile)
-    # the same shots one by one through run_shot (one timed pass)
+    # the same shots one by one through run_shot (one timed pass, profiled
+    # afterwards for the per-kernel roofline of the single-shot kernels)
     torch.cuda.synchronize()
     t0 = time.perf_counter()
     P.zero_gradient()
@@ -162,6 +168,33 @@ def bench_batch(args, w):
         P.run_shot(s, 1, o)
     torch.cuda.synchronize()
     dt_seq = time.perf_counter() - t0
+    roof, kernels = None, {}
+    if not args.no_profile:
+        P.profile(True)
+        P.profile_read(reset=True)
+        P.zero_gradient()
+        for s, o in zip(w.shots, obs):
+            P.run_shot(s, 1, o)
+        prof = P.profile_read(reset=True)
+        P.profile(False)
+        for k, (ms, cnt) in prof.items():
+            if k in ALG_FLOATS and cnt:
+                kernels[k] = {"avg_us": round(ms / cnt * 1e3, 2), "launches": cnt,
+                              "alg_GBps": round(alg_bytes_per_launch(k, g) / (ms / cnt * 1e-3) / 1e9, 1)}
+        if kernels:
+            dom = max(kernels, key=lambda k: prof[k][0])
+            ach = kernels[dom]["alg_GBps"]
+            roof = {"bound": "hbm", "kernel": dom + " (single-shot kernels, per shot)", "achieved": ach,
+                    "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": round(ach / HBM_PEAK_GBS, 4), "traffic": None,
+                    "alg_bytes": round(alg_bytes_per_launch(dom, g))}
+    cpu = None
+    if not args.no_cpu_baseline:
+        print("[bench] cpu baseline ...", file=sys.stderr, flush=True)
+        c = cpu_baseline(w, min(args.cpu_steps, g.nt))
+        # the oracle's cell-update rate as shots of this workload per second
+        cpu = dict(c, value=round(c["value"] * 1e9 / (g.ncells * g.nt), 6), unit="shots/s",
+                   gcell_updates_per_s=c["value"])
+    pipe_gbs = alg_bytes_pipeline(g) * n / dt_step / 1e9
     out = {
         "metric": "batched fwd+adj+grad shots/s",
         "value": round(n / dt_step, 4),
@@ -176,14 +209,16 @@ def bench_batch(args, w):
         "dtype": "fp32",
         "data": "synthetic",
         "config": {"workload": f"{w.name}: {g.nz - 2 * g.npml}x{g.nx - 2 * g.npml}x{g.ny - 2 * g.npml} physical + "
-                               f"{g.npml} PML cells, padded {g.nz}x{g.nx}x{g.ny}, nt={g.nt}, {n} shots in one batch, "
-                               f"{sum(s.nrec for s in w.shots)} receivers in all, graph per time step, "
-                               f"tile {best[0][0]}x{best[0][1]}, batches of {best[1]} shots, queue take {best[2]}, "
-                               f"static share {best[3]}%",
-                   "batch_trials_shots_per_s": {f"t{c[0][0]}x{c[0][1]}/b{c[1]}/k{c[2]}/s{c[3]}": round(n / t, 4)
-                                                for c, t in trials.items()},
+                               f"{g.npml} PML cells, padded {g.nz}x{g.nx}x{g.ny}, nt={g.nt}, {n} shots in one "
+                               f"run_batch call, {sum(s.nrec for s in w.shots)} receivers in all, {label(best)}",
+                   "batch_trials_shots_per_s": {label(c): round(n / t, 4) for c, t in trials.items()},
                    "gcell_updates_per_s": round(g.ncells * g.nt * n / dt_step / 1e9, 3),
-                   "sequential_shots_per_s": round(n / dt_seq, 4)},
+                   "pipeline_alg_GBps": round(pipe_gbs, 1),
+                   "pipeline_hbm_frac": round(pipe_gbs / HBM_PEAK_GBS, 4),
+                   "sequential_shots_per_s": round(n / dt_seq, 4),
+                   "kernels": kernels},
+        "roofline": roof,
+        "cpu_baseline": cpu,
     }
     print(json.dumps(out), flush=True)
     P.close()

@@ -22,7 +22,37 @@ void Ctx::release_batch() {
   batch.cs = nullptr;
 }
 
-int Ctx::ensure_batch(int n) {
+// Shots per persistent launch, or 0: one shot after another through the
+// single-shot kernels. The persistent launches pay for their work queue (an
+// atomic claim per few tiles) and, over many shots, lose the Infinity Cache
+// reuse between consecutive kernels that a single shot has; they win where a
+// shot alone leaves the device idle. Automatic (group = 0): where one shot
+// already gives kBatchFill workgroup tiles per compute unit, the single-shot
+// kernels; otherwise groups of enough shots to reach that many tiles.
+int Ctx::batch_group_size(int n) const {
+  if (batch.group > 0) return std::min(batch.group, n);
+  if (batch.group < 0) return 0;
+  const Tile t = tiles[K_STRESS_FWD].zdepth >= 16 ? Tile{4, 1} : tiles[K_STRESS_FWD];
+  const long per_shot = make_sweep(2, cfg.nz - 3, 2, cfg.nx - 3, 2, cfg.ny - 3, t).blocks;
+  const long want = (long)kBatchFill * std::max(batch.cus, 1);
+  if (per_shot >= want) return 0;
+  return (int)std::max(1L, std::min((long)n, (want + per_shot - 1) / std::max(per_shot, 1L)));
+}
+
+// Point the context's state (wavefields, C-PML arrays, model and gradients)
+// at the field groups g.
+void Ctx::bind_state(const Groups& g) {
+  const size_t fsf = G.fs / sizeof(float);
+  F = g;
+  wave_of(F.w, fsf, W);
+  wave_of(F.a, fsf, A);
+  psi_of(F.p, fsf, P);
+  lam = F.m + M_LAM * fsf; mu = F.m + M_MU * fsf; byc = F.m + M_BYC * fsf;
+  Gr.glam = F.m + M_GLAM * fsf; Gr.gmu = F.m + M_GMU * fsf; Gr.grho = F.m + M_GRHO * fsf;
+  M.lam = lam; M.mu = mu; M.byc = byc;
+}
+
+int Ctx::ensure_batch(int n, int nring) {
   int rc;
   if (!batch.cs) {
     HIP_TRY(hipStreamCreateWithFlags(&batch.cs, hipStreamNonBlocking));
@@ -30,20 +60,23 @@ int Ctx::ensure_batch(int n) {
     if ((rc = alloc((void**)&batch.it_dev, sizeof(int) * 64))) return rc;
     batch.cus = device_cus(cfg.device);
   }
+  const size_t ring_floats = (size_t)B.n * 9 * (size_t)(cfg.nt - 1);
+  // rings: one per shot of a group (groups run one after another)
+  while ((int)batch.ring.size() < std::min(n, nring)) {
+    float* ring_k = nullptr;
+    if ((rc = alloc((void**)&ring_k, sizeof(float) * ring_floats))) return rc;
+    batch.ring.push_back(ring_k);
+  }
   if (n <= batch.n) return EFWI_OK;
   const size_t fs = G.fs;
-  const size_t ring_floats = (size_t)B.n * 9 * (size_t)(cfg.nt - 1);
   for (int k = batch.n; k < n; k++) {
     Groups g{};
     if ((rc = alloc((void**)&g.w, fs * F_WAVE_COUNT)) || (rc = alloc((void**)&g.a, fs * F_WAVE_COUNT)) ||
         (rc = alloc((void**)&g.p, fs * P_PSI_COUNT)) || (rc = alloc((void**)&g.m, fs * M_COUNT)))
       return rc;
     batch.F.push_back(g);
-    float *ring_k = nullptr, *stf_k = nullptr;
-    if ((rc = alloc((void**)&ring_k, sizeof(float) * ring_floats)) ||
-        (rc = alloc((void**)&stf_k, sizeof(float) * cfg.nt)))
-      return rc;
-    batch.ring.push_back(ring_k);
+    float* stf_k = nullptr;
+    if ((rc = alloc((void**)&stf_k, sizeof(float) * cfg.nt))) return rc;
     batch.stf.push_back(stf_k);
     batch.syn_t.push_back(nullptr);
     batch.obs_t.push_back(nullptr);
@@ -68,7 +101,13 @@ int Ctx::run_batch(const efwi_shot* shots, int n, int calc_id, const float* cons
   if (cfg.memory_mode == 1) return fail(EFWI_ERR_ARG, "batched shots keep the full ring (memory_mode 0)");
   HIP_TRY(hipSetDevice(cfg.device));
   HIP_TRY(hipStreamSynchronize(s));
-  int rc = ensure_batch(n);
+  if (!batch.cs) {
+    int rc0 = ensure_batch(0, 0);
+    if (rc0) return rc0;
+  }
+  const int gsz0 = batch_group_size(n);
+  const int gsz = gsz0 > 0 ? gsz0 : 1;
+  int rc = ensure_batch(n, calc_id == EFWI_CALC_GRADIENT && gsz0 > 0 ? gsz : 0);
   if (rc) return rc;
   const int nt = cfg.nt, form = cfg.formulation, p = cfg.npml;
   const bool grad = calc_id == EFWI_CALC_GRADIENT;
@@ -76,7 +115,6 @@ int Ctx::run_batch(const efwi_shot* shots, int n, int calc_id, const float* cons
   const size_t fsf = G.fs / sizeof(float);
   // ---- shot table
   std::vector<ShotDev> tab(n);
-  int max_nrec = 0;
   for (int k = 0; k < n; k++) {
     const efwi_shot* sh = shots + k;
     if (!sh->stf) return fail(EFWI_ERR_ARG, "shot " + std::to_string(k) + ": stf is NULL");
@@ -96,7 +134,6 @@ int Ctx::run_batch(const efwi_shot* shots, int n, int calc_id, const float* cons
     }
     const int nrec_k = sh->nrec;
     batch.nrec[k] = nrec_k;
-    max_nrec = std::max(max_nrec, nrec_k);
     std::vector<float> stf(nt);
     for (int it = 0; it < nt; it++) stf[it] = sh->stf[it] * taper_plain_w2(it, nt, cfg.dt, 0.001f);
     HIP_TRY(hipMemcpy(batch.stf[k], stf.data(), sizeof(float) * nt, hipMemcpyHostToDevice));
@@ -123,7 +160,7 @@ int Ctx::run_batch(const efwi_shot* shots, int n, int calc_id, const float* cons
     d.nrec = nrec_k;
     d.syn_t = batch.syn_t[k];
     d.res_t = batch.res_t[k];
-    d.ring = batch.ring[k];
+    d.ring = calc_id == EFWI_CALC_GRADIENT && gsz0 > 0 ? batch.ring[k % gsz] : nullptr;
   }
   HIP_TRY(hipMemcpy(batch.shots_dev, tab.data(), sizeof(ShotDev) * n, hipMemcpyHostToDevice));
   // ---- state: the model into every shot's model group, zero wavefields and gradients
@@ -137,7 +174,33 @@ int Ctx::run_batch(const efwi_shot* shots, int n, int calc_id, const float* cons
     launch_fill(batch.syn_t[k], (long)nt * 3 * std::max(batch.nrec[k], 1), 0.0f, q);
   }
   HIP_TRY(hipGetLastError());
-  BatchLaunch L0{batch.shots_dev, n, batch.counters, batch.it_dev, batch.cus, batch.take, batch.static_pct, 0};
+  if (gsz0 == 0) {
+    // one shot after another on the single-shot kernels, each on its own state
+    // (its model copy and gradients in batch.F[k]); the context's state, fused
+    // step and illumination are left as they were
+    const Groups own = F;
+    const Tile fused = tiles[K_FWD_FUSED];
+    const bool illum0 = illum_on;
+    tiles[K_FWD_FUSED].rows = 0;
+    illum_on = false;
+    for (int k = 0; k < n && !rc; k++) {
+      bind_state(batch.F[k]);
+      double j = 0.0;
+      rc = run_shot(shots + k, calc_id, obs ? obs[k] : nullptr, syn ? syn[k] : nullptr, nullptr, &j, q);
+      if (misfits) misfits[k] = calc_id == EFWI_CALC_SYNTHETIC ? 0.0 : j;
+    }
+    bind_state(own);
+    tiles[K_FWD_FUSED] = fused;
+    illum_on = illum0;
+    if (rc) return rc;
+    HIP_TRY(hipStreamSynchronize(q));
+    return EFWI_OK;
+  }
+  for (int g0 = 0; g0 < n; g0 += gsz) {
+  const int gn = std::min(gsz, n - g0);
+  int gmax_nrec = 0;
+  for (int k = g0; k < g0 + gn; k++) gmax_nrec = std::max(gmax_nrec, batch.nrec[k]);
+  BatchLaunch L0{batch.shots_dev + g0, gn, batch.counters, batch.it_dev, batch.cus, batch.take, batch.static_pct, 0};
   BatchLaunch L1 = L0;
   L1.counters = batch.counters + 8 * kQueueStride;
   L1.reverse = 1;
@@ -145,7 +208,8 @@ int Ctx::run_batch(const efwi_shot* shots, int n, int calc_id, const float* cons
                                 "," + std::to_string(tiles[K_VELOCITY_FWD].rows) + "x" + std::to_string(tiles[K_VELOCITY_FWD].zdepth) +
                                 "," + std::to_string(tiles[K_VELOCITY_BWD].rows) + "x" + std::to_string(tiles[K_VELOCITY_BWD].zdepth) +
                                 "," + std::to_string(tiles[K_STRESS_BWD].rows) + "x" + std::to_string(tiles[K_STRESS_BWD].zdepth);
-  const std::string key = std::to_string(n) + "|" + std::to_string(max_nrec) + "|" + tiles_key + "|" +
+  const std::string key = std::to_string(n) + "|" + std::to_string(g0) + "+" + std::to_string(gn) + "|" +
+                          std::to_string(gmax_nrec) + "|" + tiles_key + "|" +
                           std::to_string(batch.take) + "|" + std::to_string(batch.static_pct) + "|";
   // ---- forward loop (Src/libCUFD.cu:312-354 per shot): ring save of the
   // state before step it, stress + source, velocity, record of step it + 1.
@@ -159,12 +223,12 @@ int Ctx::run_batch(const efwi_shot* shots, int n, int calc_id, const float* cons
   BatchRing restore = save;
   restore.mode = 2;
   auto fwd_step = [&](hipStream_t st) {
-    if (grad && !fold) launch_ring_batch(B, G, batch.shots_dev, n, batch.it_dev, 0, 9, 0, st);
+    if (grad && !fold) launch_ring_batch(B, G, batch.shots_dev + g0, gn, batch.it_dev, 0, 9, 0, st);
     launch_fill(reinterpret_cast<float*>(batch.counters), 8 * kQueueStride, 0.0f, st);
     launch_stress_fwd_batch(tiles[K_STRESS_FWD], G, C, L0, st, fold ? &save : nullptr);
     launch_fill(reinterpret_cast<float*>(batch.counters + 8 * kQueueStride), 8 * kQueueStride, 0.0f, st);
     launch_velocity_fwd_batch(tiles[K_VELOCITY_FWD], form, G, C, L1, st, fold ? &save : nullptr);
-    launch_record_batch(batch.shots_dev, n, max_nrec, batch.it_dev, 1, st);
+    launch_record_batch(batch.shots_dev + g0, gn, gmax_nrec, batch.it_dev, 1, st);
     launch_step_add(batch.it_dev, 1, st);
   };
   auto run_steps = [&](hipGraphExec_t* exec, std::string* have, const std::string& want, int count,
@@ -190,14 +254,14 @@ int Ctx::run_batch(const efwi_shot* shots, int n, int calc_id, const float* cons
   };
   const int zero = 0;
   HIP_TRY(hipMemcpyAsync(batch.it_dev, &zero, sizeof(int), hipMemcpyHostToDevice, q));
-  if (fold) launch_ring_batch(B, G, batch.shots_dev, n, batch.it_dev, 0, 9, 0, q);
+  if (fold) launch_ring_batch(B, G, batch.shots_dev + g0, gn, batch.it_dev, 0, 9, 0, q);
   if ((rc = run_steps(&batch.gf, &batch.key_f, key + (grad ? "g" : "f"), nt - 1, fwd_step))) return rc;
   HIP_TRY(hipGetLastError());
-  for (int k = 0; k < n; k++)
+  for (int k = g0; k < g0 + gn; k++)
     if (syn && syn[k] && batch.nrec[k] > 0) launch_from_time_major(batch.syn_t[k], syn[k], batch.nrec[k], nt, q);
   // ---- misfit and adjoint source of every shot (Src/libCUFD.cu:371-441)
   if (calc_id != EFWI_CALC_SYNTHETIC) {
-    for (int k = 0; k < n; k++) {
+    for (int k = g0; k < g0 + gn; k++) {
       double j = 0.0;
       if (batch.nrec[k] > 0) {
         launch_to_time_major(obs[k], batch.obs_t[k], batch.nrec[k], nt, q);
@@ -212,7 +276,7 @@ int Ctx::run_batch(const efwi_shot* shots, int n, int calc_id, const float* cons
   }
   // ---- backward loop (Src/libCUFD.cu:469-613 per shot), as Ctx::run_shot
   if (grad) {
-    for (int k = 0; k < n; k++) {
+    for (int k = g0; k < g0 + gn; k++) {
       const Groups& g = batch.F[k];
       launch_fill(g.a, (long)fsf * F_WAVE_COUNT, 0.0f, q);
       launch_fill(g.p, (long)fsf * P_PSI_COUNT, 0.0f, q);
@@ -225,11 +289,11 @@ int Ctx::run_batch(const efwi_shot* shots, int n, int calc_id, const float* cons
     auto bwd_step = [&](hipStream_t st) {
       launch_fill(reinterpret_cast<float*>(batch.counters), 8 * kQueueStride, 0.0f, st);
       launch_velocity_bwd_batch(tiles[K_VELOCITY_BWD], form, G, C, L0, st, fold ? &restore : nullptr);
-      if (!fold) launch_ring_batch(B, G, batch.shots_dev, n, batch.it_dev, 0, 3, 1, st);
-      launch_inject_batch(batch.shots_dev, n, max_nrec, batch.it_dev, st);
+      if (!fold) launch_ring_batch(B, G, batch.shots_dev + g0, gn, batch.it_dev, 0, 3, 1, st);
+      launch_inject_batch(batch.shots_dev + g0, gn, gmax_nrec, batch.it_dev, st);
       launch_fill(reinterpret_cast<float*>(batch.counters + 8 * kQueueStride), 8 * kQueueStride, 0.0f, st);
       launch_stress_bwd_batch(tiles[K_STRESS_BWD], G, C, L1, st, fold ? &restore : nullptr);
-      if (!fold) launch_ring_batch(B, G, batch.shots_dev, n, batch.it_dev, 3, 9, 1, st);
+      if (!fold) launch_ring_batch(B, G, batch.shots_dev + g0, gn, batch.it_dev, 3, 9, 1, st);
       launch_step_add(batch.it_dev, -1, st);
     };
     const int start = nt - 2;
@@ -237,6 +301,7 @@ int Ctx::run_batch(const efwi_shot* shots, int n, int calc_id, const float* cons
     if ((rc = run_steps(&batch.gb, &batch.key_b, key + "b", nt - 1, bwd_step))) return rc;
     HIP_TRY(hipGetLastError());
   }
+  }  // groups of shots
   HIP_TRY(hipStreamSynchronize(q));
   return EFWI_OK;
 }
@@ -275,6 +340,13 @@ int efwi_batch_config(efwi_ctx* ctx, int take, int static_pct) {
   if (static_pct < 0 || static_pct > 100) return fail(EFWI_ERR_ARG, "static_pct must be in [0, 100]");
   reinterpret_cast<Ctx*>(ctx)->batch.take = take;
   reinterpret_cast<Ctx*>(ctx)->batch.static_pct = static_pct;
+  return EFWI_OK;
+}
+
+int efwi_batch_group(efwi_ctx* ctx, int shots_per_launch) {
+  if (!ctx) return fail(EFWI_ERR_ARG, "NULL context");
+  if (shots_per_launch < -1) return fail(EFWI_ERR_ARG, "shots_per_launch must be >= -1");
+  reinterpret_cast<Ctx*>(ctx)->batch.group = shots_per_launch;
   return EFWI_OK;
 }
 

@@ -62,6 +62,7 @@ struct BatchState {
   int cus = 0;
   int take = 2;        // queue items per atomic (efwi_batch_config)
   int static_pct = 0;  // static share of the queues, percent
+  int group = 0;       // shots per persistent launch (efwi_batch_group), 0 = automatic
 };
 
 struct Ctx {
@@ -144,7 +145,10 @@ struct Ctx {
   int autotune(int reps, hipStream_t s, double* report);
   // batched shots (efwi_batch.cpp)
   BatchState batch;
-  int ensure_batch(int n);
+  static constexpr int kBatchFill = 16;  // automatic groups: workgroup tiles per CU and launch
+  int batch_group_size(int n) const;
+  void bind_state(const Groups& g);
+  int ensure_batch(int n, int nring);
   int run_batch(const efwi_shot* shots, int n, int calc_id, const float* const* obs, float* const* syn,
                 double* misfits, int use_graph, hipStream_t s);
   int batch_gradient(int shot, float* gl, float* gm, float* gr, int on_host, hipStream_t s);

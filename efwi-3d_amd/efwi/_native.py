@@ -59,6 +59,7 @@ SIGNATURES = {
                                  C.POINTER(C.c_void_p), _dp, C.c_int, _vp]),
     "efwi_batch_gradient": (C.c_int, [_vp, C.c_int, _vp, _vp, _vp, C.c_int, _vp]),
     "efwi_batch_config": (C.c_int, [_vp, C.c_int, C.c_int]),
+    "efwi_batch_group": (C.c_int, [_vp, C.c_int]),
     "efwi_device_bytes": (C.c_longlong, [_vp]),
     "efwi_dot64": (C.c_int, [_vp, _vp, C.c_longlong, _dp, _vp]),
     "efwi_set_tile": (C.c_int, [_vp, C.c_int, C.c_int, C.c_int]),

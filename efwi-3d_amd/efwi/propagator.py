@@ -271,6 +271,13 @@ class Propagator:
         misfits = None if calc_id == N.CALC_SYNTHETIC else [mis[k] for k in range(n)]
         return misfits, (syns if (want_syn or calc_id == N.CALC_SYNTHETIC) else None)
 
+    def batch_group(self, shots_per_launch: int = 0):
+        """How run_batch runs: groups of this many shots per persistent launch,
+        -1 for one shot after another on the single-shot kernels, 0 for the
+        automatic choice (single-shot kernels where one shot fills the
+        device). Results do not depend on it."""
+        N.check(N.lib().efwi_batch_group(self._h, int(shots_per_launch)), "efwi_batch_group")
+
     def batch_config(self, take: int = 2, static_pct: int = 0):
         """Work distribution of the batched launches: each persistent workgroup
         runs static_pct percent of an even share of its XCD's queue, then

@@ -53,22 +53,34 @@ def _p(t, offset_elems=0):
     return C.c_void_p(t.data_ptr() + offset_elems * t.element_size())
 
 
+def init_params(seed: int = 0):
+    """Initial FP32 parameters (CPU tensors) from a seeded generator: He-normal
+    3x3x3 weights [27][cout][cin] (the padded input channels of enc1a zero),
+    zero biases, a small head (the initial scaling exp(o) is close to 1, the
+    plain gradient). Returns (w, b, wh, bh)."""
+    g = torch.Generator().manual_seed(seed)
+    w, b = {}, {}
+    for name, cin, cout, _ in LAYERS:
+        fan_in = 27 * (IN_CHANNELS if name == "enc1a" else cin)
+        wt = torch.randn(27, cout, cin, generator=g) * math.sqrt(2.0 / fan_in)
+        if name == "enc1a":
+            wt[:, :, IN_CHANNELS:] = 0.0
+        w[name] = wt.contiguous()
+        b[name] = torch.zeros(cout)
+    wh = torch.randn(16, generator=g) * 0.01
+    return w, b, wh, torch.zeros(1)
+
+
 class UNet3D:
     def __init__(self, device: int = 0, seed: int = 0):
         self.dev = torch.device(f"cuda:{device}")
         self.lib = N.lib()
-        g = torch.Generator().manual_seed(seed)
-        self.w, self.b, self.w16 = {}, {}, {}
-        for name, cin, cout, _ in LAYERS:
-            fan_in = 27 * (IN_CHANNELS if name == "enc1a" else cin)
-            w = torch.randn(27, cout, cin, generator=g) * math.sqrt(2.0 / fan_in)
-            if name == "enc1a":
-                w[:, :, IN_CHANNELS:] = 0.0
-            self.w[name] = w.to(self.dev).contiguous()
-            self.b[name] = torch.zeros(cout, device=self.dev)
-        # small head: the initial scaling exp(o) is close to 1 (the plain gradient)
-        self.wh = (torch.randn(16, generator=g) * 0.01).to(self.dev)
-        self.bh = torch.zeros(1, device=self.dev)
+        w, b, wh, bh = init_params(seed)
+        self.w = {k: v.to(self.dev) for k, v in w.items()}
+        self.b = {k: v.to(self.dev) for k, v in b.items()}
+        self.w16 = {}
+        self.wh = wh.to(self.dev)
+        self.bh = bh.to(self.dev)
         self._refresh_bf16()
         self.m1 = {k: torch.zeros_like(v) for k, v in self._params().items()}
         self.m2 = {k: torch.zeros_like(v) for k, v in self._params().items()}

@@ -177,6 +177,17 @@ int efwi_run_batch(efwi_ctx* ctx, const efwi_shot* shots, int nshots, int calc_i
  * `take` tiles per atomic (1..4096; lowered for small batches). Defaults:
  * take 2, static_pct 0. Results do not depend on either. */
 int efwi_batch_config(efwi_ctx* ctx, int take, int static_pct);
+/* How efwi_run_batch runs its shots. shots_per_launch > 0: groups of that
+ * many shots, one group after another, each group one persistent launch per
+ * stencil step over its (shot, tile) queue with its own captured graphs
+ * (gradient batches keep a boundary ring per shot of a group only); -1: one
+ * shot after another through the single-shot kernels (each shot on its own
+ * state, as efwi_run_shot runs it); 0 (default): automatic, the single-shot
+ * kernels where one shot already gives 16 workgroup tiles per compute unit
+ * (the persistent queue's claims and the lost cache reuse between kernels
+ * make it slower there), else groups of enough shots to reach that. Results
+ * do not depend on it. */
+int efwi_batch_group(efwi_ctx* ctx, int shots_per_launch);
 /* Gradient (lambda, mu, rho) of shot `shot` of the last efwi_run_batch. */
 int efwi_batch_gradient(efwi_ctx* ctx, int shot, float* g_lambda, float* g_mu, float* g_rho, int on_host,
                         void* stream);

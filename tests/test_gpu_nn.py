@@ -209,58 +209,10 @@ def test_pool_up_add_head_loss_adam_pack():
 
 
 def _oracle_network(net, feat, dims, target, mask, count):
-    """The U-Net's forward and weight gradients with the oracle's operations,
-    rounding to BF16 where the GPU stores BF16."""
-    r = O.bf16_round
-    W = {k: r(net.w[k].cpu().numpy()) for k in net.w}
-    Bs = {k: net.b[k].cpu().numpy() for k in net.b}
-    wh, bh = net.wh.cpu().numpy(), float(net.bh.cpu()[0])
-    f = lambda a: r(a.astype(np.float32))
-    x = _host(feat).reshape(dims + (16,))
-    conv = lambda n, a: f(O.conv3(a, W[n], Bs[n], relu=True))
-    a1 = conv("enc1a", x)
-    e1 = conv("enc1b", a1)
-    p1 = f(O.pool2(e1))
-    a2 = conv("enc2a", p1)
-    e2 = conv("enc2b", a2)
-    p2 = f(O.pool2(e2))
-    a3 = conv("bot_a", p2)
-    b3 = conv("bot_b", a3)
-    cat2 = np.concatenate([O.up2(b3), e2], axis=3)
-    d2a = conv("dec2a", cat2)
-    d2b = conv("dec2b", d2a)
-    cat1 = np.concatenate([O.up2(d2b), e1], axis=3)
-    d1a = conv("dec1a", cat1)
-    d1b = conv("dec1b", d1a)
-    o = O.head(d1b, wh, bh).astype(np.float32)
-    out = O.gate(x, o).astype(np.float32)
-    _, dout = O.mse(out, target.reshape(dims), mask.reshape(dims), count)
-    do = O.gate_bwd(o, out, dout.astype(np.float32)).astype(np.float32)
-    G = {}
-    g1b, G["head.w"], _ = O.head_bwd(d1b, wh, do)
-    g1b = f(g1b)
-
-    def layer(n, gpre, src, mask_src):
-        G[n + ".w"] = O.conv3_wgrad(gpre, src)
-        G[n + ".b"] = gpre.reshape(-1, gpre.shape[3]).sum(0)
-        if mask_src is None:
-            return f(O.conv3_dgrad(gpre, W[n]))
-        return f(O.conv3_dgrad(gpre, W[n]) * (mask_src > 0))
-    g = layer("dec1b", g1b, d1a, d1a)
-    gcat1 = layer("dec1a", g, cat1, cat1)
-    g = f(O.up2_bwd(gcat1[..., :32]))
-    g = layer("dec2b", g, d2a, d2a)
-    gcat2 = layer("dec2a", g, cat2, cat2)
-    g = f(O.up2_bwd(gcat2[..., :64]))
-    g = layer("bot_b", g, a3, a3)
-    gp2 = layer("bot_a", g, p2, None)
-    ge2 = f(f(gcat2[..., 64:] + O.pool2_bwd(gp2)) * (e2 > 0))
-    g = layer("enc2b", ge2, a2, a2)
-    gp1 = layer("enc2a", g, p1, None)
-    ge1 = f(f(gcat1[..., 32:] + O.pool2_bwd(gp1)) * (e1 > 0))
-    g = layer("enc1b", ge1, a1, a1)
-    layer("enc1a", g, x, None)
-    return out, G
+    from unet_reference import oracle_network
+    return oracle_network({k: v.cpu().numpy() for k, v in net.w.items()},
+                          {k: v.cpu().numpy() for k, v in net.b.items()}, net.wh.cpu().numpy(),
+                          float(net.bh.cpu()[0]), _host(feat), dims, target, mask, count)
 
 
 def test_unet_forward_and_weight_gradients_match_oracle():
