@@ -13,6 +13,20 @@ __global__ void k_fill(float* __restrict__ p, long n, float v) {
     p[i] = v;
 }
 
+// p[i] = scale * u(i, seed) with u uniform in [-1, 1) from an integer hash
+// (values of a live wavefield's magnitude for timing runs)
+__global__ void k_fill_hash(float* __restrict__ p, long n, unsigned seed, float scale) {
+  for (long i = (long)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (long)gridDim.x * blockDim.x) {
+    unsigned h = (unsigned)i * 0x9E3779B1u ^ seed;
+    h ^= h >> 16;
+    h *= 0x85EBCA6Bu;
+    h ^= h >> 13;
+    h *= 0xC2B2AE35u;
+    h ^= h >> 16;
+    p[i] = scale * ((float)(h >> 8) * (1.0f / 8388608.0f) - 1.0f);
+  }
+}
+
 // Source illumination of the pseudo-Hessian (diagonal of J^T J for a point
 // source, Shin et al. 2001): illum += vz^2 + vx^2 + vy^2 of the forward field
 // of this step. No reference counterpart.
@@ -308,6 +322,9 @@ __global__ void k_step_add(int* it, int delta) {
 
 void launch_fill(float* p, long n, float v, hipStream_t s) {
   hipLaunchKernelGGL(k_fill, dim3(blocks_for(n, 256, 8192)), dim3(256), 0, s, p, n, v);
+}
+void launch_fill_hash(float* p, long n, unsigned seed, float scale, hipStream_t s) {
+  hipLaunchKernelGGL(k_fill_hash, dim3(blocks_for(n, 256, 8192)), dim3(256), 0, s, p, n, seed, scale);
 }
 void launch_illum(const float* vz, const float* vx, const float* vy, float* illum, long n, hipStream_t s) {
   hipLaunchKernelGGL(k_illum, dim3(blocks_for(n, 256, 8192)), dim3(256), 0, s, vz, vx, vy, illum, n);

@@ -732,13 +732,13 @@ int Ctx::run_shot(const efwi_shot* sh, int calc_id, const float* obs, float* syn
         rf.B = B;
         rf.slot = ring + slot * 9 * B.n;
         prof.begin(K_VELOCITY_BWD, s);
-        const bool fv = launch_velocity_bwd(tiles[K_VELOCITY_BWD], form, G, F, C, s, &rf);
+        const bool fv = launch_velocity_bwd(tiles[K_VELOCITY_BWD], form, G, F, C, s, &rf, split_bwd == 1);
         prof.end(s);
         if (!fv) launch_ring(B, G, F9, ring, slot, F_VZ, F_VY + 1, 1, s);
         if (!src_interior) launch_add_source(W.szz, W.sxx, W.syy, src_cell, stf_dev, it, G.dt, -1, s);
         launch_inject(A, rec_dev, nrec, it, res_t, s);
         prof.begin(K_STRESS_BWD, s);
-        const bool fs = launch_stress_bwd(tiles[K_STRESS_BWD], G, F, C, S, it, s, &rf);
+        const bool fs = launch_stress_bwd(tiles[K_STRESS_BWD], G, F, C, S, it, s, &rf, split_bwd == 1);
         prof.end(s);
         if (!fs) launch_ring(B, G, F9, ring, slot, F_SZZ, F_WAVE_COUNT, 1, s);
       }
@@ -751,8 +751,12 @@ int Ctx::run_shot(const efwi_shot* sh, int calc_id, const float* obs, float* syn
 
 // Autotuner: time each stencil kernel of the time loop under every tile
 // shape (rows per workgroup x planes per thread) on this grid, with HIP events
-// on the stream, and keep the fastest. Runs on zeroed state (no source), which
-// the kernels leave zero; the state is zeroed again afterwards.
+// on the stream, and keep the fastest; then the interior/rim split of the two
+// backward kernels at their chosen shapes. The kernels run as in the time
+// loops: on a live wavefield (every field, adjoint and C-PML array filled with
+// values of a wavefield's magnitude) and with the boundary ring folded into
+// them where the time loops fold it (a one-slot ring of its own). No source.
+// The state is zeroed again afterwards.
 int Ctx::autotune(int reps, hipStream_t s, double* report) {
   HIP_TRY(hipSetDevice(cfg.device));
   if (!model_set) return fail(EFWI_ERR_STATE, "efwi_set_model was not called");
@@ -760,9 +764,25 @@ int Ctx::autotune(int reps, hipStream_t s, double* report) {
   static const Tile cands[] = {{1, 1}, {2, 1}, {4, 1}, {1, 2}, {1, 4}, {2, 2},
                                {4, 16}, {4, 32}, {4, 64}, {8, 16}, {8, 32}, {8, 64}};
   const int ncand = sizeof(cands) / sizeof(cands[0]);
-  zero_wave(W, N, s);
-  zero_wave(A, N, s);
-  zero_psi(P, N, s);
+  {
+    Fields9 fw, fa;
+    fields9(W, fw);
+    fields9(A, fa);
+    for (int k = 0; k < 9; k++) {
+      launch_fill_hash(fw.f[k], N, 17u + k, 1e-3f, s);
+      launch_fill_hash(fa.f[k], N, 101u + k, 1e-3f, s);
+    }
+    float* const* pp = reinterpret_cast<float* const*>(&P);
+    for (int k = 0; k < P_PSI_COUNT; k++) launch_fill_hash(pp[k], N, 211u + k, 1e-4f, s);
+  }
+  if (!tune_slot) {
+    const int rc0 = alloc((void**)&tune_slot, sizeof(float) * 9 * (size_t)B.n);
+    if (rc0) return rc0;
+  }
+  launch_fill_hash(tune_slot, 9 * B.n, 307u, 1e-3f, s);
+  RingFold rf{};
+  rf.B = B;
+  rf.slot = tune_slot;
   Source none;
   none.cell = -1;
   none.stf = stf_dev;
@@ -777,10 +797,10 @@ int Ctx::autotune(int reps, hipStream_t s, double* report) {
     for (int c = 0; c < ncand; c++) {
       const Tile t = cands[c];
       auto launch = [&]() {
-        if (k == K_STRESS_FWD) launch_stress_fwd(t, G, F, C, none, 0, s);
-        else if (k == K_VELOCITY_FWD) launch_velocity_fwd(t, cfg.formulation, G, F, C, s);
-        else if (k == K_VELOCITY_BWD) launch_velocity_bwd(t, cfg.formulation, G, F, C, s);
-        else launch_stress_bwd(t, G, F, C, none, 0, s);
+        if (k == K_STRESS_FWD) launch_stress_fwd(t, G, F, C, none, 0, s, &rf);
+        else if (k == K_VELOCITY_FWD) launch_velocity_fwd(t, cfg.formulation, G, F, C, s, &rf);
+        else if (k == K_VELOCITY_BWD) launch_velocity_bwd(t, cfg.formulation, G, F, C, s, &rf);
+        else launch_stress_bwd(t, G, F, C, none, 0, s, &rf);
       };
       launch();  // warm-up
       HIP_TRY(hipEventRecord(e0, s));
@@ -797,6 +817,26 @@ int Ctx::autotune(int reps, hipStream_t s, double* report) {
       }
     }
     tiles[k] = bt;
+  }
+  // the interior/rim split of the backward kernels at their chosen shapes
+  split_bwd = 0;
+  if (has_interior_segments(G) && tiles[K_VELOCITY_BWD].zdepth < 16 && tiles[K_STRESS_BWD].zdepth < 16) {
+    double t2[2] = {0.0, 0.0};
+    for (int sp = 0; sp < 2; sp++) {
+      auto launch = [&]() {
+        launch_velocity_bwd(tiles[K_VELOCITY_BWD], cfg.formulation, G, F, C, s, &rf, sp == 1);
+        launch_stress_bwd(tiles[K_STRESS_BWD], G, F, C, none, 0, s, &rf, sp == 1);
+      };
+      launch();
+      HIP_TRY(hipEventRecord(e0, s));
+      for (int r = 0; r < reps; r++) launch();
+      HIP_TRY(hipEventRecord(e1, s));
+      HIP_TRY(hipEventSynchronize(e1));
+      float ms = 0.0f;
+      HIP_TRY(hipEventElapsedTime(&ms, e0, e1));
+      t2[sp] = ms / reps;
+    }
+    split_bwd = t2[1] < t2[0] ? 1 : 0;
   }
   hipEventDestroy(e0);
   hipEventDestroy(e1);
@@ -933,6 +973,19 @@ int efwi_set_tile(efwi_ctx* ctx, int kernel, int rows, int zdepth) {
   if (zdepth >= 16 && rows != 1 && rows != 2 && rows != 4 && rows != 8)
     return fail(EFWI_ERR_ARG, "rows must be 1, 2, 4 or 8 (z-marching sweep, zdepth >= 16)");
   reinterpret_cast<Ctx*>(ctx)->tiles[kernel] = efwi::Tile{rows, zdepth};
+  return EFWI_OK;
+}
+
+int efwi_set_split(efwi_ctx* ctx, int on) {
+  if (!ctx) return fail(EFWI_ERR_ARG, "NULL context");
+  if (on != 0 && on != 1) return fail(EFWI_ERR_ARG, "on must be 0 or 1");
+  reinterpret_cast<Ctx*>(ctx)->split_bwd = on;
+  return EFWI_OK;
+}
+
+int efwi_get_split(const efwi_ctx* ctx, int* on) {
+  if (!ctx || !on) return fail(EFWI_ERR_ARG, "NULL argument");
+  *on = reinterpret_cast<const Ctx*>(ctx)->split_bwd;
   return EFWI_OK;
 }
 

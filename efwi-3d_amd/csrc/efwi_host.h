@@ -73,6 +73,10 @@ struct Ctx {
   // workgroup}, rows 0 = off. They advance the state from F into Falt (w and p
   // groups), and the two sets are swapped after every step.
   Groups Falt{};
+  // interior/rim split of the two backward kernels (efwi_set_split; chosen by
+  // efwi_autotune): 0 off, 1 on
+  int split_bwd = 0;
+  float* tune_slot = nullptr;  // the autotuner's one-slot ring
   efwi_config cfg{};
   Geom G{};
   long N = 0;

@@ -33,6 +33,10 @@ struct Sweep {
   int z0, nz, x0, rows_base, rows_rem, rg_max, y0, y1, yb0, nyb, R, Z, nzc;  // y segments yb0 .. yb0+nyb-1
   int zrev;  // plane chunks last to first (consecutive kernels alternate, so that each starts where the
              // previous one ended, on data the Infinity Cache still holds)
+  // cells the sweep leaves out (another launch covers them): planes
+  // [sk_z0, sk_z1] of rows [sk_x0, sk_x1] in y segments [sk_yb0, sk_yb1];
+  // empty when sk_z0 > sk_z1
+  int sk_z0, sk_z1, sk_x0, sk_x1, sk_yb0, sk_yb1;
   long blocks;
 };
 Sweep make_sweep(int z0, int z1, int x0, int x1, int y0, int y1, Tile t);
@@ -64,10 +68,17 @@ bool launch_stress_fwd(Tile t, const Geom& G, const Groups& F, const Prof& C, co
                        hipStream_t s, const RingFold* ring = nullptr);
 bool launch_velocity_fwd(Tile t, int form, const Geom& G, const Groups& F, const Prof& C, hipStream_t s,
                          const RingFold* ring = nullptr);
+// split: the wavefronts clear of every C-PML slab (rows inside the x and z
+// interior, 64-cell y segments inside the y interior) run as a launch of
+// their own on a kernel without the C-PML terms (fewer registers, more
+// wavefronts per SIMD), the rest as the general kernel's launch that skips
+// them. Flat sweeps only; results do not depend on it.
 bool launch_velocity_bwd(Tile t, int form, const Geom& G, const Groups& F, const Prof& C, hipStream_t s,
-                         const RingFold* ring = nullptr);
+                         const RingFold* ring = nullptr, bool split = false);
 bool launch_stress_bwd(Tile t, const Geom& G, const Groups& F, const Prof& C, const Source& S, int it,
-                       hipStream_t s, const RingFold* ring = nullptr);
+                       hipStream_t s, const RingFold* ring = nullptr, bool split = false);
+// whether the grid has wavefronts clear of every C-PML slab
+bool has_interior_segments(const Geom& G);
 // One fused forward step from the state in Fi (w and p groups) to Fo (Fi.m
 // holds the model); rows R in {4, 8, 12}, zc planes per workgroup. The source
 // must lie inside the stencil box. Returns whether the ring save was folded.
@@ -102,6 +113,7 @@ int device_cus(int device);
 
 // auxiliary kernels (efwi_aux.hip)
 void launch_fill(float* p, long n, float v, hipStream_t s);
+void launch_fill_hash(float* p, long n, unsigned seed, float scale, hipStream_t s);
 void launch_illum(const float* vz, const float* vx, const float* vy, float* illum, long n, hipStream_t s);
 void launch_byc(const Geom& G, const float* rho, float* byc, hipStream_t s);
 void launch_add_source(float* szz, float* sxx, float* syy, long cell, const float* stf, int it,

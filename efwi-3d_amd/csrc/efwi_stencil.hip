@@ -220,6 +220,9 @@ __device__ __forceinline__ void sweep_block(const Sweep& S, const Geom& G, int b
   c.x = S.x0 + xcd * S.rows_base + min(xcd, S.rows_rem) + row;
   c.y = (S.yb0 + yb) * 64 + (int)threadIdx.x;
   if (c.y < S.y0 || c.y > S.y1) return;
+  // planes of this wavefront's row segment that another launch covers
+  const bool skip_col = c.x >= S.sk_x0 && c.x <= S.sk_x1 && S.yb0 + yb >= S.sk_yb0 && S.yb0 + yb <= S.sk_yb1;
+  const int skz0 = skip_col ? S.sk_z0 : 1, skz1 = skip_col ? S.sk_z1 : 0;
   const unsigned sx = 4u * (unsigned)G.py, sz = 4u * (unsigned)G.pxy;
   const int za = S.z0 + zc * S.Z;
   const int zb = min(za + S.Z, S.z0 + S.nz);
@@ -229,6 +232,10 @@ __device__ __forceinline__ void sweep_block(const Sweep& S, const Geom& G, int b
   unsigned oy = 4u * ((unsigned)za * (unsigned)G.pxy + (unsigned)c.x * (unsigned)G.py + (unsigned)c.y) - 8u;
 #pragma unroll 1
   for (int z = za; z < zb; z++) {
+    if (z >= skz0 && z <= skz1) {
+      oy += sz;
+      continue;
+    }
     const unsigned o = oy + 8u;
     c.z = z;
     c.o = o;
@@ -743,6 +750,29 @@ __global__ __launch_bounds__(256) EFWI_BWD_ATTR void k_stress_bwd(Sweep Sw, Geom
       else stress_bwd_cell<RV, false, false>(G, acc, C, c, so, amp);
     }
     if (RV) rr.store(acc);
+  });
+}
+
+// The same two backward steps for wavefronts clear of every C-PML slab only
+// (the interior launch of the split): one variant, no C-PML terms.
+template <int FORM>
+__global__ __launch_bounds__(256) void k_velocity_bwd_in(Sweep Sw, Geom G, Groups F, Prof C, RingFold Rf) {
+  sweep(Sw, G, [&](const Cell& c) {
+    const FlatAcc acc = flat_acc(G, F, c);
+    const RingRestore<F_VZ, F_VY + 1> rr(Rf, c);
+    velocity_bwd_cell<FORM, true, false, false>(G, acc, C, c);
+    rr.store(acc);
+  });
+}
+__global__ __launch_bounds__(256) void k_stress_bwd_in(Sweep Sw, Geom G, Groups F, Prof C, Source S, int it,
+                                                       RingFold Rf) {
+  const unsigned so = source_off(S);
+  const float amp = source_amp(S, G, it);
+  sweep(Sw, G, [&](const Cell& c) {
+    const FlatAcc acc = flat_acc(G, F, c);
+    const RingRestore<F_SZZ, F_WAVE_COUNT> rr(Rf, c);
+    stress_bwd_cell<true, false, false>(G, acc, C, c, so, amp);
+    rr.store(acc);
   });
 }
 
@@ -1310,6 +1340,7 @@ Sweep make_sweep(int z0, int z1, int x0, int x1, int y0, int y1, Tile t) {
   S.nyb = y1 / 64 - S.yb0 + 1;
   S.nzc = (S.nz + S.Z - 1) / S.Z;
   S.zrev = 0;
+  S.sk_z0 = 1; S.sk_z1 = 0; S.sk_x0 = 1; S.sk_x1 = 0; S.sk_yb0 = 1; S.sk_yb1 = 0;
   S.blocks = 8L * S.rg_max * S.nyb * S.nzc;
   return S;
 }
@@ -1387,8 +1418,32 @@ bool launch_velocity_fwd(Tile t, int form, const Geom& G, const Groups& F, const
   else hipLaunchKernelGGL(k_velocity_fwd<0>, grid_of(Sw), block_of(Sw), 0, s, Sw, G, F, C, Rf);
   return Rf.slot != nullptr;
 }
+// y segments wholly inside the y interior [p, ny - 1 - p]: [lo, hi] (empty if lo > hi)
+static inline void interior_segments(const Geom& G, int& lo, int& hi) {
+  const int p = G.npml;
+  lo = (p + 63) / 64;
+  hi = (G.ny - p) / 64 - 1;
+}
+bool has_interior_segments(const Geom& G) {
+  int lo, hi;
+  interior_segments(G, lo, hi);
+  return lo <= hi && G.nz - 2 * G.npml > 0 && G.nx - 2 * G.npml > 0;
+}
+// The interior launch of the split and the general sweep that leaves it out.
+static inline Sweep split_sweeps(const Geom& G, Tile t, Sweep& rim) {
+  int lo, hi;
+  interior_segments(G, lo, hi);
+  const int p = G.npml;
+  Sweep in = make_sweep(p, G.nz - 1 - p, p, G.nx - 1 - p, 64 * lo, 64 * hi + 63, t);
+  rim = make_sweep(2, G.nz - 3, 2, G.nx - 3, 2, G.ny - 3, t);
+  rim.sk_z0 = p; rim.sk_z1 = G.nz - 1 - p;
+  rim.sk_x0 = p; rim.sk_x1 = G.nx - 1 - p;
+  rim.sk_yb0 = lo; rim.sk_yb1 = hi;
+  return in;
+}
+
 bool launch_velocity_bwd(Tile t, int form, const Geom& G, const Groups& F, const Prof& C, hipStream_t s,
-                         const RingFold* ring) {
+                         const RingFold* ring, bool split) {
   if (marching(t)) {
     const Sweep Sw = sweep_march(G, t);
     if (form == 1) {
@@ -1400,21 +1455,41 @@ bool launch_velocity_bwd(Tile t, int form, const Geom& G, const Groups& F, const
     }
     return false;
   }
-  const Sweep Sw = sweep_full(G, t);
   const RingFold Rf = fold_for(G, ring);
+  if (split && has_interior_segments(G)) {
+    Sweep rim;
+    const Sweep in = split_sweeps(G, t, rim);
+    if (form == 1) {
+      hipLaunchKernelGGL(k_velocity_bwd_in<1>, grid_of(in), block_of(in), 0, s, in, G, F, C, Rf);
+      hipLaunchKernelGGL((k_velocity_bwd<1, true>), grid_of(rim), block_of(rim), 0, s, rim, G, F, C, Rf);
+    } else {
+      hipLaunchKernelGGL(k_velocity_bwd_in<0>, grid_of(in), block_of(in), 0, s, in, G, F, C, Rf);
+      hipLaunchKernelGGL((k_velocity_bwd<0, true>), grid_of(rim), block_of(rim), 0, s, rim, G, F, C, Rf);
+    }
+    return Rf.slot != nullptr;
+  }
+  const Sweep Sw = sweep_full(G, t);
   if (form == 1) hipLaunchKernelGGL((k_velocity_bwd<1, true>), grid_of(Sw), block_of(Sw), 0, s, Sw, G, F, C, Rf);
   else hipLaunchKernelGGL((k_velocity_bwd<0, true>), grid_of(Sw), block_of(Sw), 0, s, Sw, G, F, C, Rf);
   return Rf.slot != nullptr;
 }
 bool launch_stress_bwd(Tile t, const Geom& G, const Groups& F, const Prof& C, const Source& S, int it,
-                       hipStream_t s, const RingFold* ring) {
+                       hipStream_t s, const RingFold* ring, bool split) {
   if (marching(t)) {
     EFWI_MARCH(k_stress_bwd_m, 1, Sw, G, F, C, S, it);
     return false;
   }
+  const RingFold Rf = fold_for(G, ring);
+  if (split && has_interior_segments(G)) {
+    Sweep rim;
+    Sweep in = split_sweeps(G, t, rim);
+    in.zrev = rim.zrev = 1;
+    hipLaunchKernelGGL(k_stress_bwd_in, grid_of(in), block_of(in), 0, s, in, G, F, C, S, it, Rf);
+    hipLaunchKernelGGL(k_stress_bwd<true>, grid_of(rim), block_of(rim), 0, s, rim, G, F, C, S, it, Rf);
+    return Rf.slot != nullptr;
+  }
   Sweep Sw = sweep_full(G, t);
   Sw.zrev = 1;  // the velocity kernel before it ran first to last
-  const RingFold Rf = fold_for(G, ring);
   hipLaunchKernelGGL(k_stress_bwd<true>, grid_of(Sw), block_of(Sw), 0, s, Sw, G, F, C, S, it, Rf);
   return Rf.slot != nullptr;
 }

@@ -142,6 +142,15 @@ class Propagator:
         12, 0 = two kernels). Results do not depend on it."""
         N.check(N.lib().efwi_set_fusion(self._h, int(pass_), int(rows), int(zchunk)), "efwi_set_fusion")
 
+    def set_split(self, on: bool):
+        """Interior/rim split of the backward kernels (results do not depend on it)."""
+        N.check(N.lib().efwi_set_split(self._h, int(bool(on))), "efwi_set_split")
+
+    def split(self) -> bool:
+        v = C.c_int()
+        N.check(N.lib().efwi_get_split(self._h, C.byref(v)), "efwi_get_split")
+        return bool(v.value)
+
     def tiles(self) -> dict:
         out = {}
         for k in N.TUNED_KERNELS:

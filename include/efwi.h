@@ -141,6 +141,22 @@ int efwi_profile_read(efwi_ctx* ctx, double* ms, long long* launches, int reset)
  * average milliseconds in that order. No reference counterpart (the
  * reference launches fixed 32 x 1 x 32 blocks, Src/libCUFD.cu:93-97). */
 int efwi_set_tile(efwi_ctx* ctx, int kernel, int rows, int zdepth);
+int efwi_get_tile(const efwi_ctx* ctx, int kernel, int* rows, int* zdepth);
+/* efwi_autotune times the candidates as the time loops run them: on a live
+ * wavefield (the state filled with values of a wavefield's magnitude) and
+ * with the boundary ring folded in where the loops fold it; it then decides
+ * the split below for the chosen backward shapes. */
+int efwi_autotune(efwi_ctx* ctx, int reps, double* report, void* stream);
+
+/* Interior/rim split of the two backward kernels (flat shapes): on = 1 runs
+ * the wavefronts clear of every C-PML slab (rows inside the x and z
+ * interior, 64-cell y segments inside the y interior) as a launch of their
+ * own, on a kernel without C-PML terms (fewer registers, twice the
+ * wavefronts per SIMD), and the rest as the general kernel's launch. Results
+ * are bit-identical either way. Default 0; efwi_autotune sets it. No
+ * reference counterpart. */
+int efwi_set_split(efwi_ctx* ctx, int on);
+int efwi_get_split(const efwi_ctx* ctx, int* on);
 
 /* Fused time steps. pass 0: the forward step's stress and velocity updates in
  * one launch that marches z over workgroups of 64 x (rows + 4) threads, each
@@ -153,8 +169,6 @@ int efwi_set_tile(efwi_ctx* ctx, int kernel, int rows, int zdepth);
  * box use the two-kernel step. No reference counterpart (the reference
  * launches each kernel separately, Src/libCUFD.cu:312-354). */
 int efwi_set_fusion(efwi_ctx* ctx, int pass, int rows, int zchunk);
-int efwi_get_tile(const efwi_ctx* ctx, int kernel, int* rows, int* zdepth);
-int efwi_autotune(efwi_ctx* ctx, int reps, double* report, void* stream);
 
 /* Batched shots: run `nshots` shots of this grid and model together. Each
  * stencil step of all shots is one persistent launch over a (shot, tile) work

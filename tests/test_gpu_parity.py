@@ -193,3 +193,37 @@ def test_autotuner_picks_a_candidate(small_case):
         assert len(times) == 12 and all(t > 0 for t in times.values())
         best = min(times, key=times.get)
         assert tiles[k] == best, (k, tiles[k], best)
+
+
+@pytest.mark.parametrize("formulation,tile", [(0, (1, 1)), (1, (2, 2)), (0, (4, 3))])
+def test_backward_split_is_bit_identical(orc, formulation, tile):
+    """The interior/rim split of the backward kernels (efwi_set_split): a grid
+    212 cells long in y (segments 1 and 2 wholly in the y interior, the source
+    among them) with the split on, against the oracle and the unsplit run,
+    seismograms, residuals and gradients bit for bit, at flat shapes that
+    march several planes per thread."""
+    from efwi import synth
+    from efwi.propagator import Propagator
+    w = synth.small(n=20, npml=6, nt=120, ny=200, formulation=formulation)
+    obs = [s.astype(np.float32) for s in oracle_run(orc, w, model=w.true, calc_id=2).syn]
+    outs = []
+    for split in (False, True):
+        print(f"[gpu] backward split={split} form={formulation} tile={tile}", flush=True)
+        with Propagator(w.grid) as P:
+            for k in ("stress_fwd", "velocity_fwd", "stress_bwd", "velocity_bwd"):
+                P.set_tile(k, *tile)
+            P.set_split(split)
+            assert P.split() == split
+            P.set_model(*(torch.from_numpy(a) for a in w.model))
+            P.zero_gradient()
+            j, syn, res = P.run_shot(w.shots[0], 1, torch.from_numpy(obs[0]), want_syn=True, want_res=True)
+            outs.append((j, res.cpu().numpy(), [g.cpu().numpy() for g in P.gradient()]))
+            torch.cuda.synchronize()
+    (ja, ra, ga), (jb, rb, gb) = outs
+    ref = oracle_run(orc, w, obs=obs, calc_id=1)
+    assert ja == jb
+    assert np.array_equal(ra, rb)
+    for a, b, c in zip(ga, gb, (ref.glam, ref.gmu, ref.grho)):
+        assert np.abs(c[:, :, 64:192]).max() > 0
+        assert np.array_equal(a, b)
+        assert np.array_equal(b, c.astype(np.float32))
