@@ -145,6 +145,7 @@ def bench_batch(args, w):
     trials = {}
     cands = [(0, None, 2), (-1, None, 2)] + [(grp, tile, take) for grp in (n, max(1, n // 4))
                                               for tile in ((2, 1), (2, 2)) for take in (1, 4)]
+    cands += [(1, (2, 2), 1)]
     for cfg in cands:
         timed(cfg)  # warm-up (graph capture)
         trials[cfg] = timed(cfg)

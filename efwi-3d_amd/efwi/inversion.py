@@ -18,6 +18,7 @@ from __future__ import annotations
 import time
 from dataclasses import dataclass, field
 
+import numpy as np
 import torch
 
 from .lbfgs import LBFGS, armijo, default_dot
@@ -58,10 +59,22 @@ def vp_to_lame(vp, vs, rho_kg):
     return (vp ** 2 - 2.0 * vs ** 2) * rho, vs ** 2 * rho, rho
 
 
+def stable_vp_max(grid) -> float:
+    """The largest P velocity the grid's time step keeps stable: Courant number
+    vp dt sqrt(3) (9/8 + 1/24) / min(dz, dx, dy) at most 1 (efwi_set_model
+    refuses models above it; the reference's compCourantNumber,
+    Src/utilities.cu:186-202, exits there)."""
+    h = min(grid.dz, grid.dx, grid.dy)
+    return float(h / (float(np.float32(grid.dt)) * np.sqrt(3.0) * (9.0 / 8.0 + 1.0 / 24.0)))
+
+
 @dataclass
 class Problem:
     """Invert vp on the padded grid with vs and rho fixed, as the reference's
-    driver does (3D_inversion_BFGS.py:138-152); `mask` is 1 where vp is free."""
+    driver does (3D_inversion_BFGS.py:138-152); `mask` is 1 where vp is free.
+    Trial models are projected onto [vmin, vmax], with vmax lowered to just
+    below the grid's stability limit (the bounds of the reference's L-BFGS-B,
+    3D_inversion_BFGS.py:188)."""
     prop: Propagator
     shots: list
     obs: list
@@ -72,6 +85,9 @@ class Problem:
     vmax: float = 6000.0
     grad_scale: float = 2.0
     nfev: int = 0
+
+    def __post_init__(self):
+        self.vmax = min(self.vmax, 0.999 * stable_vp_max(self.prop.grid))
 
     def evaluate(self, vp: torch.Tensor):
         """(misfit, dJ/dvp masked) at vp."""

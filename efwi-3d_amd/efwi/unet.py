@@ -289,3 +289,34 @@ def pack_input(g, illum, npml, device=0):
     nn_check(lib.efwi_nn_pack(_p(g), _p(illum), nz, nx, ny, npml, 1.0 / max(gmax, 1e-30), 1.0 / max(imax, 1e-30),
                               _p(feat), s), "pack")
     return feat, dims
+
+
+def _time_training(dims, steps, seed=0):
+    """Seconds per training step of the U-Net on a random 2-channel volume of
+    `dims` voxels (after 3 untimed steps)."""
+    import time
+    rng = np.random.default_rng(seed)
+    v = dims[0] * dims[1] * dims[2]
+    net = UNet3D(0, seed=1)
+    f0 = np.zeros((v, 16), np.float32)
+    f0[:, :IN_CHANNELS] = rng.standard_normal((v, IN_CHANNELS))
+    feat = torch.from_numpy(f0).to(net.dev).to(torch.bfloat16)
+    target = torch.from_numpy(rng.standard_normal(v).astype(np.float32)).to(net.dev)
+    mask = torch.ones(v, device=net.dev)
+    for _ in range(3):
+        net.train_step(feat, target, mask, v, dims)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        net.train_step(feat, target, mask, v, dims)
+    torch.cuda.synchronize()
+    return (time.perf_counter() - t0) / steps
+
+
+if __name__ == "__main__":
+    # python -m efwi.unet [nz nx ny [steps]]: time the training step
+    import sys
+    a = [int(x) for x in sys.argv[1:]]
+    dims = tuple(a[:3]) if len(a) >= 3 else (96, 128, 128)
+    steps = a[3] if len(a) >= 4 else 10
+    print(f"U-Net training step at {dims}: {_time_training(dims, steps) * 1e3:.2f} ms", flush=True)
