@@ -132,53 +132,98 @@ __global__ __launch_bounds__(256) void k_conv3(const uint16_t* __restrict__ src,
   }
 }
 
-// Weight gradient partials: wavefront (chunk, tap, m-block) runs `per` row
-// segments of 16 voxels and writes part[chunk][t][m][n] for its 16 m and all n.
-template <int NBN>
-__global__ __launch_bounds__(64) void k_wgrad(const uint16_t* __restrict__ dpre, int ldp, int mc,
-                                              const uint16_t* __restrict__ src, int lds, int kc,
-                                              float* __restrict__ part, Vol V, int per) {
-  const int lane = threadIdx.x;
-  const int chunk = blockIdx.x, t = blockIdx.y, mb = blockIdx.z;
+// Weight gradient partials, operands staged through LDS. A block of three
+// wavefronts takes `per` row segments of 16 voxels (a chunk), one 16-channel
+// block of m and NB 16-channel blocks of n; wavefront w takes the 9 taps with
+// dz = w - 1. Per segment the block loads the dpre tile (16 voxels x 16 m)
+// and each wavefront its three rows of src (dx = -1, 0, 1; 18 voxels along y
+// for dy = -1, 0, 1) with 16-byte loads, transposes them into LDS (the src
+// rows once per dy shift, so every MFMA operand is one aligned 8-byte LDS
+// read), and runs 9 x NB MFMAs on one A fragment. Writes
+// part[chunk][t][m][n] for its taps, m and n.
+template <int NB>
+__global__ __launch_bounds__(192) void k_wgrad(const uint16_t* __restrict__ dpre, int ldp, int mc,
+                                               const uint16_t* __restrict__ src, int lds, int kc,
+                                               float* __restrict__ part, Vol V, int per) {
+  __shared__ __align__(16) uint16_t At[16 * 16];             // [m][voxel]
+  __shared__ __align__(16) uint16_t Bt[3][3][3][NB * 16][16];  // [dz][dx][dy][n][voxel]
+  const int tid = threadIdx.x, wv = tid >> 6, lane = tid & 63;
+  const int chunk = blockIdx.x, mb = blockIdx.y, nb0 = blockIdx.z * NB;
+  const int nbl = min(NB, kc / 16 - nb0);  // n blocks of this block (uniform)
+  const int nvec = nbl * 2;                // 8-channel vectors per voxel
   const int i = lane & 15, kq = (lane >> 4) * 4;
   const int tpr = (V.ny + 15) / 16;
   const long ntiles = (long)V.nz * V.nx * tpr;
-  int dz, dx, dy;
-  tap(t, dz, dx, dy);
-  f4 acc[NBN];
+  const int dz = wv - 1;
+  f4 acc[3][3][NB];
 #pragma unroll
-  for (int b = 0; b < NBN; b++) acc[b] = f4{0.f, 0.f, 0.f, 0.f};
-  const int m = mb * 16 + i;
-  for (long tile = (long)chunk * per; tile < std::min(ntiles, (long)(chunk + 1) * per); tile++) {
+  for (int a = 0; a < 3; a++)
+#pragma unroll
+    for (int b = 0; b < 3; b++)
+#pragma unroll
+      for (int c = 0; c < NB; c++) acc[a][b][c] = f4{0.f, 0.f, 0.f, 0.f};
+  const long t0 = (long)chunk * per, t1 = std::min(ntiles, t0 + per);
+  for (long tile = t0; tile < t1; tile++) {
     const int row = (int)(tile / tpr);
     const int y0 = (int)(tile - (long)row * tpr) * 16;
     const int z = row / V.nx, x = row - z * V.nx;
-    s4 a;  // A[m][k] = dpre[voxel k][m]
+    // A tile: voxel v, channels h..h+7 of the m block (threads 0..31)
+    if (tid < 32) {
+      const int v = tid >> 1, h = (tid & 1) * 8;
+      uint4 q = {0u, 0u, 0u, 0u};
+      if (y0 + v < V.ny) q = *reinterpret_cast<const uint4*>(dpre + V.vox(z, x, y0 + v) * ldp + mb * 16 + h);
+      const uint16_t* e = reinterpret_cast<const uint16_t*>(&q);
 #pragma unroll
-    for (int j = 0; j < 4; j++) {
-      const int y = y0 + kq + j;
-      a[j] = y < V.ny ? (short)dpre[V.vox(z, x, y) * ldp + m] : (short)0;
+      for (int j = 0; j < 8; j++) At[(h + j) * 16 + v] = e[j];
     }
-    // B[k][n] = src[voxel k + d_t][n]
-    const int zs = z + dz, xs = x + dx;
+    // this wavefront's src rows: voxel vv (y = y0 - 1 + vv) is operand column
+    // k = vv - dyi of shift dyi (dy = dyi - 1)
+    const int items = 3 * 18 * nvec;
+    for (int it = lane; it < items; it += 64) {
+      const int dxi = it / (18 * nvec), r = it - dxi * 18 * nvec;
+      const int vv = r / nvec, c8 = (r - vv * nvec) * 8;
+      const int zs = z + dz, xs = x + dxi - 1, ys = y0 - 1 + vv;
+      uint4 q = {0u, 0u, 0u, 0u};
+      if (V.in(zs, xs, ys)) q = *reinterpret_cast<const uint4*>(src + V.vox(zs, xs, ys) * lds + nb0 * 16 + c8);
+      const uint16_t* e = reinterpret_cast<const uint16_t*>(&q);
 #pragma unroll
-    for (int b = 0; b < NBN; b++) {
-      const int n = b * 16 + i;
-      s4 bw;
+      for (int dyi = 0; dyi < 3; dyi++) {
+        const int k = vv - dyi;
+        if (k >= 0 && k < 16) {
 #pragma unroll
-      for (int j = 0; j < 4; j++) {
-        const int y = y0 + kq + j, ys = y + dy;
-        bw[j] = (y < V.ny && V.in(zs, xs, ys)) ? (short)src[V.vox(zs, xs, ys) * lds + n] : (short)0;
+          for (int j = 0; j < 8; j++) Bt[wv][dxi][dyi][c8 + j][k] = e[j];
+        }
       }
-      acc[b] = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(a, bw, acc[b], 0, 0, 0);
     }
+    __syncthreads();
+    const s4 a = *reinterpret_cast<const s4*>(&At[i * 16 + kq]);  // A[m][k] = dpre[voxel k][m]
+#pragma unroll
+    for (int dxi = 0; dxi < 3; dxi++)
+#pragma unroll
+      for (int dyi = 0; dyi < 3; dyi++)
+#pragma unroll
+        for (int b = 0; b < NB; b++) {
+          if (b >= nbl) break;
+          // B[k][n] = src[voxel k + d_t][n]
+          const s4 bw = *reinterpret_cast<const s4*>(&Bt[wv][dxi][dyi][b * 16 + i][kq]);
+          acc[dxi][dyi][b] = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(a, bw, acc[dxi][dyi][b], 0, 0, 0);
+        }
+    __syncthreads();
   }
-  // D[m][n]: lane holds m = mb*16 + kq + j, n = b*16 + i
-  float* out = part + ((long)chunk * 27 + t) * mc * kc;
+  // D[m][n]: lane holds m = mb*16 + kq + j, n = (nb0 + b)*16 + i
 #pragma unroll
-  for (int b = 0; b < NBN; b++)
+  for (int dxi = 0; dxi < 3; dxi++)
 #pragma unroll
-    for (int j = 0; j < 4; j++) out[(long)(mb * 16 + kq + j) * kc + b * 16 + i] = acc[b][j];
+    for (int dyi = 0; dyi < 3; dyi++) {
+      const int t = 9 * wv + 3 * dxi + dyi;
+      float* out = part + ((long)chunk * 27 + t) * mc * kc;
+#pragma unroll
+      for (int b = 0; b < NB; b++) {
+        if (b >= nbl) break;
+#pragma unroll
+        for (int j = 0; j < 4; j++) out[(long)(mb * 16 + kq + j) * kc + (nb0 + b) * 16 + i] = acc[dxi][dyi][b][j];
+      }
+    }
 }
 
 // out[k] += sum over chunks [blockIdx.y * kChunkGroup, +kChunkGroup) of part[chunk][k]
@@ -465,7 +510,8 @@ long efwi_nn_wgrad_scratch(int mc, int kc, int nz, int nx, int ny, int per) {
 static bool wgrad_args_ok(const uint16_t* dpre, int ldp, int mc, const uint16_t* src, int lds, int kc, float* dw,
                           int nz, int nx, int ny, int per) {
   return dpre && src && dw && mc % 16 == 0 && kc % 16 == 0 && mc > 0 && kc > 0 && kc <= 96 && per > 0 &&
-         ldp % 16 == 0 && lds % 16 == 0 && ldp >= mc && lds >= kc && nz > 0 && nx > 0 && ny > 0;
+         ldp % 16 == 0 && lds % 16 == 0 && ldp >= mc && lds >= kc && nz > 0 && nx > 0 && ny > 0 &&
+         ((uintptr_t)dpre | (uintptr_t)src) % 16 == 0;
 }
 static int wgrad_launch(const uint16_t* dpre, int ldp, int mc, const uint16_t* src, int lds, int kc, float* scratch,
                         float* dw, int nz, int nx, int ny, int per, hipStream_t s);
@@ -519,15 +565,10 @@ static int wgrad_launch(const uint16_t* dpre, int ldp, int mc, const uint16_t* s
   const Vol V{nz, nx, ny};
   const long tiles = (long)nz * nx * ((ny + 15) / 16);
   const int chunks = (int)((tiles + per - 1) / per);
-  const dim3 grid((unsigned)chunks, 27, (unsigned)(mc / 16));
-  switch (kc / 16) {
-    case 1: hipLaunchKernelGGL((k_wgrad<1>), grid, dim3(64), 0, s, dpre, ldp, mc, src, lds, kc, scratch, V, per); break;
-    case 2: hipLaunchKernelGGL((k_wgrad<2>), grid, dim3(64), 0, s, dpre, ldp, mc, src, lds, kc, scratch, V, per); break;
-    case 3: hipLaunchKernelGGL((k_wgrad<3>), grid, dim3(64), 0, s, dpre, ldp, mc, src, lds, kc, scratch, V, per); break;
-    case 4: hipLaunchKernelGGL((k_wgrad<4>), grid, dim3(64), 0, s, dpre, ldp, mc, src, lds, kc, scratch, V, per); break;
-    case 5: hipLaunchKernelGGL((k_wgrad<5>), grid, dim3(64), 0, s, dpre, ldp, mc, src, lds, kc, scratch, V, per); break;
-    default: hipLaunchKernelGGL((k_wgrad<6>), grid, dim3(64), 0, s, dpre, ldp, mc, src, lds, kc, scratch, V, per); break;
-  }
+  // one 16-channel n block per workgroup: two per workgroup (A tiles shared)
+  // measured slower, at half the occupancy (185 registers against 112)
+  hipLaunchKernelGGL((k_wgrad<1>), dim3((unsigned)chunks, (unsigned)(mc / 16), (unsigned)(kc / 16)), dim3(192), 0, s,
+                     dpre, ldp, mc, src, lds, kc, scratch, V, per);
   const long width = 27L * mc * kc;
   hipLaunchKernelGGL(k_zero, dim3((unsigned)((width + 255) / 256)), dim3(256), 0, s, reinterpret_cast<unsigned*>(dw),
                      (int)width);

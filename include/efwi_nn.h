@@ -41,9 +41,12 @@ int efwi_nn_conv3(const uint16_t* src, int lds, int kc, const uint16_t* w, const
 
 /* Weight gradient of a 3x3x3 convolution, dw[t][m][n] = sum_v dpre[v][m]
  * src[v + d_t][n] (FP32 result), dpre = gradient of the layer's
- * pre-activation (mc channels), src = its input (kc <= 96 channels). Each
- * wavefront reduces `per` segments of 16 voxels into FP32 partials in
- * `scratch` (efwi_nn_wgrad_scratch floats), which are then summed. */
+ * pre-activation (mc channels), src = its input (kc <= 96 channels); dpre and
+ * src 16-byte aligned (a channel slice starts at a multiple of 8 channels).
+ * Each workgroup reduces `per` segments of 16 voxels into FP32 partials in
+ * `scratch` (efwi_nn_wgrad_scratch floats), which are then summed; its
+ * operands go through LDS with 16-byte loads and each A fragment feeds the 9
+ * taps of a wavefront. */
 long efwi_nn_wgrad_scratch(int mc, int kc, int nz, int nx, int ny, int per);
 int efwi_nn_wgrad(const uint16_t* dpre, int ldp, int mc, const uint16_t* src, int lds, int kc, float* scratch,
                   long scratch_floats, float* dw, int nz, int nx, int ny, int per, void* stream);
