@@ -89,13 +89,34 @@ class _DeviceCache:
 _cache = _DeviceCache()
 
 
+def _window_factors(grid, shot):
+    """The misfit's factors w(t)^2 (x weights) per trace, [nrec][nt] or [nt]
+    (Src/utilities.cu:524-614), as the library applies them."""
+    def fp(a):
+        return a.ctypes.data_as(C.POINTER(C.c_float))
+    if grid.if_win:
+        out = np.zeros((shot.nrec, grid.nt), np.float32)
+        ws, we = (np.ascontiguousarray(a, np.float32) for a in (shot.win_start, shot.win_end))
+        wt = None if shot.weights is None else np.ascontiguousarray(shot.weights, np.float32)
+        N.check(N.lib().efwi_window_factors(grid.nt, grid.dt, shot.nrec, fp(ws), fp(we), None if wt is None else fp(wt),
+                                            float(shot.src_weight), fp(out)), "efwi_window_factors")
+        return out
+    out = np.zeros(grid.nt, np.float32)
+    N.check(N.lib().efwi_taper_w2(grid.nt, grid.dt, 0.005, fp(out)), "efwi_taper_w2")
+    return out
+
+
 def _run_device(Lambda, Mu, Rho, Stf, calc_id, device, shot_ids, para_fname):
-    """The cufd pipeline for device tensors through the context API."""
+    """The cufd pipeline for device tensors through the context API, with the
+    host path's files: synthetic data for calc_id 2, and for gradients with a
+    scratch_dir_name the windowed residual, synthetic and observed data
+    (Src/libCUFD.cu:660-725)."""
     para = eio.read_para(para_fname)
     grid = eio.grid_from_para(para)
     survey = eio.read_survey(para["survey_fname"])
     stf = Stf.detach().to("cpu", torch.float32).numpy()
     prop = _cache.get(grid, device)
+    scratch = para.get("scratch_dir_name") if calc_id == N.CALC_GRADIENT else None
     with torch.cuda.device(device):
         prop.set_model(Lambda, Mu, Rho)
         prop.zero_gradient()
@@ -105,11 +126,16 @@ def _run_device(Lambda, Mu, Rho, Stf, calc_id, device, shot_ids, para_fname):
             obs = None
             if calc_id != N.CALC_SYNTHETIC:
                 obs = torch.from_numpy(eio.read_shot_data(para["data_dir_name"], sid, shot.nrec, grid.nt))
-            j, syn, _ = prop.run_shot(shot, calc_id, obs)
+            j, syn, res = prop.run_shot(shot, calc_id, obs, want_syn=scratch is not None, want_res=scratch is not None)
             if calc_id == N.CALC_SYNTHETIC:
                 eio.write_shot_data(para["data_dir_name"], sid, syn.cpu().numpy())
             else:
                 total += j
+            if scratch is not None:
+                fac = _window_factors(grid, shot)
+                eio.write_shot_data(scratch, sid, res.cpu().numpy(), prefix="Residual_Shot_")
+                eio.write_shot_data(scratch, sid, syn.cpu().numpy() * fac, prefix="Syn_Shot_")
+                eio.write_shot_data(scratch, sid, obs.numpy() * fac, prefix="CondObs_Shot_")
         grads = prop.gradient() if calc_id == N.CALC_GRADIENT else None
         torch.cuda.synchronize(device)
     return total, grads

@@ -1045,6 +1045,16 @@ int efwi_courant_number(double vp_max, float dt, float dz, float dx, float dy, d
   return EFWI_OK;
 }
 
+int efwi_window_factors(int nt, float dt, int nrec, const float* win_start, const float* win_end,
+                        const float* weights, float src_weight, float* out) {
+  if (nt <= 0 || nrec < 0 || !out || (nrec > 0 && (!win_start || !win_end))) return fail(EFWI_ERR_ARG, "bad argument");
+  for (int r = 0; r < nrec; r++)
+    for (int it = 0; it < nt; it++)
+      out[(size_t)r * nt + it] = efwi::taper_window_fac(it, nt, dt, 0.005f, win_start[r], win_end[r],
+                                                        weights ? weights[r] : 1.0f, src_weight);
+  return EFWI_OK;
+}
+
 int efwi_taper_w2(int nt, float dt, float ratio, float* out) {
   if (nt <= 0 || !out) return fail(EFWI_ERR_ARG, "bad argument");
   for (int it = 0; it < nt; it++) out[it] = efwi::taper_plain_w2(it, nt, dt, ratio);

@@ -72,6 +72,7 @@ SIGNATURES = {
     "efwi_profile_read": (C.c_int, [_vp, C.POINTER(C.c_double), C.POINTER(C.c_longlong), C.c_int]),
     "efwi_cpml_profile": (C.c_int, [C.c_int, C.c_int, C.c_float, C.c_float, C.c_float, _fp]),
     "efwi_taper_w2": (C.c_int, [C.c_int, C.c_float, C.c_float, _fp]),
+    "efwi_window_factors": (C.c_int, [C.c_int, C.c_float, C.c_int, _fp, _fp, _fp, C.c_float, _fp]),
     "efwi_courant_number": (C.c_int, [C.c_double, C.c_float, C.c_float, C.c_float, C.c_float, _dp]),
     "efwi_cufd": (C.c_int, [_fp, _fp, _fp, _fp, _fp, _fp, _fp, _fp, C.c_int, C.c_int, C.c_int, _ip,
                             C.c_char_p]),

@@ -219,6 +219,12 @@ long long efwi_device_bytes(const efwi_ctx* ctx);
  * rows K, a, b, K_half, a_half, b_half (Src/utilities.cu:204-320), and the
  * squared time taper w(t)^2 (Src/utilities.cu:578-614). */
 int efwi_cpml_profile(int n, int npml, float h, float f0, float dt, float* out6n);
+/* The per-trace factors w(t)^2 * weight * src_weight of the if_win misfit
+ * (Src/utilities.cu:524-576, window ratio 0.005), [nrec][nt] into out;
+ * weights may be NULL (all 1). The plain misfit's factors are
+ * efwi_taper_w2(nt, dt, 0.005). */
+int efwi_window_factors(int nt, float dt, int nrec, const float* win_start, const float* win_end,
+                        const float* weights, float src_weight, float* out);
 /* Courant number of the 3-D scheme, vp_max dt sqrt(3) (9/8 + 1/24) /
  * min(dz, dx, dy); efwi_set_model refuses models above 1. The reference's
  * compCourantNumber (Src/utilities.cu:186-202) uses sqrt(2) / min(dz, dx),

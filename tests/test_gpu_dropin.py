@@ -135,3 +135,33 @@ def test_cufd_scratch_files_and_gradient_scaling(orc, tmp_path):
         assert f.exists() and os.path.getsize(f) == m * e["nt"] * 4
     rx = np.fromfile(tmp_path / "scratch" / "Residual_Shot_x1.bin", np.float32).reshape(m, e["nt"])
     assert rel_l2(rx, ref.res[0][0]) <= 1e-5
+
+
+@pytest.mark.parametrize("if_win", [False, True])
+def test_device_path_writes_the_host_paths_scratch_files(tmp_path, if_win):
+    """With CUDA tensors the drop-in op runs the device-resident path; with a
+    scratch_dir_name in the para file it writes the same Residual/Syn/CondObs
+    files as the host path (efwi_cufd, Src/libCUFD.cu:660-701), bit for bit."""
+    import FWI_ops_3d as F
+    print(f"[gpu] scratch files host vs device, if_win={if_win}", flush=True)
+    files = {}
+    for dev in ("cpu", "cuda"):
+        d = tmp_path / dev
+        d.mkdir()
+        e = _experiment(d, if_win=if_win)
+        ids = torch.tensor([0, 1], dtype=torch.int32)
+        F.FWI_obscalc(torch.tensor(e["vp"]), torch.tensor(e["vs"]),
+                      torch.tensor((e["rho"] / np.float32(1e3)).astype(np.float32)), torch.tensor(e["stf"]),
+                      e["opt"])(ids)
+        dv = torch.device(dev)
+        fwi = F.FWI_3d(torch.tensor(e["vp0"], device=dv), torch.tensor(e["vs"], device=dv),
+                       torch.tensor(e["rho"], device=dv), torch.tensor(e["stf"]), e["opt"])
+        fwi(ids)
+        out = {}
+        for p in sorted((d / "scratch").iterdir()):
+            out[p.name] = np.fromfile(p, np.float32)
+        files[dev] = out
+    assert len(files["cpu"]) == 2 * 3 * 3 and set(files["cpu"]) == set(files["cuda"])
+    for name, a in files["cpu"].items():
+        assert np.array_equal(a, files["cuda"][name]), name
+        assert np.abs(a).max() > 0, name
