@@ -29,6 +29,7 @@ namespace {
 
 typedef short s4 __attribute__((ext_vector_type(4)));
 typedef float f4 __attribute__((ext_vector_type(4)));
+typedef __bf16 bf8 __attribute__((ext_vector_type(8)));
 
 __device__ __forceinline__ float bf2f(uint16_t h) { return __uint_as_float((unsigned)h << 16); }
 // round to nearest even (finite inputs)
@@ -52,79 +53,122 @@ __device__ __forceinline__ void tap(int t, int& dz, int& dx, int& dy) {
   dy = t % 3 - 1;
 }
 
-// One wavefront per row segment of 16 MB voxels along y (MB row blocks of the
-// MFMA's A operand), NB output blocks of 16 channels starting at block
-// blockIdx.y * NB: every B fragment (weights) is used MB times, every A
-// fragment NB times. MODE 0 forward, 1 dgrad.
-template <int NB, int MB, int MODE>
+// 3x3x3 convolution as implicit GEMM on v_mfma_f32_16x16x32_bf16, the input
+// staged through LDS. A workgroup of four wavefronts owns a brick of 2 (z) x
+// 2 (x) rows of TY = 16 MB voxels along y and NB blocks of 16 output
+// channels; wavefront w takes row (z0 + w / 2, x0 + w % 2), MB row blocks of
+// the A operand (16 voxels each) against NB column blocks. Per chunk of CK
+// input channels the block copies the brick's halo (4 x 4 rows of TY + 2
+// voxels, zero outside the volume) into LDS with 16-byte loads; every tap
+// then reads its A fragments from there with one 16-byte LDS read per lane
+// (the 8-channel granules of a voxel are XOR-swizzled with its position so
+// that the 16 lanes of a fragment row hit distinct banks). The K dimension
+// runs over (tap, channel) of the chunk in steps of 32: lane group g = l / 16
+// holds K entries 8 g .. 8 g + 7, i.e. taps t = (32 q + 8 g) / CK, so with
+// CK = 16 one MFMA covers two taps. B fragments (weights) come from global
+// memory: w[t][n][k] in mode 0, one 16-byte load per lane; mode 1 (input
+// gradient with the forward layer's weights, taps flipped, B[k][n] =
+// w[26 - t][k][n]) gathers them with 2-byte loads, which a caller avoids by
+// passing the flipped copy of efwi_nn_flip_weights to mode 0.
+template <int CK>
+__device__ __forceinline__ int granule_swizzle(int hv) {
+  // G = CK / 8 granules of 16 bytes per voxel; 16 / G voxels per 64-bank row
+  return CK == 32 ? ((hv >> 2) & 3) : ((hv >> 3) & 1);
+}
+
+template <int CK, int MB, int NB, int MODE>
 __global__ __launch_bounds__(256) void k_conv3(const uint16_t* __restrict__ src, int lds, int kc,
                                                const uint16_t* __restrict__ w, const float* __restrict__ bias,
                                                uint16_t* __restrict__ dst, int ldd, int nc,
-                                               const uint16_t* __restrict__ mask, int ldm, Vol V, int relu) {
-  const int lane = threadIdx.x & 63;
-  const long tile = (long)blockIdx.x * 4 + (threadIdx.x >> 6);
-  const int tpr = (V.ny + 16 * MB - 1) / (16 * MB);
-  const long rows = (long)V.nz * V.nx;
-  if (tile >= rows * tpr) return;  // wavefront-uniform
-  const int row = (int)(tile / tpr);
-  const int y0 = (int)(tile - (long)row * tpr) * 16 * MB;
-  const int z = row / V.nx, x = row - z * V.nx;
-  const int i = lane & 15, kq = (lane >> 4) * 4;
-  const int nb0 = blockIdx.y * NB, nbt = nc / 16;
+                                               const uint16_t* __restrict__ mask, int ldm, Vol V, int relu,
+                                               int ntz, int ntx, int nty, int per_xcd) {
+  constexpr int TY = 16 * MB, HW = TY + 2, G = CK / 8;
+  constexpr int NQ = (27 * CK + 31) / 32;
+  __shared__ __align__(16) uint16_t halo[16 * HW * CK];
+  const int tid = threadIdx.x, lane = tid & 63, wv = tid >> 6;
+  // blocks b, b + 8, b + 16, ... run on one XCD: give each XCD a contiguous
+  // run of bricks (y fastest), whose halos overlap in its L2
+  const int b = blockIdx.x;
+  const long brick = (long)(b & 7) * per_xcd + (b >> 3);
+  if (brick >= (long)ntz * ntx * nty) return;  // whole block
+  const int yt = (int)(brick % nty);
+  const long r2 = brick / nty;
+  const int xt = (int)(r2 % ntx), zt = (int)(r2 / ntx);
+  const int z0 = 2 * zt, x0 = 2 * xt, y0 = yt * TY;
+  const int wz = wv >> 1, wx = wv & 1;
+  const int i = lane & 15, g = lane >> 4;
+  const int nb0 = blockIdx.y * NB;
   f4 acc[MB][NB];
 #pragma unroll
   for (int m = 0; m < MB; m++)
 #pragma unroll
-    for (int b = 0; b < NB; b++) acc[m][b] = f4{0.f, 0.f, 0.f, 0.f};
-  const int sgn = MODE == 0 ? 1 : -1;
-  for (int t = 0; t < 27; t++) {
-    int dz, dx, dy;
-    tap(t, dz, dx, dy);
-    const int zs = z + sgn * dz, xs = x + sgn * dx;
-    bool ok[MB];
-    long vs[MB];
-#pragma unroll
-    for (int m = 0; m < MB; m++) {
-      const int ys = y0 + 16 * m + i + sgn * dy;
-      ok[m] = V.in(zs, xs, ys);
-      vs[m] = ok[m] ? V.vox(zs, xs, ys) : 0;
+    for (int c = 0; c < NB; c++) acc[m][c] = f4{0.f, 0.f, 0.f, 0.f};
+  for (int c0 = 0; c0 < kc; c0 += CK) {
+    // halo rows r = 4 hz + hx at (z0 - 1 + hz, x0 - 1 + hx), voxels hv at y0 - 1 + hv
+    for (int it = tid; it < 16 * HW * G; it += 256) {
+      const int gr = it % G, rem = it / G;
+      const int hv = rem % HW, r = rem / HW;
+      const int zs = z0 - 1 + (r >> 2), xs = x0 - 1 + (r & 3), ys = y0 - 1 + hv;
+      uint4 q = {0u, 0u, 0u, 0u};
+      if (V.in(zs, xs, ys)) q = *reinterpret_cast<const uint4*>(src + V.vox(zs, xs, ys) * lds + c0 + 8 * gr);
+      *reinterpret_cast<uint4*>(&halo[(r * HW + hv) * CK + 8 * (gr ^ granule_swizzle<CK>(hv))]) = q;
     }
-    for (int kb = 0; kb < kc; kb += 16) {
-      s4 a[MB];
+    __syncthreads();
+    // not unrolled: the compiler would hoist every tap's weight loads
+#pragma unroll 1
+    for (int q = 0; q < NQ; q++) {
+      const int k = 32 * q + 8 * g;
+      const int t = k / CK, gr = (k % CK) >> 3;
+      const bool tv = t < 27;
+      const int tt = tv ? t : 26;
+      const int dz = tt / 9 - 1, dx = (tt / 3) % 3 - 1, dy = tt % 3 - 1;
+      const int row = (wz + dz + 1) * 4 + (wx + dx + 1);
+      uint4 a[MB], bw[NB];
 #pragma unroll
       for (int m = 0; m < MB; m++) {
-        a[m] = s4{0, 0, 0, 0};
-        if (ok[m]) a[m] = *reinterpret_cast<const s4*>(src + vs[m] * lds + kb + kq);
+        const int hv = 16 * m + i + dy + 1;
+        a[m] = *reinterpret_cast<const uint4*>(&halo[(row * HW + hv) * CK + 8 * (gr ^ granule_swizzle<CK>(hv))]);
+        if (!tv) a[m] = uint4{0u, 0u, 0u, 0u};
       }
 #pragma unroll
-      for (int b = 0; b < NB; b++) {
-        if (nb0 + b >= nbt) break;
-        const int n = (nb0 + b) * 16 + i;
-        s4 bw;
+      for (int c = 0; c < NB; c++) {
+        const int n = (nb0 + c) * 16 + i;
         if (MODE == 0) {
-          bw = *reinterpret_cast<const s4*>(w + ((long)t * nc + n) * kc + kb + kq);  // w[t][n][k]
+          bw[c] = *reinterpret_cast<const uint4*>(w + ((long)tt * nc + n) * kc + c0 + 8 * gr);
         } else {
+          const uint16_t* p = w + ((long)(26 - tt) * kc + c0 + 8 * gr) * nc + n;
+          uint16_t e[8];
 #pragma unroll
-          for (int j = 0; j < 4; j++) bw[j] = (short)w[((long)t * kc + kb + kq + j) * nc + n];  // w[t][k][n]
+          for (int j = 0; j < 8; j++) e[j] = p[(long)j * nc];
+          bw[c] = uint4{e[0] | (unsigned)e[1] << 16, e[2] | (unsigned)e[3] << 16, e[4] | (unsigned)e[5] << 16,
+                        e[6] | (unsigned)e[7] << 16};
         }
-#pragma unroll
-        for (int m = 0; m < MB; m++) acc[m][b] = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(a[m], bw, acc[m][b], 0, 0, 0);
+        if (!tv) bw[c] = uint4{0u, 0u, 0u, 0u};
       }
-    }
-  }
 #pragma unroll
-  for (int b = 0; b < NB; b++) {
-    if (nb0 + b >= nbt) break;
-    const int n = (nb0 + b) * 16 + i;
+      for (int c = 0; c < NB; c++)
+#pragma unroll
+        for (int m = 0; m < MB; m++)
+          acc[m][c] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(__builtin_bit_cast(bf8, a[m]), __builtin_bit_cast(bf8, bw[c]),
+                                                              acc[m][c], 0, 0, 0);
+    }
+    __syncthreads();
+  }
+  // D[voxel 4 g + j][channel i] of each (row block, column block)
+  const int z = z0 + wz, x = x0 + wx;
+  if (z >= V.nz || x >= V.nx) return;
+#pragma unroll
+  for (int c = 0; c < NB; c++) {
+    const int n = (nb0 + c) * 16 + i;
     const float bn = bias ? bias[n] : 0.f;
 #pragma unroll
     for (int m = 0; m < MB; m++)
 #pragma unroll
       for (int j = 0; j < 4; j++) {
-        const int y = y0 + 16 * m + kq + j;
+        const int y = y0 + 16 * m + 4 * g + j;
         if (y >= V.ny) continue;
         const long v = V.vox(z, x, y);
-        float r = acc[m][b][j] + bn;
+        float r = acc[m][c][j] + bn;
         if (relu) r = fmaxf(r, 0.f);
         if (mask && !(bf2f(mask[v * ldm + n]) > 0.f)) r = 0.f;
         dst[v * ldd + n] = f2bf(r);
@@ -132,98 +176,118 @@ __global__ __launch_bounds__(256) void k_conv3(const uint16_t* __restrict__ src,
   }
 }
 
-// Weight gradient partials, operands staged through LDS. A block of three
-// wavefronts takes `per` row segments of 16 voxels (a chunk), one 16-channel
-// block of m and NB 16-channel blocks of n; wavefront w takes the 9 taps with
-// dz = w - 1. Per segment the block loads the dpre tile (16 voxels x 16 m)
-// and each wavefront its three rows of src (dx = -1, 0, 1; 18 voxels along y
-// for dy = -1, 0, 1) with 16-byte loads, transposes them into LDS (the src
-// rows once per dy shift, so every MFMA operand is one aligned 8-byte LDS
-// read), and runs 9 x NB MFMAs on one A fragment. Writes
-// part[chunk][t][m][n] for its taps, m and n.
-template <int NB>
-__global__ __launch_bounds__(192) void k_wgrad(const uint16_t* __restrict__ dpre, int ldp, int mc,
+// wt[t][k][n] = w[26 - t][n][k]: the weights of a layer's input gradient as a
+// forward convolution (taps mirrored, channels transposed)
+__global__ void k_flip_weights(const uint16_t* __restrict__ w, int kc, int nc, uint16_t* __restrict__ wt) {
+  const long n_all = 27L * kc * nc;
+  for (long e = (long)blockIdx.x * blockDim.x + threadIdx.x; e < n_all; e += (long)gridDim.x * blockDim.x) {
+    const int k = (int)(e % kc);
+    const long r = e / kc;
+    const int n = (int)(r % nc), t = (int)(r / nc);
+    wt[((long)(26 - t) * kc + k) * nc + n] = w[e];
+  }
+}
+
+// Weight gradient partials on v_mfma_f32_16x16x32_bf16: K = voxels. A block
+// of four wavefronts takes `per` bricks of 2 (z) x 2 (x) rows of TY voxels
+// along y (the convolution's bricks), one 16-channel block of m and one of n.
+// Per brick it copies into LDS, channel-major (y contiguous, rows padded to TY
+// + 8 so that the 16 lanes of a fragment row hit distinct banks): dpre of the
+// brick (zero outside the volume) and the brick's src halo (4 x 4 rows of TY
+// + 2 voxels). Wavefront w takes output row w and every tap: per 32 voxels it
+// reads one A fragment (A[m][k] = dpre[voxel k][m]) and, per (dz, dx), two
+// 16-byte granules of the halo row from which the three dy shifts of the B
+// fragment (B[k][n] = src[voxel k + d_t][n]) follow by dword selection and
+// byte alignment. The four wavefronts' sums meet in LDS, and the block writes
+// part[chunk][t][m][n] for its m and n blocks.
+template <int TY>
+__global__ __launch_bounds__(256) void k_wgrad(const uint16_t* __restrict__ dpre, int ldp, int mc,
                                                const uint16_t* __restrict__ src, int lds, int kc,
-                                               float* __restrict__ part, Vol V, int per) {
-  __shared__ __align__(16) uint16_t At[16 * 16];             // [m][voxel]
-  __shared__ __align__(16) uint16_t Bt[3][3][3][NB * 16][16];  // [dz][dx][dy][n][voxel]
+                                               float* __restrict__ part, Vol V, int ntz, int ntx, int nty, int per) {
+  constexpr int HW = TY + 2, P = TY + 8, NK = TY / 32;
+  constexpr int A_BYTES = 4 * 16 * P * 2, B_BYTES = 16 * 16 * P * 2, R_BYTES = 9 * 4 * 256 * 4;
+  constexpr int S_BYTES = A_BYTES + (B_BYTES > R_BYTES ? B_BYTES : R_BYTES);
+  __shared__ __align__(16) unsigned char smem[S_BYTES];
+  uint16_t* Ah = reinterpret_cast<uint16_t*>(smem);            // [row][m][y]
+  uint16_t* Bh = reinterpret_cast<uint16_t*>(smem + A_BYTES);  // [halo row][n][hv]
+  float* red = reinterpret_cast<float*>(smem + A_BYTES);       // after the last brick
   const int tid = threadIdx.x, wv = tid >> 6, lane = tid & 63;
-  const int chunk = blockIdx.x, mb = blockIdx.y, nb0 = blockIdx.z * NB;
-  const int nbl = min(NB, kc / 16 - nb0);  // n blocks of this block (uniform)
-  const int nvec = nbl * 2;                // 8-channel vectors per voxel
-  const int i = lane & 15, kq = (lane >> 4) * 4;
-  const int tpr = (V.ny + 15) / 16;
-  const long ntiles = (long)V.nz * V.nx * tpr;
-  const int dz = wv - 1;
-  f4 acc[3][3][NB];
+  const int i = lane & 15, g = lane >> 4;
+  const int wz = wv >> 1, wx = wv & 1;
+  const int mb = blockIdx.y, nb = blockIdx.z;
+  const long nbricks = (long)ntz * ntx * nty;
+  const long b0 = (long)blockIdx.x * per, b1 = std::min(nbricks, b0 + per);
+  f4 acc[27];
 #pragma unroll
-  for (int a = 0; a < 3; a++)
-#pragma unroll
-    for (int b = 0; b < 3; b++)
-#pragma unroll
-      for (int c = 0; c < NB; c++) acc[a][b][c] = f4{0.f, 0.f, 0.f, 0.f};
-  const long t0 = (long)chunk * per, t1 = std::min(ntiles, t0 + per);
-  for (long tile = t0; tile < t1; tile++) {
-    const int row = (int)(tile / tpr);
-    const int y0 = (int)(tile - (long)row * tpr) * 16;
-    const int z = row / V.nx, x = row - z * V.nx;
-    // A tile: voxel v, channels h..h+7 of the m block (threads 0..31)
-    if (tid < 32) {
-      const int v = tid >> 1, h = (tid & 1) * 8;
+  for (int t = 0; t < 27; t++) acc[t] = f4{0.f, 0.f, 0.f, 0.f};
+  for (long brick = b0; brick < b1; brick++) {
+    const int yt = (int)(brick % nty);
+    const long r2 = brick / nty;
+    const int xt = (int)(r2 % ntx), zt = (int)(r2 / ntx);
+    const int z0 = 2 * zt, x0 = 2 * xt, y0 = yt * TY;
+    for (int it = tid; it < 4 * TY * 2; it += 256) {
+      const int gr = it & 1, rem = it >> 1;
+      const int y = rem % TY, row = rem / TY;
+      const int z = z0 + (row >> 1), x = x0 + (row & 1), yy = y0 + y;
       uint4 q = {0u, 0u, 0u, 0u};
-      if (y0 + v < V.ny) q = *reinterpret_cast<const uint4*>(dpre + V.vox(z, x, y0 + v) * ldp + mb * 16 + h);
+      if (V.in(z, x, yy)) q = *reinterpret_cast<const uint4*>(dpre + V.vox(z, x, yy) * ldp + mb * 16 + 8 * gr);
       const uint16_t* e = reinterpret_cast<const uint16_t*>(&q);
 #pragma unroll
-      for (int j = 0; j < 8; j++) At[(h + j) * 16 + v] = e[j];
+      for (int j = 0; j < 8; j++) Ah[(row * 16 + 8 * gr + j) * P + y] = e[j];
     }
-    // this wavefront's src rows: voxel vv (y = y0 - 1 + vv) is operand column
-    // k = vv - dyi of shift dyi (dy = dyi - 1)
-    const int items = 3 * 18 * nvec;
-    for (int it = lane; it < items; it += 64) {
-      const int dxi = it / (18 * nvec), r = it - dxi * 18 * nvec;
-      const int vv = r / nvec, c8 = (r - vv * nvec) * 8;
-      const int zs = z + dz, xs = x + dxi - 1, ys = y0 - 1 + vv;
+    for (int it = tid; it < 16 * HW * 2; it += 256) {
+      const int gr = it & 1, rem = it >> 1;
+      const int hv = rem % HW, r = rem / HW;
+      const int zs = z0 - 1 + (r >> 2), xs = x0 - 1 + (r & 3), ys = y0 - 1 + hv;
       uint4 q = {0u, 0u, 0u, 0u};
-      if (V.in(zs, xs, ys)) q = *reinterpret_cast<const uint4*>(src + V.vox(zs, xs, ys) * lds + nb0 * 16 + c8);
+      if (V.in(zs, xs, ys)) q = *reinterpret_cast<const uint4*>(src + V.vox(zs, xs, ys) * lds + nb * 16 + 8 * gr);
       const uint16_t* e = reinterpret_cast<const uint16_t*>(&q);
 #pragma unroll
-      for (int dyi = 0; dyi < 3; dyi++) {
-        const int k = vv - dyi;
-        if (k >= 0 && k < 16) {
-#pragma unroll
-          for (int j = 0; j < 8; j++) Bt[wv][dxi][dyi][c8 + j][k] = e[j];
-        }
-      }
+      for (int j = 0; j < 8; j++) Bh[(r * 16 + 8 * gr + j) * P + hv] = e[j];
     }
     __syncthreads();
-    const s4 a = *reinterpret_cast<const s4*>(&At[i * 16 + kq]);  // A[m][k] = dpre[voxel k][m]
+#pragma unroll 1
+    for (int c = 0; c < NK; c++) {
+      const uint4 a = *reinterpret_cast<const uint4*>(&Ah[(wv * 16 + i) * P + 32 * c + 8 * g]);
 #pragma unroll
-    for (int dxi = 0; dxi < 3; dxi++)
-#pragma unroll
-      for (int dyi = 0; dyi < 3; dyi++)
-#pragma unroll
-        for (int b = 0; b < NB; b++) {
-          if (b >= nbl) break;
-          // B[k][n] = src[voxel k + d_t][n]
-          const s4 bw = *reinterpret_cast<const s4*>(&Bt[wv][dxi][dyi][b * 16 + i][kq]);
-          acc[dxi][dyi][b] = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(a, bw, acc[dxi][dyi][b], 0, 0, 0);
-        }
+      for (int dzdx = 0; dzdx < 9; dzdx++) {
+        const int hr = (wz + dzdx / 3) * 4 + (wx + dzdx % 3);
+        const uint16_t* bp = &Bh[(hr * 16 + i) * P + 32 * c + 8 * g];
+        const uint4 g0 = *reinterpret_cast<const uint4*>(bp);
+        const uint4 g1 = *reinterpret_cast<const uint4*>(bp + 8);
+        // halo voxels 32c + 8g + (dy + 1) + j, j = 0..7
+        const uint4 s1 = {__builtin_amdgcn_alignbyte(g0.y, g0.x, 2), __builtin_amdgcn_alignbyte(g0.z, g0.y, 2),
+                          __builtin_amdgcn_alignbyte(g0.w, g0.z, 2), __builtin_amdgcn_alignbyte(g1.x, g0.w, 2)};
+        const uint4 s2 = {g0.y, g0.z, g0.w, g1.x};
+        const bf8 av = __builtin_bit_cast(bf8, a);
+        acc[3 * dzdx] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(av, __builtin_bit_cast(bf8, g0), acc[3 * dzdx], 0, 0, 0);
+        acc[3 * dzdx + 1] =
+            __builtin_amdgcn_mfma_f32_16x16x32_bf16(av, __builtin_bit_cast(bf8, s1), acc[3 * dzdx + 1], 0, 0, 0);
+        acc[3 * dzdx + 2] =
+            __builtin_amdgcn_mfma_f32_16x16x32_bf16(av, __builtin_bit_cast(bf8, s2), acc[3 * dzdx + 2], 0, 0, 0);
+      }
+    }
     __syncthreads();
   }
-  // D[m][n]: lane holds m = mb*16 + kq + j, n = (nb0 + b)*16 + i
+  // sum of the four wavefronts, 9 taps at a time; lane (i, g) holds
+  // D[m = 4 g + j][n = i] of each tap
 #pragma unroll
-  for (int dxi = 0; dxi < 3; dxi++)
+  for (int p = 0; p < 3; p++) {
 #pragma unroll
-    for (int dyi = 0; dyi < 3; dyi++) {
-      const int t = 9 * wv + 3 * dxi + dyi;
-      float* out = part + ((long)chunk * 27 + t) * mc * kc;
+    for (int tt = 0; tt < 9; tt++)
 #pragma unroll
-      for (int b = 0; b < NB; b++) {
-        if (b >= nbl) break;
+      for (int j = 0; j < 4; j++) red[(tt * 4 + wv) * 256 + lane * 4 + j] = acc[9 * p + tt][j];
+    __syncthreads();
+    const int l = tid >> 2, j = tid & 3;
+    const int m = mb * 16 + 4 * (l >> 4) + j, n = nb * 16 + (l & 15);
 #pragma unroll
-        for (int j = 0; j < 4; j++) out[(long)(mb * 16 + kq + j) * kc + (nb0 + b) * 16 + i] = acc[dxi][dyi][b][j];
-      }
+    for (int tt = 0; tt < 9; tt++) {
+      const float s = red[(tt * 4 + 0) * 256 + tid] + red[(tt * 4 + 1) * 256 + tid] + red[(tt * 4 + 2) * 256 + tid] +
+                      red[(tt * 4 + 3) * 256 + tid];
+      part[(((long)blockIdx.x * 27 + 9 * p + tt) * mc + m) * kc + n] = s;
     }
+    __syncthreads();
+  }
 }
 
 // out[k] += sum over chunks [blockIdx.y * kChunkGroup, +kChunkGroup) of part[chunk][k]
@@ -253,6 +317,34 @@ __global__ void k_chan_sum(const uint16_t* __restrict__ x, int ld, int c, long n
   if (threadIdx.x < c) {
     float t = 0.f;
     for (int q = 0; q < lanes; q++) t += red[q * c + threadIdx.x];
+    atomicAdd(out + threadIdx.x, t);
+  }
+}
+// The same with one 16-byte load (8 channels) per thread and voxel: blockDim
+// 256 = (256 / (c / 8)) voxel lanes x (c / 8) granules (c and ld multiples of
+// 8, x 16-byte aligned).
+__global__ void k_chan_sum8(const uint16_t* __restrict__ x, int ld, int c, long nvox, float* __restrict__ out) {
+  __shared__ float red[256 * 8];
+  const int G = c / 8, lanes = 256 / G;
+  const int gr = threadIdx.x % G, vl = threadIdx.x / G;
+  float s[8] = {0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f};
+  if (vl < lanes)
+    for (long v = (long)blockIdx.x * lanes + vl; v < nvox; v += (long)gridDim.x * lanes) {
+      const uint4 q = *reinterpret_cast<const uint4*>(x + v * ld + 8 * gr);
+      const unsigned u[4] = {q.x, q.y, q.z, q.w};
+#pragma unroll
+      for (int j = 0; j < 4; j++) {
+        s[2 * j] += __uint_as_float(u[j] << 16);
+        s[2 * j + 1] += __uint_as_float(u[j] & 0xFFFF0000u);
+      }
+    }
+#pragma unroll
+  for (int j = 0; j < 8; j++) red[threadIdx.x * 8 + j] = s[j];
+  __syncthreads();
+  if (threadIdx.x < c) {
+    const int g = threadIdx.x / 8, j = threadIdx.x % 8;
+    float t = 0.f;
+    for (int q = 0; q < lanes; q++) t += red[(q * G + g) * 8 + j];
     atomicAdd(out + threadIdx.x, t);
   }
 }
@@ -474,42 +566,69 @@ static int nn_check(hipStream_t s) {
 int efwi_nn_conv3(const uint16_t* src, int lds, int kc, const uint16_t* w, const float* bias, uint16_t* dst, int ldd,
                   int nc, const uint16_t* mask, int ldm, int nz, int nx, int ny, int mode, int relu, void* stream) {
   if (!src || !w || !dst || kc <= 0 || nc <= 0 || kc % 16 || nc % 16 || lds % 16 || ldd % 16 || lds < kc ||
-      ldd < nc || (mask && (ldm % 16 || ldm < nc)) || nz <= 0 || nx <= 0 || ny <= 0 || (mode != 0 && mode != 1))
+      ldd < nc || (mask && (ldm % 16 || ldm < nc)) || nz <= 0 || nx <= 0 || ny <= 0 || (mode != 0 && mode != 1) ||
+      ((uintptr_t)src | (uintptr_t)w) % 16)
     return EFWI_NN_ERR_ARG;
   hipStream_t s = (hipStream_t)stream;
   const Vol V{nz, nx, ny};
-  // row segments of 64 voxels (4 MFMA row blocks) where the rows are long enough
-  const int MBv = ny >= 48 ? 4 : 1;
-  const long tiles = (long)nz * nx * ((ny + 16 * MBv - 1) / (16 * MBv));
+  // bricks of 2 x 2 rows of 64, 32 or 16 voxels along y; 1 to 4 output blocks
+  // per wavefront (a divisor of the block count, so no block is partial)
+  const int MB = ny >= 64 ? 4 : ny >= 32 ? 2 : 1;
   const int nbt = nc / 16;
-  const int NB = nbt >= 4 ? 4 : nbt;  // output blocks per wavefront
-  const dim3 grid((unsigned)((tiles + 3) / 4), (unsigned)((nbt + NB - 1) / NB));
-#define EFWI_NN_CONV_M(B, M)                                                                                       \
-  if (mode == 0)                                                                                                 \
-    hipLaunchKernelGGL((k_conv3<B, M, 0>), grid, dim3(256), 0, s, src, lds, kc, w, bias, dst, ldd, nc, mask, ldm, V, relu); \
-  else                                                                                                           \
-    hipLaunchKernelGGL((k_conv3<B, M, 1>), grid, dim3(256), 0, s, src, lds, kc, w, bias, dst, ldd, nc, mask, ldm, V, relu)
-#define EFWI_NN_CONV(B)                \
-  if (MBv == 4) { EFWI_NN_CONV_M(B, 4); } \
-  else { EFWI_NN_CONV_M(B, 1); }
-  if (NB == 1) { EFWI_NN_CONV(1); }
-  else if (NB == 2) { EFWI_NN_CONV(2); }
-  else if (NB == 3) { EFWI_NN_CONV(3); }
-  else { EFWI_NN_CONV(4); }
-#undef EFWI_NN_CONV
-#undef EFWI_NN_CONV_M
+  const int NB = nbt % 4 == 0 ? 4 : nbt % 3 == 0 ? 3 : nbt % 2 == 0 ? 2 : 1;
+  const int ntz = (nz + 1) / 2, ntx = (nx + 1) / 2, nty = (ny + 16 * MB - 1) / (16 * MB);
+  const long bricks = (long)ntz * ntx * nty;
+  const int per_xcd = (int)((bricks + 7) / 8);
+  const dim3 grid((unsigned)(8L * per_xcd), (unsigned)(nbt / NB));
+  const bool ck32 = kc % 32 == 0;
+#define EFWI_NN_CONV_LAUNCH(CK, M, B, MODE)                                                                       \
+  hipLaunchKernelGGL((k_conv3<CK, M, B, MODE>), grid, dim3(256), 0, s, src, lds, kc, w, bias, dst, ldd, nc, mask, \
+                     ldm, V, relu, ntz, ntx, nty, per_xcd)
+#define EFWI_NN_CONV_MODE(CK, M, B)                         \
+  if (mode == 0) { EFWI_NN_CONV_LAUNCH(CK, M, B, 0); }    \
+  else { EFWI_NN_CONV_LAUNCH(CK, M, B, 1); }
+#define EFWI_NN_CONV_NB(CK, M)                              \
+  if (NB == 4) { EFWI_NN_CONV_MODE(CK, M, 4) }             \
+  else if (NB == 3) { EFWI_NN_CONV_MODE(CK, M, 3) }        \
+  else if (NB == 2) { EFWI_NN_CONV_MODE(CK, M, 2) }        \
+  else { EFWI_NN_CONV_MODE(CK, M, 1) }
+#define EFWI_NN_CONV_MB(CK)                                 \
+  if (MB == 4) { EFWI_NN_CONV_NB(CK, 4) }                  \
+  else if (MB == 2) { EFWI_NN_CONV_NB(CK, 2) }             \
+  else { EFWI_NN_CONV_NB(CK, 1) }
+  if (ck32) { EFWI_NN_CONV_MB(32) }
+  else { EFWI_NN_CONV_MB(16) }
+#undef EFWI_NN_CONV_MB
+#undef EFWI_NN_CONV_NB
+#undef EFWI_NN_CONV_MODE
+#undef EFWI_NN_CONV_LAUNCH
   return nn_check(s);
 }
 
+int efwi_nn_flip_weights(const uint16_t* w, int kc, int nc, uint16_t* wt, void* stream) {
+  if (!w || !wt || kc <= 0 || nc <= 0 || w == wt) return EFWI_NN_ERR_ARG;
+  hipStream_t s = (hipStream_t)stream;
+  hipLaunchKernelGGL(k_flip_weights, dim3(grid_for(27L * kc * nc, 256)), dim3(256), 0, s, w, kc, nc, wt);
+  return nn_check(s);
+}
+
+// bricks of the weight gradient: 2 x 2 rows of 64 voxels along y, or of 32
+// where the rows are that short
+static inline int wgrad_ty(int ny) { return ny > 32 ? 64 : 32; }
+static inline long wgrad_bricks(int nz, int nx, int ny) {
+  const int ty = wgrad_ty(ny);
+  return (long)((nz + 1) / 2) * ((nx + 1) / 2) * ((ny + ty - 1) / ty);
+}
+
 long efwi_nn_wgrad_scratch(int mc, int kc, int nz, int nx, int ny, int per) {
-  const long tiles = (long)nz * nx * ((ny + 15) / 16);
-  const long chunks = (tiles + per - 1) / per;
+  if (per <= 0 || nz <= 0 || nx <= 0 || ny <= 0) return 0;
+  const long chunks = (wgrad_bricks(nz, nx, ny) + per - 1) / per;
   return chunks * 27L * mc * kc;
 }
 
 static bool wgrad_args_ok(const uint16_t* dpre, int ldp, int mc, const uint16_t* src, int lds, int kc, float* dw,
                           int nz, int nx, int ny, int per) {
-  return dpre && src && dw && mc % 16 == 0 && kc % 16 == 0 && mc > 0 && kc > 0 && kc <= 96 && per > 0 &&
+  return dpre && src && dw && mc % 16 == 0 && kc % 16 == 0 && mc > 0 && kc > 0 && per > 0 &&
          ldp % 16 == 0 && lds % 16 == 0 && ldp >= mc && lds >= kc && nz > 0 && nx > 0 && ny > 0 &&
          ((uintptr_t)dpre | (uintptr_t)src) % 16 == 0;
 }
@@ -563,12 +682,14 @@ int efwi_nn_wgrad_noalloc(const uint16_t* dpre, int ldp, int mc, const uint16_t*
 static int wgrad_launch(const uint16_t* dpre, int ldp, int mc, const uint16_t* src, int lds, int kc, float* scratch,
                         float* dw, int nz, int nx, int ny, int per, hipStream_t s) {
   const Vol V{nz, nx, ny};
-  const long tiles = (long)nz * nx * ((ny + 15) / 16);
-  const int chunks = (int)((tiles + per - 1) / per);
-  // one 16-channel n block per workgroup: two per workgroup (A tiles shared)
-  // measured slower, at half the occupancy (185 registers against 112)
-  hipLaunchKernelGGL((k_wgrad<1>), dim3((unsigned)chunks, (unsigned)(mc / 16), (unsigned)(kc / 16)), dim3(192), 0, s,
-                     dpre, ldp, mc, src, lds, kc, scratch, V, per);
+  const int ty = wgrad_ty(ny);
+  const int ntz = (nz + 1) / 2, ntx = (nx + 1) / 2, nty = (ny + ty - 1) / ty;
+  const int chunks = (int)((wgrad_bricks(nz, nx, ny) + per - 1) / per);
+  const dim3 grid((unsigned)chunks, (unsigned)(mc / 16), (unsigned)(kc / 16));
+  if (ty == 64)
+    hipLaunchKernelGGL((k_wgrad<64>), grid, dim3(256), 0, s, dpre, ldp, mc, src, lds, kc, scratch, V, ntz, ntx, nty, per);
+  else
+    hipLaunchKernelGGL((k_wgrad<32>), grid, dim3(256), 0, s, dpre, ldp, mc, src, lds, kc, scratch, V, ntz, ntx, nty, per);
   const long width = 27L * mc * kc;
   hipLaunchKernelGGL(k_zero, dim3((unsigned)((width + 255) / 256)), dim3(256), 0, s, reinterpret_cast<unsigned*>(dw),
                      (int)width);
@@ -578,10 +699,13 @@ static int wgrad_launch(const uint16_t* dpre, int ldp, int mc, const uint16_t* s
 }
 
 int efwi_nn_chan_sum(const uint16_t* x, int ld, int c, long nvox, float* out, void* stream) {
-  if (!x || !out || c <= 0 || c > 256 || 256 % c) return EFWI_NN_ERR_ARG;
+  if (!x || !out || c <= 0 || c > 256 || 256 % c || ld < c) return EFWI_NN_ERR_ARG;
   hipStream_t s = (hipStream_t)stream;
   hipLaunchKernelGGL(k_zero, dim3((c + 255) / 256), dim3(256), 0, s, reinterpret_cast<unsigned*>(out), c);
-  hipLaunchKernelGGL(k_chan_sum, dim3(grid_for(nvox * c, 256, 256)), dim3(256), 0, s, x, ld, c, nvox, out);
+  if (c % 8 == 0 && ld % 8 == 0 && (uintptr_t)x % 16 == 0)
+    hipLaunchKernelGGL(k_chan_sum8, dim3(grid_for(nvox * c / 8, 256, 2048)), dim3(256), 0, s, x, ld, c, nvox, out);
+  else
+    hipLaunchKernelGGL(k_chan_sum, dim3(grid_for(nvox * c, 256, 2048)), dim3(256), 0, s, x, ld, c, nvox, out);
   return nn_check(s);
 }
 

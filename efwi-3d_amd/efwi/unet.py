@@ -41,7 +41,7 @@ LAYERS = [
     ("dec1a", 48, 16, 1), ("dec1b", 16, 16, 1),
 ]
 IN_CHANNELS = 2       # raw gradient, log illumination
-WGRAD_PER = 64        # 16-voxel segments per weight-gradient partial
+WGRAD_PER = 8         # bricks (2 x 2 rows of up to 64 voxels) per weight-gradient partial
 
 
 def nn_check(rc: int, what: str):
@@ -79,6 +79,7 @@ class UNet3D:
         self.w = {k: v.to(self.dev) for k, v in w.items()}
         self.b = {k: v.to(self.dev) for k, v in b.items()}
         self.w16 = {}
+        self.wt16 = {}
         self.wh = wh.to(self.dev)
         self.bh = bh.to(self.dev)
         self._refresh_bf16()
@@ -101,6 +102,16 @@ class UNet3D:
     def _refresh_bf16(self):
         for name, *_ in LAYERS:
             self.w16[name] = self.w[name].to(torch.bfloat16)
+            self.wt16[name] = torch.empty_like(self.w16[name])
+        self._flip()
+
+    def _flip(self):
+        """The input-gradient weights (taps mirrored, channels transposed)
+        from the BF16 weights, for the mode-0 convolution of the backward."""
+        s = self._stream()
+        for name, cin, cout, _ in LAYERS:
+            nn_check(self.lib.efwi_nn_flip_weights(_p(self.w16[name]), cin, cout, _p(self.wt16[name]), s),
+                     f"flip {name}")
 
     def state_dict(self):
         return {k: v.detach().cpu().clone() for k, v in self._params().items()}
@@ -152,8 +163,8 @@ class UNet3D:
 
     def _dgrad(self, name, gout, ldg, gin, ldi, mask, ldm, dims, goff=0, ioff=0, moff=0):
         _, cin, cout, _ = next(l for l in LAYERS if l[0] == name)
-        nn_check(self.lib.efwi_nn_conv3(_p(gout, goff), ldg, cout, _p(self.w16[name]), None, _p(gin, ioff), ldi,
-                                        cin, None if mask is None else _p(mask, moff), ldm, *dims, 1, 0,
+        nn_check(self.lib.efwi_nn_conv3(_p(gout, goff), ldg, cout, _p(self.wt16[name]), None, _p(gin, ioff), ldi,
+                                        cin, None if mask is None else _p(mask, moff), ldm, *dims, 0, 0,
                                         self._stream()), f"dgrad {name}")
 
     def _wgrad(self, name, gpre, ldg, src, lds, dims, goff=0, soff=0):
@@ -258,6 +269,7 @@ class UNet3D:
             nn_check(self.lib.efwi_nn_adam(_p(w), _p(self.grads[k]), _p(self.m1[k]), _p(self.m2[k]), w.numel(),
                                            lr, b1, b2, eps, self.steps, None if w16 is None else _p(w16), s),
                      f"adam {k}")
+        self._flip()
 
     def train_step(self, feat, target, mask, count, dims, lr=1e-3, log_scaling=False):
         """One Adam step on the mean-square error of the update (target: the

@@ -27,26 +27,35 @@ extern "C" {
 #define EFWI_NN_ERR_SCRATCH 3 /* efwi_nn_wgrad_noalloc: partial sums beyond EFWI_NN_WGRAD_STATIC_FLOATS */
 
 /* 3x3x3 convolution with zero padding, BF16 in and out, FP32 accumulation on
- * the matrix cores (v_mfma_f32_16x16x16_bf16).
+ * the matrix cores (v_mfma_f32_16x16x32_bf16, the input staged through LDS).
  *   mode 0 (forward): dst[v][n] = relu?(sum_t,k src[v + d_t][k] w[t][n][k] + bias[n]),
  *                     kc = cin, nc = cout, bias may be NULL;
  *   mode 1 (input gradient): dst[v][n] = sum_t,k src[v - d_t][k] w[t][k][n],
  *                     src = output gradient (kc = cout), nc = cin.
+ * Mode 1 gathers its weights with 2-byte loads; mode 0 with the weights of
+ * efwi_nn_flip_weights computes the same input gradient at the forward rate.
  * When mask is not NULL, dst[v][n] is zeroed where mask[v * ldm + n] <= 0 (the
  * ReLU derivative of a layer's activation). kc, nc, lds, ldd, ldm: multiples
- * of 16, lds >= kc, ldd >= nc. */
+ * of 16, lds >= kc, ldd >= nc; src and w 16-byte aligned (a channel slice
+ * starts at a multiple of 8 channels). */
 int efwi_nn_conv3(const uint16_t* src, int lds, int kc, const uint16_t* w, const float* bias, uint16_t* dst,
                   int ldd, int nc, const uint16_t* mask, int ldm, int nz, int nx, int ny, int mode, int relu,
                   void* stream);
 
+/* wt[t][k][n] = w[26 - t][n][k] for a layer's weights w[27][nc][kc]: the
+ * input gradient of that layer is the mode-0 convolution of the output
+ * gradient (kc' = nc input channels, nc' = kc outputs) with wt. wt must not
+ * alias w. */
+int efwi_nn_flip_weights(const uint16_t* w, int kc, int nc, uint16_t* wt, void* stream);
+
 /* Weight gradient of a 3x3x3 convolution, dw[t][m][n] = sum_v dpre[v][m]
  * src[v + d_t][n] (FP32 result), dpre = gradient of the layer's
- * pre-activation (mc channels), src = its input (kc <= 96 channels); dpre and
- * src 16-byte aligned (a channel slice starts at a multiple of 8 channels).
- * Each workgroup reduces `per` segments of 16 voxels into FP32 partials in
- * `scratch` (efwi_nn_wgrad_scratch floats), which are then summed; its
- * operands go through LDS with 16-byte loads and each A fragment feeds the 9
- * taps of a wavefront. */
+ * pre-activation (mc channels), src = its input (kc channels); dpre and src
+ * 16-byte aligned (a channel slice starts at a multiple of 8 channels). The
+ * volume is cut into bricks of 2 x 2 rows of 64 voxels along y (32 where ny
+ * <= 32); each workgroup reduces `per` bricks for one 16 x 16 block of (m, n)
+ * on v_mfma_f32_16x16x32_bf16 into FP32 partials in `scratch`
+ * (efwi_nn_wgrad_scratch floats), which are then summed. */
 long efwi_nn_wgrad_scratch(int mc, int kc, int nz, int nx, int ny, int per);
 int efwi_nn_wgrad(const uint16_t* dpre, int ldp, int mc, const uint16_t* src, int lds, int kc, float* scratch,
                   long scratch_floats, float* dw, int nz, int nx, int ny, int per, void* stream);
@@ -57,14 +66,16 @@ int efwi_nn_wgrad(const uint16_t* dpre, int ldp, int mc, const uint16_t* src, in
  * process's lifetime. Calls whose efwi_nn_wgrad_scratch(mc, kc, nz, nx, ny,
  * per) exceeds it return EFWI_NN_ERR_SCRATCH (raise `per`). Every call on a
  * GPU uses that GPU's one array: calls that overlap on one GPU (concurrent
- * streams or threads) race. Results equal efwi_nn_wgrad's bit for bit. No
+ * streams or threads) race. Same kernels as efwi_nn_wgrad: results equal its
+ * results up to the order of the FP32 atomic additions that combine groups
+ * of 32 chunks (which varies between calls once there are three or more). No
  * counterpart in the reference; for callers that cannot allocate device
  * memory (code under graph capture, training loops that forbid it). */
 #define EFWI_NN_WGRAD_STATIC_FLOATS (4L << 20)
 int efwi_nn_wgrad_noalloc(const uint16_t* dpre, int ldp, int mc, const uint16_t* src, int lds, int kc, float* dw,
                           int nz, int nx, int ny, int per, void* stream);
 
-/* out[c] = sum_v x[v][c] (bias gradients), c divides 256. */
+/* out[c] = sum_v x[v][c] (bias gradients), c divides 256, ld >= c. */
 int efwi_nn_chan_sum(const uint16_t* x, int ld, int c, long nvox, float* out, void* stream);
 
 /* 2x2x2 average pooling of a (nz, nx, ny) volume (even sizes) and its
