@@ -59,21 +59,33 @@ __device__ __forceinline__ void tap(int t, int& dz, int& dx, int& dy) {
 // channels; wavefront w takes row (z0 + w / 2, x0 + w % 2), MB row blocks of
 // the A operand (16 voxels each) against NB column blocks. Per chunk of CK
 // input channels the block copies the brick's halo (4 x 4 rows of TY + 2
-// voxels, zero outside the volume) into LDS with 16-byte loads; every tap
-// then reads its A fragments from there with one 16-byte LDS read per lane
-// (the 8-channel granules of a voxel are XOR-swizzled with its position so
-// that the 16 lanes of a fragment row hit distinct banks). The K dimension
-// runs over (tap, channel) of the chunk in steps of 32: lane group g = l / 16
-// holds K entries 8 g .. 8 g + 7, i.e. taps t = (32 q + 8 g) / CK, so with
-// CK = 16 one MFMA covers two taps. B fragments (weights) come from global
-// memory: w[t][n][k] in mode 0, one 16-byte load per lane; mode 1 (input
-// gradient with the forward layer's weights, taps flipped, B[k][n] =
+// voxels, zero outside the volume) into LDS with 16-byte loads: a thread
+// keeps one (voxel, granule) pair and walks the 16 rows, whose offsets are
+// uniform. Every tap then reads its A fragments from there with one 16-byte
+// LDS read per lane (the 8-channel granules of a voxel are XOR-swizzled with
+// its position so that each 8-lane phase of a read hits distinct banks). The
+// K dimension runs over (tap, channel) of the chunk in steps of 32: lane group
+// g = l / 16 holds K entries 8 g .. 8 g + 7, i.e. taps t = (32 q + 8 g) / CK,
+// so with CK = 16 one MFMA covers two taps (the lower and upper lane halves).
+// A lane's LDS offsets are a per-lane base (one per dy) plus a constant per
+// tap. B fragments (weights): w[t][n][k] in mode 0, from LDS where the chunk's
+// weights are small (CK = 16, NB = 1; the K loop is then unrolled and its
+// offsets fold to constants), else one 16-byte global load per lane; mode 1
+// (input gradient with the forward layer's weights, taps flipped, B[k][n] =
 // w[26 - t][k][n]) gathers them with 2-byte loads, which a caller avoids by
 // passing the flipped copy of efwi_nn_flip_weights to mode 0.
 template <int CK>
 __device__ __forceinline__ int granule_swizzle(int hv) {
-  // G = CK / 8 granules of 16 bytes per voxel; 16 / G voxels per 64-bank row
-  return CK == 32 ? ((hv >> 2) & 3) : ((hv >> 3) & 1);
+  // G = CK / 8 granules of 16 bytes per voxel. A 16-byte LDS read is served
+  // 8 lanes at a time across 32 banks: the granules of 8 consecutive voxels
+  // must cover distinct 16-byte bank groups (measured on MI355X: no bank
+  // conflicts with this swizzle, against 0.66 M conflict cycles without one
+  // and 7.5 M with a swizzle for 16-lane, 64-bank service)
+  return CK == 32 ? ((hv >> 1) & 3) : ((hv >> 2) & 1);
+}
+
+__device__ __forceinline__ uint4 lds_read16(const unsigned char* base, int byte_off) {
+  return *reinterpret_cast<const uint4*>(base + byte_off);
 }
 
 template <int CK, int MB, int NB, int MODE>
@@ -84,7 +96,12 @@ __global__ __launch_bounds__(256) void k_conv3(const uint16_t* __restrict__ src,
                                                int ntz, int ntx, int nty, int per_xcd) {
   constexpr int TY = 16 * MB, HW = TY + 2, G = CK / 8;
   constexpr int NQ = (27 * CK + 31) / 32;
-  __shared__ __align__(16) uint16_t halo[16 * HW * CK];
+  constexpr int H_PAIRS = HW * G, H_NPASS = (H_PAIRS + 255) / 256;
+  constexpr bool WL = CK == 16 && NB == 1;
+  constexpr int NBC = NB * 16;
+  constexpr int VB = CK * 2;  // bytes per voxel in the halo
+  __shared__ __align__(16) unsigned char halo[16 * HW * VB];
+  __shared__ __align__(16) unsigned char wl[WL ? 27 * NBC * VB : 16];
   const int tid = threadIdx.x, lane = tid & 63, wv = tid >> 6;
   // blocks b, b + 8, b + 16, ... run on one XCD: give each XCD a contiguous
   // run of bricks (y fastest), whose halos overlap in its L2
@@ -95,55 +112,121 @@ __global__ __launch_bounds__(256) void k_conv3(const uint16_t* __restrict__ src,
   const long r2 = brick / nty;
   const int xt = (int)(r2 % ntx), zt = (int)(r2 / ntx);
   const int z0 = 2 * zt, x0 = 2 * xt, y0 = yt * TY;
+  const bool interior = z0 >= 1 && z0 + 2 < V.nz && x0 >= 1 && x0 + 2 < V.nx && y0 >= 1 && y0 + TY < V.ny;
   const int wz = wv >> 1, wx = wv & 1;
   const int i = lane & 15, g = lane >> 4;
+  const int hi = CK == 16 ? g >> 1 : 0;    // CK = 16: upper lane half takes the odd tap
+  const int grl = CK == 16 ? g & 1 : g;    // granule of the lane's 8 K entries
   const int nb0 = blockIdx.y * NB;
+  // LDS byte offsets of the lane's A fragment for tap (dz, dx, dy) = (-1, -1, dy)
+  // and row block 0: abase[dy + 1]; other taps and blocks add constants
+  int abase[3];
+#pragma unroll
+  for (int d = 0; d < 3; d++) abase[d] = ((wz * 4 + wx) * HW + i + d) * VB + 16 * (grl ^ granule_swizzle<CK>(i + d));
+  const int wbase = (i * CK + 8 * (grl ^ granule_swizzle<CK>(i))) * 2 + hi * NBC * VB;
+  // halo corner voxel (z0 - 1, x0 - 1, y0 - 1)
+  const long corner = ((long)(z0 - 1) * V.nx + (x0 - 1)) * V.ny + (y0 - 1);
   f4 acc[MB][NB];
 #pragma unroll
   for (int m = 0; m < MB; m++)
 #pragma unroll
     for (int c = 0; c < NB; c++) acc[m][c] = f4{0.f, 0.f, 0.f, 0.f};
   for (int c0 = 0; c0 < kc; c0 += CK) {
-    // halo rows r = 4 hz + hx at (z0 - 1 + hz, x0 - 1 + hx), voxels hv at y0 - 1 + hv
-    for (int it = tid; it < 16 * HW * G; it += 256) {
-      const int gr = it % G, rem = it / G;
-      const int hv = rem % HW, r = rem / HW;
-      const int zs = z0 - 1 + (r >> 2), xs = x0 - 1 + (r & 3), ys = y0 - 1 + hv;
-      uint4 q = {0u, 0u, 0u, 0u};
-      if (V.in(zs, xs, ys)) q = *reinterpret_cast<const uint4*>(src + V.vox(zs, xs, ys) * lds + c0 + 8 * gr);
-      *reinterpret_cast<uint4*>(&halo[(r * HW + hv) * CK + 8 * (gr ^ granule_swizzle<CK>(hv))]) = q;
+    // halo rows r = 4 hz + hx at (z0 - 1 + hz, x0 - 1 + hx), voxels hv at
+    // y0 - 1 + hv. A thread keeps one (hv, granule) pair and walks the 16
+    // rows (row offsets are uniform), eight loads in flight before the
+    // stores.
+#pragma unroll
+    for (int v = 0; v < H_NPASS; v++) {
+      const int pidx = tid + 256 * v;
+      if (H_PAIRS % 256 == 0 || pidx < H_PAIRS) {
+        const int gr = pidx % G, hv = pidx / G;
+        const int ys = y0 - 1 + hv;
+        const bool y_ok = interior || (ys >= 0 && ys < V.ny);
+        const uint16_t* tp = src + (corner + hv) * lds + c0 + 8 * gr;
+        unsigned char* lp = halo + hv * VB + 16 * (gr ^ granule_swizzle<CK>(hv));
+#pragma unroll
+        for (int r0 = 0; r0 < 16; r0 += 8) {
+          uint4 hq[8];
+#pragma unroll
+          for (int rr = 0; rr < 8; rr++) {
+            const int hz = (r0 + rr) >> 2, hx = (r0 + rr) & 3;
+            const bool ok = interior || (y_ok && (unsigned)(z0 - 1 + hz) < (unsigned)V.nz &&
+                                         (unsigned)(x0 - 1 + hx) < (unsigned)V.nx);
+            hq[rr] = uint4{0u, 0u, 0u, 0u};
+            if (ok) hq[rr] = *reinterpret_cast<const uint4*>(tp + ((long)hz * V.nx + hx) * V.ny * lds);
+          }
+#pragma unroll
+          for (int rr = 0; rr < 8; rr++) *reinterpret_cast<uint4*>(lp + (r0 + rr) * HW * VB) = hq[rr];
+        }
+      }
+    }
+    if (WL) {
+      // thread: (n, granule) = (tid % 16, tid / 16 % 2), taps tid / 32 + 8 u
+      const int nl = tid & 15, gr = (tid >> 4) & 1, t0 = tid >> 5;
+      const int n = nb0 * 16 + nl;
+      uint4 wq[4];
+#pragma unroll
+      for (int u = 0; u < 4; u++) {
+        const int t = t0 + 8 * u;
+        wq[u] = uint4{0u, 0u, 0u, 0u};
+        if (t < 27) {
+          if (MODE == 0) {
+            wq[u] = *reinterpret_cast<const uint4*>(w + ((long)t * nc + n) * kc + c0 + 8 * gr);
+          } else {
+            const uint16_t* p = w + ((long)(26 - t) * kc + c0 + 8 * gr) * nc + n;
+            unsigned e[8];
+#pragma unroll
+            for (int jj = 0; jj < 8; jj++) e[jj] = p[(long)jj * nc];
+            wq[u] = uint4{e[0] | e[1] << 16, e[2] | e[3] << 16, e[4] | e[5] << 16, e[6] | e[7] << 16};
+          }
+        }
+      }
+      unsigned char* lp = wl + (t0 * NBC + nl) * VB + 16 * (gr ^ granule_swizzle<CK>(nl));
+#pragma unroll
+      for (int u = 0; u < 4; u++)
+        if (t0 + 8 * u < 27) *reinterpret_cast<uint4*>(lp + 8 * u * NBC * VB) = wq[u];
     }
     __syncthreads();
-    // not unrolled: the compiler would hoist every tap's weight loads
-#pragma unroll 1
+    // weights in LDS: unrolled (the lane halves' taps differ; their offsets
+    // fold to constants); weights in global memory: not unrolled (the
+    // compiler would hoist every step's weight loads)
+#pragma unroll(WL ? NQ : 1)
     for (int q = 0; q < NQ; q++) {
-      const int k = 32 * q + 8 * g;
-      const int t = k / CK, gr = (k % CK) >> 3;
-      const bool tv = t < 27;
-      const int tt = tv ? t : 26;
-      const int dz = tt / 9 - 1, dx = (tt / 3) % 3 - 1, dy = tt % 3 - 1;
-      const int row = (wz + dz + 1) * 4 + (wx + dx + 1);
+      // taps of the lower (tl) and, with CK = 16, upper (th) lane halves
+      const int tl = CK == 16 ? 2 * q : q, th = CK == 16 ? 2 * q + 1 : q;
+      const int thc = th < 27 ? th : 26;
+      auto aoff = [&](int t) {
+        const int dz = t / 9, dx = (t / 3) % 3, dy = t % 3;
+        return (dy == 0 ? abase[0] : dy == 1 ? abase[1] : abase[2]) + (dz * 4 + dx) * HW * VB;
+      };
+      const int ao = (CK == 16 && hi) ? aoff(thc) : aoff(tl);
+      const bool zero = CK == 16 && hi && th >= 27;
       uint4 a[MB], bw[NB];
 #pragma unroll
       for (int m = 0; m < MB; m++) {
-        const int hv = 16 * m + i + dy + 1;
-        a[m] = *reinterpret_cast<const uint4*>(&halo[(row * HW + hv) * CK + 8 * (gr ^ granule_swizzle<CK>(hv))]);
-        if (!tv) a[m] = uint4{0u, 0u, 0u, 0u};
+        a[m] = lds_read16(halo, ao + 16 * m * VB);
+        if (zero) a[m] = uint4{0u, 0u, 0u, 0u};
       }
 #pragma unroll
       for (int c = 0; c < NB; c++) {
         const int n = (nb0 + c) * 16 + i;
-        if (MODE == 0) {
-          bw[c] = *reinterpret_cast<const uint4*>(w + ((long)tt * nc + n) * kc + c0 + 8 * gr);
+        if (WL) {
+          bw[c] = lds_read16(wl, wbase + tl * NBC * VB);
         } else {
-          const uint16_t* p = w + ((long)(26 - tt) * kc + c0 + 8 * gr) * nc + n;
-          uint16_t e[8];
+          const int t = (CK == 16 && hi) ? thc : tl;
+          const int gr = grl;
+          if (MODE == 0) {
+            bw[c] = *reinterpret_cast<const uint4*>(w + ((long)t * nc + n) * kc + c0 + 8 * gr);
+          } else {
+            const uint16_t* p = w + ((long)(26 - t) * kc + c0 + 8 * gr) * nc + n;
+            unsigned e[8];
 #pragma unroll
-          for (int j = 0; j < 8; j++) e[j] = p[(long)j * nc];
-          bw[c] = uint4{e[0] | (unsigned)e[1] << 16, e[2] | (unsigned)e[3] << 16, e[4] | (unsigned)e[5] << 16,
-                        e[6] | (unsigned)e[7] << 16};
+            for (int jj = 0; jj < 8; jj++) e[jj] = p[(long)jj * nc];
+            bw[c] = uint4{e[0] | e[1] << 16, e[2] | e[3] << 16, e[4] | e[5] << 16, e[6] | e[7] << 16};
+          }
         }
-        if (!tv) bw[c] = uint4{0u, 0u, 0u, 0u};
+        if (zero) bw[c] = uint4{0u, 0u, 0u, 0u};
       }
 #pragma unroll
       for (int c = 0; c < NB; c++)
@@ -151,27 +234,31 @@ __global__ __launch_bounds__(256) void k_conv3(const uint16_t* __restrict__ src,
         for (int m = 0; m < MB; m++)
           acc[m][c] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(__builtin_bit_cast(bf8, a[m]), __builtin_bit_cast(bf8, bw[c]),
                                                               acc[m][c], 0, 0, 0);
+      // keep each step's loads next to its MFMAs (bounded registers)
+      __builtin_amdgcn_sched_barrier(0);
     }
     __syncthreads();
   }
   // D[voxel 4 g + j][channel i] of each (row block, column block)
   const int z = z0 + wz, x = x0 + wx;
   if (z >= V.nz || x >= V.nx) return;
+  const long vrow = V.vox(z, x, y0 + 4 * g);
 #pragma unroll
   for (int c = 0; c < NB; c++) {
     const int n = (nb0 + c) * 16 + i;
     const float bn = bias ? bias[n] : 0.f;
+    uint16_t* out = dst + vrow * ldd + n;
+    const uint16_t* mk = mask ? mask + vrow * ldm + n : nullptr;
 #pragma unroll
     for (int m = 0; m < MB; m++)
 #pragma unroll
-      for (int j = 0; j < 4; j++) {
-        const int y = y0 + 16 * m + 4 * g + j;
-        if (y >= V.ny) continue;
-        const long v = V.vox(z, x, y);
-        float r = acc[m][c][j] + bn;
+      for (int jj = 0; jj < 4; jj++) {
+        const int dyv = 16 * m + jj;
+        if (!interior && y0 + 4 * g + dyv >= V.ny) continue;
+        float r = acc[m][c][jj] + bn;
         if (relu) r = fmaxf(r, 0.f);
-        if (mask && !(bf2f(mask[v * ldm + n]) > 0.f)) r = 0.f;
-        dst[v * ldd + n] = f2bf(r);
+        if (mk && !(bf2f(mk[(long)dyv * ldm]) > 0.f)) r = 0.f;
+        out[(long)dyv * ldd] = __builtin_bit_cast(uint16_t, (__bf16)r);
       }
   }
 }
@@ -192,7 +279,8 @@ __global__ void k_flip_weights(const uint16_t* __restrict__ w, int kc, int nc, u
 // of four wavefronts takes `per` bricks of 2 (z) x 2 (x) rows of TY voxels
 // along y (the convolution's bricks), one 16-channel block of m and one of n.
 // Per brick it copies into LDS, channel-major (y contiguous, rows padded to TY
-// + 8 so that the 16 lanes of a fragment row hit distinct banks): dpre of the
+// + 8 so that each 8-lane phase of a fragment read hits distinct banks; 8-byte
+// stores of 4 voxels per channel): dpre of the
 // brick (zero outside the volume) and the brick's src halo (4 x 4 rows of TY
 // + 2 voxels). Wavefront w takes output row w and every tap: per 32 voxels it
 // reads one A fragment (A[m][k] = dpre[voxel k][m]) and, per (dz, dx), two
@@ -200,6 +288,19 @@ __global__ void k_flip_weights(const uint16_t* __restrict__ w, int kc, int nc, u
 // fragment (B[k][n] = src[voxel k + d_t][n]) follow by dword selection and
 // byte alignment. The four wavefronts' sums meet in LDS, and the block writes
 // part[chunk][t][m][n] for its m and n blocks.
+// Transposed store of 4 voxels x 8 channels (q[k] = channels of voxel k):
+// channel j's four values at dst[j * pitch .. + 3], one 8-byte store each.
+__device__ __forceinline__ void store_quads(uint16_t* dst, int pitch, const uint4 (&q)[4]) {
+  const unsigned* d[4] = {&q[0].x, &q[1].x, &q[2].x, &q[3].x};
+#pragma unroll
+  for (int j = 0; j < 8; j++) {
+    const int h = j >> 1, sh = (j & 1) * 16;
+    const unsigned lo = ((d[0][h] >> sh) & 0xFFFFu) | (((d[1][h] >> sh) & 0xFFFFu) << 16);
+    const unsigned hi = ((d[2][h] >> sh) & 0xFFFFu) | (((d[3][h] >> sh) & 0xFFFFu) << 16);
+    *reinterpret_cast<uint2*>(dst + j * pitch) = uint2{lo, hi};
+  }
+}
+
 template <int TY>
 __global__ __launch_bounds__(256) void k_wgrad(const uint16_t* __restrict__ dpre, int ldp, int mc,
                                                const uint16_t* __restrict__ src, int lds, int kc,
@@ -225,25 +326,35 @@ __global__ __launch_bounds__(256) void k_wgrad(const uint16_t* __restrict__ dpre
     const long r2 = brick / nty;
     const int xt = (int)(r2 % ntx), zt = (int)(r2 / ntx);
     const int z0 = 2 * zt, x0 = 2 * xt, y0 = yt * TY;
-    for (int it = tid; it < 4 * TY * 2; it += 256) {
-      const int gr = it & 1, rem = it >> 1;
-      const int y = rem % TY, row = rem / TY;
-      const int z = z0 + (row >> 1), x = x0 + (row & 1), yy = y0 + y;
-      uint4 q = {0u, 0u, 0u, 0u};
-      if (V.in(z, x, yy)) q = *reinterpret_cast<const uint4*>(dpre + V.vox(z, x, yy) * ldp + mb * 16 + 8 * gr);
-      const uint16_t* e = reinterpret_cast<const uint16_t*>(&q);
+    // channel-major copies, 4 voxels along y per item: each channel's 4 values
+    // go out as one 8-byte LDS store (consecutive lanes take consecutive
+    // voxel quads of one channel row)
+    for (int it = tid; it < 4 * (TY / 4) * 2; it += 256) {
+      const int y4 = it % (TY / 4), rem = it / (TY / 4);
+      const int gr = rem & 1, row = rem >> 1;
+      const int z = z0 + (row >> 1), x = x0 + (row & 1), y = 4 * y4;
+      uint4 q[4];
 #pragma unroll
-      for (int j = 0; j < 8; j++) Ah[(row * 16 + 8 * gr + j) * P + y] = e[j];
+      for (int k = 0; k < 4; k++) {
+        q[k] = uint4{0u, 0u, 0u, 0u};
+        if (V.in(z, x, y0 + y + k))
+          q[k] = *reinterpret_cast<const uint4*>(dpre + V.vox(z, x, y0 + y + k) * ldp + mb * 16 + 8 * gr);
+      }
+      store_quads(Ah + (row * 16 + 8 * gr) * P + y, P, q);
     }
-    for (int it = tid; it < 16 * HW * 2; it += 256) {
-      const int gr = it & 1, rem = it >> 1;
-      const int hv = rem % HW, r = rem / HW;
-      const int zs = z0 - 1 + (r >> 2), xs = x0 - 1 + (r & 3), ys = y0 - 1 + hv;
-      uint4 q = {0u, 0u, 0u, 0u};
-      if (V.in(zs, xs, ys)) q = *reinterpret_cast<const uint4*>(src + V.vox(zs, xs, ys) * lds + nb * 16 + 8 * gr);
-      const uint16_t* e = reinterpret_cast<const uint16_t*>(&q);
+    constexpr int HW4 = (HW + 3) / 4;
+    for (int it = tid; it < 16 * HW4 * 2; it += 256) {
+      const int y4 = it % HW4, rem = it / HW4;
+      const int gr = rem & 1, r = rem >> 1;
+      const int zs = z0 - 1 + (r >> 2), xs = x0 - 1 + (r & 3), hv = 4 * y4;
+      uint4 q[4];
 #pragma unroll
-      for (int j = 0; j < 8; j++) Bh[(r * 16 + 8 * gr + j) * P + hv] = e[j];
+      for (int k = 0; k < 4; k++) {
+        q[k] = uint4{0u, 0u, 0u, 0u};
+        if (V.in(zs, xs, y0 - 1 + hv + k))
+          q[k] = *reinterpret_cast<const uint4*>(src + V.vox(zs, xs, y0 - 1 + hv + k) * lds + nb * 16 + 8 * gr);
+      }
+      store_quads(Bh + (r * 16 + 8 * gr) * P + hv, P, q);
     }
     __syncthreads();
 #pragma unroll 1
@@ -548,6 +659,14 @@ __global__ void k_pack(const float* __restrict__ g, const float* __restrict__ il
 __global__ void k_zero(unsigned* __restrict__ p, int n) {
   const int k = blockIdx.x * blockDim.x + threadIdx.x;
   if (k < n) p[k] = 0u;
+}
+
+// resident 256-thread workgroups of a kernel per CU (1 if the query fails)
+template <typename K>
+int blocks_per_cu(K kernel) {
+  int n = 0;
+  if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&n, kernel, 256, 0) != hipSuccess || n < 1) n = 1;
+  return n;
 }
 
 inline unsigned grid_for(long n, int block, long cap = 16384) {
