@@ -20,6 +20,7 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <climits>
 #include <cstdint>
 #include <mutex>
 
@@ -432,32 +433,46 @@ __global__ void k_chan_sum(const uint16_t* __restrict__ x, int ld, int c, long n
   }
 }
 // The same with one 16-byte load (8 channels) per thread and voxel: blockDim
-// 256 = (256 / (c / 8)) voxel lanes x (c / 8) granules (c and ld multiples of
-// 8, x 16-byte aligned).
-__global__ void k_chan_sum8(const uint16_t* __restrict__ x, int ld, int c, long nvox, float* __restrict__ out) {
-  __shared__ float red[256 * 8];
-  const int G = c / 8, lanes = 256 / G;
+// 1024 = (1024 / (c / 8)) voxel lanes x (c / 8) granules (c and ld multiples
+// of 8, x 16-byte aligned). Few large blocks: every block adds its c sums
+// with atomics to the same c addresses, which serialise.
+__global__ __launch_bounds__(1024) void k_chan_sum8(const uint16_t* __restrict__ x, int ld, int c, long nvox,
+                                                    float* __restrict__ out) {
+  __shared__ float red[1024 * 8];
+  const int G = c / 8, lanes = 1024 / G;
   const int gr = threadIdx.x % G, vl = threadIdx.x / G;
   float s[8] = {0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f};
-  if (vl < lanes)
-    for (long v = (long)blockIdx.x * lanes + vl; v < nvox; v += (long)gridDim.x * lanes) {
-      const uint4 q = *reinterpret_cast<const uint4*>(x + v * ld + 8 * gr);
-      const unsigned u[4] = {q.x, q.y, q.z, q.w};
+  const long step = (long)gridDim.x * lanes;
+  // four voxels in flight per thread
+  for (long v = (long)blockIdx.x * lanes + vl; v < nvox; v += 4 * step) {
+    uint4 q[4];
+#pragma unroll
+    for (int k = 0; k < 4; k++) {
+      q[k] = uint4{0u, 0u, 0u, 0u};
+      if (v + k * step < nvox) q[k] = *reinterpret_cast<const uint4*>(x + (v + k * step) * ld + 8 * gr);
+    }
+#pragma unroll
+    for (int k = 0; k < 4; k++) {
+      const unsigned u[4] = {q[k].x, q[k].y, q[k].z, q[k].w};
 #pragma unroll
       for (int j = 0; j < 4; j++) {
         s[2 * j] += __uint_as_float(u[j] << 16);
         s[2 * j + 1] += __uint_as_float(u[j] & 0xFFFF0000u);
       }
     }
+  }
+  // tree over the voxel lanes (lanes is a power of two: G divides 32)
 #pragma unroll
   for (int j = 0; j < 8; j++) red[threadIdx.x * 8 + j] = s[j];
   __syncthreads();
-  if (threadIdx.x < c) {
-    const int g = threadIdx.x / 8, j = threadIdx.x % 8;
-    float t = 0.f;
-    for (int q = 0; q < lanes; q++) t += red[(q * G + g) * 8 + j];
-    atomicAdd(out + threadIdx.x, t);
+  for (int half = lanes / 2; half > 0; half >>= 1) {
+    if (vl < half) {
+#pragma unroll
+      for (int j = 0; j < 8; j++) red[threadIdx.x * 8 + j] += red[(threadIdx.x + half * G) * 8 + j];
+    }
+    __syncthreads();
   }
+  if (threadIdx.x < c) atomicAdd(out + threadIdx.x, red[threadIdx.x]);
 }
 
 // 2x2x2 average pooling; (nz, nx, ny) are the fine dimensions (even)
@@ -536,6 +551,148 @@ __global__ void k_add_masked(uint16_t* __restrict__ a, int lda, const uint16_t* 
   }
 }
 
+// 8-channel (16-byte) granule forms of the pooling, upsampling and masked-add
+// kernels, for channel counts, strides and slices that are multiples of 8
+// channels: thread = (voxel, granule) with the granule fastest (a block of
+// 256 / G voxels x G granules, G = c / 8), so a wavefront reads and writes
+// whole runs of channels. Arithmetic per channel is that of the per-channel
+// kernels (same order, same rounding).
+__device__ __forceinline__ void unpack8(const uint4& q, float (&f)[8]) {
+  const unsigned u[4] = {q.x, q.y, q.z, q.w};
+#pragma unroll
+  for (int j = 0; j < 4; j++) {
+    f[2 * j] = __uint_as_float(u[j] << 16);
+    f[2 * j + 1] = __uint_as_float(u[j] & 0xFFFF0000u);
+  }
+}
+__device__ __forceinline__ uint4 pack8(const float (&f)[8]) {
+  unsigned u[4];
+#pragma unroll
+  for (int j = 0; j < 4; j++) u[j] = (unsigned)f2bf(f[2 * j]) | ((unsigned)f2bf(f[2 * j + 1]) << 16);
+  return uint4{u[0], u[1], u[2], u[3]};
+}
+template <typename T>
+__device__ __forceinline__ uint4 ld16(const T* p) { return *reinterpret_cast<const uint4*>(p); }
+template <typename T>
+__device__ __forceinline__ void st16(T* p, const uint4& q) { *reinterpret_cast<uint4*>(p) = q; }
+
+// (voxel, channel offset) walk of a granule kernel over n voxels
+struct GranuleWalk {
+  long v, step;
+  int g8;
+  __device__ GranuleWalk(int c) {
+    const int G = c / 8, per = blockDim.x / G;
+    g8 = 8 * (int)(threadIdx.x % G);
+    const int vl = (int)(threadIdx.x / G);
+    v = vl < per ? (long)blockIdx.x * per + vl : LONG_MAX;
+    step = (long)gridDim.x * per;
+  }
+};
+
+__global__ void k_pool2_v(const uint16_t* __restrict__ src, int lds, uint16_t* __restrict__ dst, int ldd, int c, Vol F) {
+  const Vol Cv{F.nz / 2, F.nx / 2, F.ny / 2};
+  const long n = (long)Cv.nz * Cv.nx * Cv.ny;
+  GranuleWalk gw(c);
+  for (long v = gw.v; v < n; v += gw.step) {
+    const int y = (int)(v % Cv.ny);
+    const long r = v / Cv.ny;
+    const int x = (int)(r % Cv.nx), z = (int)(r / Cv.nx);
+    float s[8] = {0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f};
+#pragma unroll
+    for (int q = 0; q < 8; q++) {
+      float f[8];
+      unpack8(ld16(src + F.vox(2 * z + (q >> 2), 2 * x + ((q >> 1) & 1), 2 * y + (q & 1)) * lds + gw.g8), f);
+#pragma unroll
+      for (int j = 0; j < 8; j++) s[j] += f[j];
+    }
+#pragma unroll
+    for (int j = 0; j < 8; j++) s[j] *= 0.125f;
+    st16(dst + v * ldd + gw.g8, pack8(s));
+  }
+}
+__global__ void k_pool2_bwd_v(const uint16_t* __restrict__ dc, int ldc, uint16_t* __restrict__ df, int ldf, int c, Vol F,
+                              int accumulate) {
+  const Vol Cv{F.nz / 2, F.nx / 2, F.ny / 2};
+  const long n = (long)F.nz * F.nx * F.ny;
+  GranuleWalk gw(c);
+  for (long v = gw.v; v < n; v += gw.step) {
+    const int y = (int)(v % F.ny);
+    const long r = v / F.ny;
+    const int x = (int)(r % F.nx), z = (int)(r / F.nx);
+    float gv[8];
+    unpack8(ld16(dc + Cv.vox(z / 2, x / 2, y / 2) * ldc + gw.g8), gv);
+#pragma unroll
+    for (int j = 0; j < 8; j++) gv[j] = 0.125f * gv[j];
+    if (accumulate) {
+      float a[8];
+      unpack8(ld16(df + v * ldf + gw.g8), a);
+#pragma unroll
+      for (int j = 0; j < 8; j++) gv[j] += a[j];
+    }
+    st16(df + v * ldf + gw.g8, pack8(gv));
+  }
+}
+__global__ void k_up2_v(const uint16_t* __restrict__ cs, int ldc, uint16_t* __restrict__ fd, int ldf, int c, Vol F) {
+  const Vol Cv{F.nz / 2, F.nx / 2, F.ny / 2};
+  const long n = (long)F.nz * F.nx * F.ny;
+  GranuleWalk gw(c);
+  for (long v = gw.v; v < n; v += gw.step) {
+    const int y = (int)(v % F.ny);
+    const long r = v / F.ny;
+    const int x = (int)(r % F.nx), z = (int)(r / F.nx);
+    st16(fd + v * ldf + gw.g8, ld16(cs + Cv.vox(z / 2, x / 2, y / 2) * ldc + gw.g8));
+  }
+}
+__global__ void k_up2_bwd_v(const uint16_t* __restrict__ df, int ldf, uint16_t* __restrict__ dc, int ldc, int c, Vol F,
+                            const uint16_t* __restrict__ mask, int ldm) {
+  const Vol Cv{F.nz / 2, F.nx / 2, F.ny / 2};
+  const long n = (long)Cv.nz * Cv.nx * Cv.ny;
+  GranuleWalk gw(c);
+  for (long v = gw.v; v < n; v += gw.step) {
+    const int y = (int)(v % Cv.ny);
+    const long r = v / Cv.ny;
+    const int x = (int)(r % Cv.nx), z = (int)(r / Cv.nx);
+    float s[8] = {0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f};
+#pragma unroll
+    for (int q = 0; q < 8; q++) {
+      float f[8];
+      unpack8(ld16(df + F.vox(2 * z + (q >> 2), 2 * x + ((q >> 1) & 1), 2 * y + (q & 1)) * ldf + gw.g8), f);
+#pragma unroll
+      for (int j = 0; j < 8; j++) s[j] += f[j];
+    }
+    if (mask) {
+      float m[8];
+      unpack8(ld16(mask + v * ldm + gw.g8), m);
+#pragma unroll
+      for (int j = 0; j < 8; j++)
+        if (!(m[j] > 0.f)) s[j] = 0.f;
+    }
+    st16(dc + v * ldc + gw.g8, pack8(s));
+  }
+}
+__global__ void k_add_masked_v(uint16_t* __restrict__ a, int lda, const uint16_t* __restrict__ b, int ldb, int c, long nvox,
+                               const uint16_t* __restrict__ mask, int ldm) {
+  GranuleWalk gw(c);
+  for (long v = gw.v; v < nvox; v += gw.step) {
+    float s[8];
+    unpack8(ld16(a + v * lda + gw.g8), s);
+    if (b) {
+      float f[8];
+      unpack8(ld16(b + v * ldb + gw.g8), f);
+#pragma unroll
+      for (int j = 0; j < 8; j++) s[j] += f[j];
+    }
+    if (mask) {
+      float m[8];
+      unpack8(ld16(mask + v * ldm + gw.g8), m);
+#pragma unroll
+      for (int j = 0; j < 8; j++)
+        if (!(m[j] > 0.f)) s[j] = 0.f;
+    }
+    st16(a + v * lda + gw.g8, pack8(s));
+  }
+}
+
 // 1x1x1 output head in FP32: out[v] = sum_c w[c] act[v][c] + b
 __global__ void k_head(const uint16_t* __restrict__ act, int ld, int c, const float* __restrict__ w, const float* __restrict__ b,
                        float* __restrict__ out, long nvox) {
@@ -569,6 +726,62 @@ __global__ void k_head_bwd(const uint16_t* __restrict__ act, int ld, int c, cons
     for (int q = 0; q < 64; q++) s += red[threadIdx.x][q];
     if (threadIdx.x < c) atomicAdd(dw + threadIdx.x, s);
     if (threadIdx.x == 16) atomicAdd(db, s);
+  }
+}
+
+// k_head for 16 channels in two 16-byte granules per voxel (same order of sums)
+__global__ void k_head16(const uint16_t* __restrict__ act, int ld, const float* __restrict__ w, const float* __restrict__ b,
+                         float* __restrict__ out, long nvox) {
+  float wr[16];
+#pragma unroll
+  for (int ch = 0; ch < 16; ch++) wr[ch] = w[ch];
+  const float b0 = b[0];
+  for (long v = (long)blockIdx.x * blockDim.x + threadIdx.x; v < nvox; v += (long)gridDim.x * blockDim.x) {
+    float a[16];
+    unpack8(ld16(act + v * ld), *reinterpret_cast<float(*)[8]>(a));
+    unpack8(ld16(act + v * ld + 8), *reinterpret_cast<float(*)[8]>(a + 8));
+    float s = b0;
+#pragma unroll
+    for (int ch = 0; ch < 16; ch++) s += wr[ch] * a[ch];
+    out[v] = s;
+  }
+}
+
+// k_head_bwd for 16 channels in two 16-byte granules per voxel (ld a multiple
+// of 8, act and dact 16-byte aligned); the same thread-to-voxel walk and
+// arithmetic order
+__global__ void k_head_bwd16(const uint16_t* __restrict__ act, int ld, const float* __restrict__ w,
+                             const float* __restrict__ dout, uint16_t* __restrict__ dact, int ldd, float* __restrict__ dw,
+                             float* __restrict__ db, long nvox) {
+  __shared__ float red[17][64];
+  float acc[17];
+  for (int k = 0; k < 17; k++) acc[k] = 0.f;
+  float wr[16];
+#pragma unroll
+  for (int ch = 0; ch < 16; ch++) wr[ch] = w[ch];
+  for (long v = (long)blockIdx.x * blockDim.x + threadIdx.x; v < nvox; v += (long)gridDim.x * blockDim.x) {
+    const float g = dout[v];
+#pragma unroll
+    for (int h = 0; h < 2; h++) {
+      float a[8], d[8];
+      unpack8(ld16(act + v * ld + 8 * h), a);
+#pragma unroll
+      for (int j = 0; j < 8; j++) {
+        d[j] = a[j] > 0.f ? wr[8 * h + j] * g : 0.f;
+        acc[8 * h + j] += a[j] * g;
+      }
+      st16(dact + v * ldd + 8 * h, pack8(d));
+    }
+    acc[16] += g;
+  }
+  // blockDim.x == 64
+  for (int k = 0; k < 17; k++) red[k][threadIdx.x] = acc[k];
+  __syncthreads();
+  if (threadIdx.x < 17) {
+    float t = 0.f;
+    for (int q = 0; q < 64; q++) t += red[threadIdx.x][q];
+    if (threadIdx.x < 16) atomicAdd(dw + threadIdx.x, t);
+    if (threadIdx.x == 16) atomicAdd(db, t);
   }
 }
 
@@ -667,6 +880,11 @@ int blocks_per_cu(K kernel) {
   int n = 0;
   if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&n, kernel, 256, 0) != hipSuccess || n < 1) n = 1;
   return n;
+}
+
+// a channel slice that the 16-byte granule kernels can take
+inline bool granules_ok(int c, const void* p, int ld) {
+  return c % 8 == 0 && c <= 2048 && ld % 8 == 0 && (uintptr_t)p % 16 == 0;
 }
 
 inline unsigned grid_for(long n, int block, long cap = 16384) {
@@ -822,7 +1040,7 @@ int efwi_nn_chan_sum(const uint16_t* x, int ld, int c, long nvox, float* out, vo
   hipStream_t s = (hipStream_t)stream;
   hipLaunchKernelGGL(k_zero, dim3((c + 255) / 256), dim3(256), 0, s, reinterpret_cast<unsigned*>(out), c);
   if (c % 8 == 0 && ld % 8 == 0 && (uintptr_t)x % 16 == 0)
-    hipLaunchKernelGGL(k_chan_sum8, dim3(grid_for(nvox * c / 8, 256, 2048)), dim3(256), 0, s, x, ld, c, nvox, out);
+    hipLaunchKernelGGL(k_chan_sum8, dim3(grid_for(nvox * c / 8, 4096, 256)), dim3(1024), 0, s, x, ld, c, nvox, out);
   else
     hipLaunchKernelGGL(k_chan_sum, dim3(grid_for(nvox * c, 256, 2048)), dim3(256), 0, s, x, ld, c, nvox, out);
   return nn_check(s);
@@ -831,8 +1049,12 @@ int efwi_nn_chan_sum(const uint16_t* x, int ld, int c, long nvox, float* out, vo
 int efwi_nn_pool2(const uint16_t* src, int lds, uint16_t* dst, int ldd, int c, int nz, int nx, int ny, void* stream) {
   if (!src || !dst || nz % 2 || nx % 2 || ny % 2 || c <= 0) return EFWI_NN_ERR_ARG;
   hipStream_t s = (hipStream_t)stream;
-  hipLaunchKernelGGL(k_pool2, dim3(grid_for((long)nz * nx * ny / 8 * c, 256)), dim3(256), 0, s, src, lds, dst, ldd, c,
-                     Vol{nz, nx, ny});
+  if (granules_ok(c, src, lds) && granules_ok(c, dst, ldd))
+    hipLaunchKernelGGL(k_pool2_v, dim3(grid_for((long)nz * nx * ny / 8, 256 / (c / 8))), dim3(256 / (c / 8) * (c / 8)), 0, s, src, lds, dst, ldd, c,
+                       Vol{nz, nx, ny});
+  else
+    hipLaunchKernelGGL(k_pool2, dim3(grid_for((long)nz * nx * ny / 8 * c, 256)), dim3(256), 0, s, src, lds, dst, ldd, c,
+                       Vol{nz, nx, ny});
   return nn_check(s);
 }
 
@@ -840,16 +1062,24 @@ int efwi_nn_pool2_bwd(const uint16_t* dc, int ldc, uint16_t* df, int ldf, int c,
                       void* stream) {
   if (!dc || !df || nz % 2 || nx % 2 || ny % 2 || c <= 0) return EFWI_NN_ERR_ARG;
   hipStream_t s = (hipStream_t)stream;
-  hipLaunchKernelGGL(k_pool2_bwd, dim3(grid_for((long)nz * nx * ny * c, 256)), dim3(256), 0, s, dc, ldc, df, ldf, c,
-                     Vol{nz, nx, ny}, accumulate);
+  if (granules_ok(c, dc, ldc) && granules_ok(c, df, ldf))
+    hipLaunchKernelGGL(k_pool2_bwd_v, dim3(grid_for((long)nz * nx * ny, 256 / (c / 8))), dim3(256 / (c / 8) * (c / 8)), 0, s, dc, ldc, df, ldf, c,
+                       Vol{nz, nx, ny}, accumulate);
+  else
+    hipLaunchKernelGGL(k_pool2_bwd, dim3(grid_for((long)nz * nx * ny * c, 256)), dim3(256), 0, s, dc, ldc, df, ldf, c,
+                       Vol{nz, nx, ny}, accumulate);
   return nn_check(s);
 }
 
 int efwi_nn_up2(const uint16_t* cs, int ldc, uint16_t* fd, int ldf, int c, int nz, int nx, int ny, void* stream) {
   if (!cs || !fd || nz % 2 || nx % 2 || ny % 2 || c <= 0) return EFWI_NN_ERR_ARG;
   hipStream_t s = (hipStream_t)stream;
-  hipLaunchKernelGGL(k_up2, dim3(grid_for((long)nz * nx * ny * c, 256)), dim3(256), 0, s, cs, ldc, fd, ldf, c,
-                     Vol{nz, nx, ny});
+  if (granules_ok(c, cs, ldc) && granules_ok(c, fd, ldf))
+    hipLaunchKernelGGL(k_up2_v, dim3(grid_for((long)nz * nx * ny, 256 / (c / 8))), dim3(256 / (c / 8) * (c / 8)), 0, s, cs, ldc, fd, ldf, c,
+                       Vol{nz, nx, ny});
+  else
+    hipLaunchKernelGGL(k_up2, dim3(grid_for((long)nz * nx * ny * c, 256)), dim3(256), 0, s, cs, ldc, fd, ldf, c,
+                       Vol{nz, nx, ny});
   return nn_check(s);
 }
 
@@ -857,8 +1087,12 @@ int efwi_nn_up2_bwd(const uint16_t* df, int ldf, uint16_t* dc, int ldc, int c, i
                     const uint16_t* mask, int ldm, void* stream) {
   if (!df || !dc || nz % 2 || nx % 2 || ny % 2 || c <= 0) return EFWI_NN_ERR_ARG;
   hipStream_t s = (hipStream_t)stream;
-  hipLaunchKernelGGL(k_up2_bwd, dim3(grid_for((long)nz * nx * ny / 8 * c, 256)), dim3(256), 0, s, df, ldf, dc, ldc, c,
-                     Vol{nz, nx, ny}, mask, ldm);
+  if (granules_ok(c, df, ldf) && granules_ok(c, dc, ldc) && (!mask || granules_ok(c, mask, ldm)))
+    hipLaunchKernelGGL(k_up2_bwd_v, dim3(grid_for((long)nz * nx * ny / 8, 256 / (c / 8))), dim3(256 / (c / 8) * (c / 8)), 0, s, df, ldf, dc, ldc, c,
+                       Vol{nz, nx, ny}, mask, ldm);
+  else
+    hipLaunchKernelGGL(k_up2_bwd, dim3(grid_for((long)nz * nx * ny / 8 * c, 256)), dim3(256), 0, s, df, ldf, dc, ldc, c,
+                       Vol{nz, nx, ny}, mask, ldm);
   return nn_check(s);
 }
 
@@ -866,14 +1100,20 @@ int efwi_nn_add(uint16_t* a, int lda, const uint16_t* b, int ldb, int c, long nv
                 void* stream) {
   if (!a || c <= 0) return EFWI_NN_ERR_ARG;
   hipStream_t s = (hipStream_t)stream;
-  hipLaunchKernelGGL(k_add_masked, dim3(grid_for(nvox * c, 256)), dim3(256), 0, s, a, lda, b, ldb, c, nvox, mask, ldm);
+  if (granules_ok(c, a, lda) && (!b || granules_ok(c, b, ldb)) && (!mask || granules_ok(c, mask, ldm)))
+    hipLaunchKernelGGL(k_add_masked_v, dim3(grid_for(nvox, 256 / (c / 8))), dim3(256 / (c / 8) * (c / 8)), 0, s, a, lda, b, ldb, c, nvox, mask, ldm);
+  else
+    hipLaunchKernelGGL(k_add_masked, dim3(grid_for(nvox * c, 256)), dim3(256), 0, s, a, lda, b, ldb, c, nvox, mask, ldm);
   return nn_check(s);
 }
 
 int efwi_nn_head(const uint16_t* act, int ld, int c, const float* w, const float* b, float* out, long nvox, void* stream) {
   if (!act || !w || !b || !out || c <= 0 || c > 16) return EFWI_NN_ERR_ARG;
   hipStream_t s = (hipStream_t)stream;
-  hipLaunchKernelGGL(k_head, dim3(grid_for(nvox, 256)), dim3(256), 0, s, act, ld, c, w, b, out, nvox);
+  if (c == 16 && granules_ok(16, act, ld))
+    hipLaunchKernelGGL(k_head16, dim3(grid_for(nvox, 256)), dim3(256), 0, s, act, ld, w, b, out, nvox);
+  else
+    hipLaunchKernelGGL(k_head, dim3(grid_for(nvox, 256)), dim3(256), 0, s, act, ld, c, w, b, out, nvox);
   return nn_check(s);
 }
 
@@ -883,7 +1123,12 @@ int efwi_nn_head_bwd(const uint16_t* act, int ld, int c, const float* w, const f
   hipStream_t s = (hipStream_t)stream;
   hipLaunchKernelGGL(k_zero, dim3(1), dim3(64), 0, s, reinterpret_cast<unsigned*>(dw), c);
   hipLaunchKernelGGL(k_zero, dim3(1), dim3(64), 0, s, reinterpret_cast<unsigned*>(db), 1);
-  hipLaunchKernelGGL(k_head_bwd, dim3(grid_for(nvox, 64, 2048)), dim3(64), 0, s, act, ld, c, w, dout, dact, ldd, dw, db, nvox);
+  if (c == 16 && granules_ok(16, act, ld) && granules_ok(16, dact, ldd))
+    hipLaunchKernelGGL(k_head_bwd16, dim3(grid_for(nvox, 64, 2048)), dim3(64), 0, s, act, ld, w, dout, dact, ldd, dw, db,
+                       nvox);
+  else
+    hipLaunchKernelGGL(k_head_bwd, dim3(grid_for(nvox, 64, 2048)), dim3(64), 0, s, act, ld, c, w, dout, dact, ldd, dw, db,
+                       nvox);
   return nn_check(s);
 }
 

@@ -142,6 +142,45 @@ def test_chan_sum(c, ld, off, nvox):
     assert np.allclose(got, ref, rtol=1e-5, atol=1e-4 * np.abs(x).sum(0).max() / nvox * np.sqrt(nvox)), (got, ref)
 
 
+@pytest.mark.parametrize("c,off", [(4, 2), (24, 8), (8, 0)])
+def test_pool_up_add_granule_and_scalar_paths(c, off):
+    """Pooling, upsampling, their gradients and the masked add on channel
+    slices at an offset `off` of a wider tensor: 16-byte granules where c and
+    the offset are multiples of 8 channels, the per-channel kernels otherwise;
+    bit-identical to each other's arithmetic (checked against the oracle)."""
+    from efwi import _native as N
+    from efwi.unet import nn_check
+    lib = N.lib()
+    rng = np.random.default_rng(c + off)
+    nz, nx, ny = 4, 6, 12
+    V, ld = nz * nx * ny, c + off + 8
+    x = _bf(rng, nz, nx, ny, c)
+    wide = np.zeros((V, ld), np.float32)
+    wide[:, off:off + c] = x.reshape(V, c)
+    xd = _dev(wide)
+    print(f"[gpu] granule paths c={c} off={off}", flush=True)
+    p = torch.zeros(V // 8, ld, dtype=torch.bfloat16, device=DEV)
+    nn_check(lib.efwi_nn_pool2(_p(xd, off), ld, _p(p, off), ld, c, nz, nx, ny, _s()), "pool")
+    assert np.array_equal(_host(p)[:, off:off + c], O.bf16_round(O.pool2(x).reshape(-1, c).astype(np.float32)))
+    assert not _host(p)[:, :off].any() and not _host(p)[:, off + c:].any()
+    gc = _bf(rng, nz // 2, nx // 2, ny // 2, c)
+    gwide = np.zeros((V // 8, ld), np.float32)
+    gwide[:, off:off + c] = gc.reshape(-1, c)
+    gcd = _dev(gwide)
+    acc = _dev(wide)
+    nn_check(lib.efwi_nn_pool2_bwd(_p(gcd, off), ld, _p(acc, off), ld, c, nz, nx, ny, 1, _s()), "pool bwd")
+    assert _ulps_ok(_host(acc)[:, off:off + c], (x + O.pool2_bwd(gc)).reshape(V, c)).all()
+    up = torch.zeros(V, ld, dtype=torch.bfloat16, device=DEV)
+    nn_check(lib.efwi_nn_up2(_p(gcd, off), ld, _p(up, off), ld, c, nz, nx, ny, _s()), "up")
+    assert np.array_equal(_host(up)[:, off:off + c], O.up2(gc).reshape(V, c))
+    dc = torch.zeros(V // 8, ld, dtype=torch.bfloat16, device=DEV)
+    nn_check(lib.efwi_nn_up2_bwd(_p(xd, off), ld, _p(dc, off), ld, c, nz, nx, ny, _p(gcd, off), ld, _s()), "up bwd")
+    assert _ulps_ok(_host(dc)[:, off:off + c], (O.up2_bwd(x) * (gc > 0)).reshape(-1, c)).all()
+    a = _dev(wide)
+    nn_check(lib.efwi_nn_add(_p(a, off), ld, _p(xd, off), ld, c, V, _p(xd, off), ld, _s()), "add")
+    assert _ulps_ok(_host(a)[:, off:off + c], (2 * x * (x > 0)).reshape(V, c)).all()
+
+
 def test_pool_up_add_head_loss_adam_pack():
     from efwi import _native as N
     from efwi.unet import nn_check
