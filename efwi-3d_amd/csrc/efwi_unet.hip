@@ -48,6 +48,26 @@ struct Vol {
   }
 };
 
+// BF16 granules (8 channels, 16 bytes) to and from FP32
+__device__ __forceinline__ void unpack8(const uint4& q, float (&f)[8]) {
+  const unsigned u[4] = {q.x, q.y, q.z, q.w};
+#pragma unroll
+  for (int j = 0; j < 4; j++) {
+    f[2 * j] = __uint_as_float(u[j] << 16);
+    f[2 * j + 1] = __uint_as_float(u[j] & 0xFFFF0000u);
+  }
+}
+__device__ __forceinline__ uint4 pack8(const float (&f)[8]) {
+  unsigned u[4];
+#pragma unroll
+  for (int j = 0; j < 4; j++) u[j] = (unsigned)f2bf(f[2 * j]) | ((unsigned)f2bf(f[2 * j + 1]) << 16);
+  return uint4{u[0], u[1], u[2], u[3]};
+}
+template <typename T>
+__device__ __forceinline__ uint4 ld16(const T* p) { return *reinterpret_cast<const uint4*>(p); }
+template <typename T>
+__device__ __forceinline__ void st16(T* p, const uint4& q) { *reinterpret_cast<uint4*>(p) = q; }
+
 __device__ __forceinline__ void tap(int t, int& dz, int& dx, int& dy) {
   dz = t / 9 - 1;
   dx = (t / 3) % 3 - 1;
@@ -94,7 +114,7 @@ __global__ __launch_bounds__(256) void k_conv3(const uint16_t* __restrict__ src,
                                                const uint16_t* __restrict__ w, const float* __restrict__ bias,
                                                uint16_t* __restrict__ dst, int ldd, int nc,
                                                const uint16_t* __restrict__ mask, int ldm, Vol V, int relu,
-                                               int ntz, int ntx, int nty, int per_xcd) {
+                                               int ntz, int ntx, int nty, int per_xcd, int vec_out) {
   constexpr int TY = 16 * MB, HW = TY + 2, G = CK / 8;
   constexpr int NQ = (27 * CK + 31) / 32;
   constexpr int H_PAIRS = HW * G, H_NPASS = (H_PAIRS + 255) / 256;
@@ -242,14 +262,55 @@ __global__ __launch_bounds__(256) void k_conv3(const uint16_t* __restrict__ src,
   }
   // D[voxel 4 g + j][channel i] of each (row block, column block)
   const int z = z0 + wz, x = x0 + wx;
-  if (z >= V.nz || x >= V.nx) return;
-  const long vrow = V.vox(z, x, y0 + 4 * g);
+  const bool row_ok = z < V.nz && x < V.nx;
+  const long vrow = V.vox(row_ok ? z : 0, row_ok ? x : 0, y0);
+  if (vec_out) {
+    // through LDS (the halo's space): the wavefront's TY x NBC tile as BF16,
+    // then 16-byte granules (8 channels of one voxel) to dst, masked
+    static_assert(4 * TY * NBC * 2 <= 16 * HW * VB, "output tile exceeds the halo space");
+    uint16_t* tile = reinterpret_cast<uint16_t*>(halo) + wv * TY * NBC;
+#pragma unroll
+    for (int c = 0; c < NB; c++) {
+      const float bn = bias ? bias[(nb0 + c) * 16 + i] : 0.f;
+#pragma unroll
+      for (int m = 0; m < MB; m++)
+#pragma unroll
+        for (int jj = 0; jj < 4; jj++) {
+          float r = acc[m][c][jj] + bn;
+          if (relu) r = fmaxf(r, 0.f);
+          tile[(16 * m + 4 * g + jj) * NBC + 16 * c + i] = f2bf(r);
+        }
+    }
+    __syncthreads();
+    if (!row_ok) return;
+    constexpr int GR = NBC / 8;
+#pragma unroll
+    for (int k0 = 0; k0 < TY * GR; k0 += 64) {
+      const int k = k0 + lane;
+      if (TY * GR % 64 != 0 && k >= TY * GR) break;
+      const int vv = k / GR, gg = k % GR;
+      if (!interior && y0 + vv >= V.ny) continue;
+      uint4 q = *reinterpret_cast<const uint4*>(tile + vv * NBC + 8 * gg);
+      if (mask) {
+        float f[8], mk[8];
+        unpack8(q, f);
+        unpack8(ld16(mask + (vrow + vv) * ldm + nb0 * 16 + 8 * gg), mk);
+#pragma unroll
+        for (int j = 0; j < 8; j++)
+          if (!(mk[j] > 0.f)) f[j] = 0.f;
+        q = pack8(f);
+      }
+      st16(dst + (vrow + vv) * ldd + nb0 * 16 + 8 * gg, q);
+    }
+    return;
+  }
+  if (!row_ok) return;
 #pragma unroll
   for (int c = 0; c < NB; c++) {
     const int n = (nb0 + c) * 16 + i;
     const float bn = bias ? bias[n] : 0.f;
-    uint16_t* out = dst + vrow * ldd + n;
-    const uint16_t* mk = mask ? mask + vrow * ldm + n : nullptr;
+    uint16_t* out = dst + (vrow + 4 * g) * ldd + n;
+    const uint16_t* mk = mask ? mask + (vrow + 4 * g) * ldm + n : nullptr;
 #pragma unroll
     for (int m = 0; m < MB; m++)
 #pragma unroll
@@ -259,7 +320,7 @@ __global__ __launch_bounds__(256) void k_conv3(const uint16_t* __restrict__ src,
         float r = acc[m][c][jj] + bn;
         if (relu) r = fmaxf(r, 0.f);
         if (mk && !(bf2f(mk[(long)dyv * ldm]) > 0.f)) r = 0.f;
-        out[(long)dyv * ldd] = __builtin_bit_cast(uint16_t, (__bf16)r);
+        out[(long)dyv * ldd] = f2bf(r);
       }
   }
 }
@@ -557,25 +618,6 @@ __global__ void k_add_masked(uint16_t* __restrict__ a, int lda, const uint16_t* 
 // 256 / G voxels x G granules, G = c / 8), so a wavefront reads and writes
 // whole runs of channels. Arithmetic per channel is that of the per-channel
 // kernels (same order, same rounding).
-__device__ __forceinline__ void unpack8(const uint4& q, float (&f)[8]) {
-  const unsigned u[4] = {q.x, q.y, q.z, q.w};
-#pragma unroll
-  for (int j = 0; j < 4; j++) {
-    f[2 * j] = __uint_as_float(u[j] << 16);
-    f[2 * j + 1] = __uint_as_float(u[j] & 0xFFFF0000u);
-  }
-}
-__device__ __forceinline__ uint4 pack8(const float (&f)[8]) {
-  unsigned u[4];
-#pragma unroll
-  for (int j = 0; j < 4; j++) u[j] = (unsigned)f2bf(f[2 * j]) | ((unsigned)f2bf(f[2 * j + 1]) << 16);
-  return uint4{u[0], u[1], u[2], u[3]};
-}
-template <typename T>
-__device__ __forceinline__ uint4 ld16(const T* p) { return *reinterpret_cast<const uint4*>(p); }
-template <typename T>
-__device__ __forceinline__ void st16(T* p, const uint4& q) { *reinterpret_cast<uint4*>(p) = q; }
-
 // (voxel, channel offset) walk of a granule kernel over n voxels
 struct GranuleWalk {
   long v, step;
@@ -918,9 +960,11 @@ int efwi_nn_conv3(const uint16_t* src, int lds, int kc, const uint16_t* w, const
   const int per_xcd = (int)((bricks + 7) / 8);
   const dim3 grid((unsigned)(8L * per_xcd), (unsigned)(nbt / NB));
   const bool ck32 = kc % 32 == 0;
+  // outputs (and mask) as 16-byte granules where the pointers allow
+  const int vec_out = (uintptr_t)dst % 16 == 0 && (!mask || (uintptr_t)mask % 16 == 0);
 #define EFWI_NN_CONV_LAUNCH(CK, M, B, MODE)                                                                       \
   hipLaunchKernelGGL((k_conv3<CK, M, B, MODE>), grid, dim3(256), 0, s, src, lds, kc, w, bias, dst, ldd, nc, mask, \
-                     ldm, V, relu, ntz, ntx, nty, per_xcd)
+                     ldm, V, relu, ntz, ntx, nty, per_xcd, vec_out)
 #define EFWI_NN_CONV_MODE(CK, M, B)                         \
   if (mode == 0) { EFWI_NN_CONV_LAUNCH(CK, M, B, 0); }    \
   else { EFWI_NN_CONV_LAUNCH(CK, M, B, 1); }

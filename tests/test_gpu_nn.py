@@ -97,6 +97,14 @@ def test_conv3_forward_and_input_gradient(dims, kc, nc, relu):
     got = _host(wide)
     assert not got[:, :16].any()
     assert _ulps_ok(got[:, 16:], ref).all(), np.abs(got[:, 16:] - ref).max()
+    # a slice 4 channels in (8 bytes: outputs written channel by channel, not
+    # as 16-byte granules): the same values
+    wide.zero_()
+    nn_check(N.lib().efwi_nn_conv3(_p(gd), nc, nc, _p(wt), None, _p(wide, 4), kc + 16, kc,
+                                   _p(md), kc, nz, nx, ny, 0, 0, _s()), "dgrad into an 8-byte aligned slice")
+    got2 = _host(wide)
+    assert not got2[:, :4].any() and not got2[:, 4 + kc:].any()
+    assert np.array_equal(got2[:, 4:4 + kc], got[:, 16:])
 
 
 @pytest.mark.parametrize("dims,mc,kc", [((6, 10, 37), 16, 16), ((4, 8, 16), 64, 32), ((4, 4, 20), 32, 96)])
