@@ -225,19 +225,6 @@ def bench_batch(args, w):
     P.close()
 
 
-def unet_flops(dims):
-    """FLOPs of one training step (forward, input and weight gradients of every
-    3x3x3 convolution) of the U-Net on a volume of `dims` voxels."""
-    from efwi.unet import LAYERS
-    v = dims[0] * dims[1] * dims[2]
-    fl = 0
-    for name, cin, cout, lev in LAYERS:
-        n = v // 8 ** (lev - 1)
-        per = 2 * 27 * cin * cout * n
-        fl += per * (2 if name == "enc1a" else 3)  # no input gradient for the first layer
-    return fl
-
-
 def bench_learned(args):
     """Learned preconditioner: training pairs from the plain method on random
     models, training on the library's BF16 MFMA kernels, then inversions of
@@ -265,7 +252,8 @@ def bench_learned(args):
     e1.record()
     torch.cuda.synchronize()
     step_ms = e0.elapsed_time(e1) / nstep
-    tflops = unet_flops(fdims) / (step_ms * 1e-3) / 1e12
+    from efwi.unet import training_flops
+    tflops = training_flops(fdims) / (step_ms * 1e-3) / 1e12
     del data
     torch.cuda.empty_cache()
     # held-out inversions at the production size

@@ -303,6 +303,19 @@ def pack_input(g, illum, npml, device=0):
     return feat, dims
 
 
+def training_flops(dims) -> float:
+    """Matrix-core work of one training step at `dims` voxels: 2 x 27 x cin x
+    cout multiply-adds per voxel for the forward convolution, the weight
+    gradient of every layer and the input gradient of every layer but the
+    first (whose input needs none); channels as computed (the input's 16)."""
+    v1 = dims[0] * dims[1] * dims[2]
+    total = 0.0
+    for name, cin, cout, lev in LAYERS:
+        per = 2.0 * 27 * cin * cout * v1 / 8 ** (lev - 1)
+        total += per * (2 if name == "enc1a" else 3)
+    return total
+
+
 def _time_training(dims, steps, seed=0):
     """Seconds per training step of the U-Net on a random 2-channel volume of
     `dims` voxels (after 3 untimed steps)."""
@@ -331,4 +344,7 @@ if __name__ == "__main__":
     a = [int(x) for x in sys.argv[1:]]
     dims = tuple(a[:3]) if len(a) >= 3 else (96, 128, 128)
     steps = a[3] if len(a) >= 4 else 10
-    print(f"U-Net training step at {dims}: {_time_training(dims, steps) * 1e3:.2f} ms", flush=True)
+    t = _time_training(dims, steps)
+    tf = training_flops(dims) / t / 1e12
+    print(f"U-Net training step at {dims}: {t * 1e3:.2f} ms, {tf:.0f} TFLOP/s "
+          f"({100 * tf / 2500:.1f} % of the 2.5 PFLOP/s dense BF16 peak)", flush=True)
