@@ -390,33 +390,46 @@ __global__ __launch_bounds__(256) void k_wgrad(const uint16_t* __restrict__ dpre
     const int z0 = 2 * zt, x0 = 2 * xt, y0 = yt * TY;
     // channel-major copies, 4 voxels along y per item: each channel's 4 values
     // go out as one 8-byte LDS store (consecutive lanes take consecutive
-    // voxel quads of one channel row)
-    for (int it = tid; it < 4 * (TY / 4) * 2; it += 256) {
-      const int y4 = it % (TY / 4), rem = it / (TY / 4);
+    // voxel quads of one channel row). Threads keep their (granule, quad,
+    // row) for the brick; in a brick clear of the volume faces no voxel needs
+    // a bounds check.
+    const bool interior = z0 >= 1 && z0 + 2 < V.nz && x0 >= 1 && x0 + 2 < V.nx && y0 >= 1 && y0 + TY + 2 < V.ny;
+    if (tid < 4 * (TY / 4) * 2) {
+      const int y4 = tid % (TY / 4), rem = tid / (TY / 4);
       const int gr = rem & 1, row = rem >> 1;
-      const int z = z0 + (row >> 1), x = x0 + (row & 1), y = 4 * y4;
+      const int z = z0 + (row >> 1), x = x0 + (row & 1), y = y0 + 4 * y4;
+      const bool zx_ok = z < V.nz && x < V.nx;
+      const uint16_t* p = dpre + V.vox(zx_ok ? z : 0, zx_ok ? x : 0, y) * ldp + mb * 16 + 8 * gr;
       uint4 q[4];
 #pragma unroll
       for (int k = 0; k < 4; k++) {
         q[k] = uint4{0u, 0u, 0u, 0u};
-        if (V.in(z, x, y0 + y + k))
-          q[k] = *reinterpret_cast<const uint4*>(dpre + V.vox(z, x, y0 + y + k) * ldp + mb * 16 + 8 * gr);
+        if (interior || (zx_ok && y + k < V.ny)) q[k] = ld16(p + (long)k * ldp);
       }
-      store_quads(Ah + (row * 16 + 8 * gr) * P + y, P, q);
+      store_quads(Ah + (row * 16 + 8 * gr) * P + 4 * y4, P, q);
     }
     constexpr int HW4 = (HW + 3) / 4;
-    for (int it = tid; it < 16 * HW4 * 2; it += 256) {
-      const int y4 = it % HW4, rem = it / HW4;
-      const int gr = rem & 1, r = rem >> 1;
-      const int zs = z0 - 1 + (r >> 2), xs = x0 - 1 + (r & 3), hv = 4 * y4;
-      uint4 q[4];
+    // halo: thread = (granule, quad, hx) walks hz = 0..3 (uniform plane offsets)
+    if (tid < 2 * HW4 * 4) {
+      const int y4 = tid % HW4, rem = tid / HW4;
+      const int gr = rem & 1, hx = rem >> 1;
+      const int xs = x0 - 1 + hx, ys = y0 - 1 + 4 * y4;
+      const bool x_ok = xs >= 0 && xs < V.nx;
+      const long v0 = V.vox(z0 - 1, x_ok ? xs : 0, ys);
+      const long plane = (long)V.nx * V.ny;
 #pragma unroll
-      for (int k = 0; k < 4; k++) {
-        q[k] = uint4{0u, 0u, 0u, 0u};
-        if (V.in(zs, xs, y0 - 1 + hv + k))
-          q[k] = *reinterpret_cast<const uint4*>(src + V.vox(zs, xs, y0 - 1 + hv + k) * lds + nb * 16 + 8 * gr);
+      for (int hz = 0; hz < 4; hz++) {
+        const int zs = z0 - 1 + hz;
+        const bool zx_ok = x_ok && zs >= 0 && zs < V.nz;
+        const uint16_t* p = src + (v0 + hz * plane) * lds + nb * 16 + 8 * gr;
+        uint4 q[4];
+#pragma unroll
+        for (int k = 0; k < 4; k++) {
+          q[k] = uint4{0u, 0u, 0u, 0u};
+          if (interior || (zx_ok && ys + k >= 0 && ys + k < V.ny)) q[k] = ld16(p + (long)k * lds);
+        }
+        store_quads(Bh + ((hz * 4 + hx) * 16 + 8 * gr) * P + 4 * y4, P, q);
       }
-      store_quads(Bh + (r * 16 + 8 * gr) * P + hv, P, q);
     }
     __syncthreads();
 #pragma unroll 1
