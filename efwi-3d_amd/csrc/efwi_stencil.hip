@@ -2,10 +2,16 @@
 // reconstruction with imaging condition, and adjoint.
 //
 // Arithmetic: every cell evaluates the same expression tree as the reference
-// kernel it replaces, with IEEE division and no fused multiply-add (this file
-// is compiled with -ffp-contract=off), so every float result is the
-// correctly rounded result of the reference's float expression. The only
-// liberties taken: a C-PML term a * d(psi) is skipped where the profile
+// kernel it replaces, in FP32 with IEEE division and no fused multiply-add
+// (this file is compiled with -ffp-contract=off). Where the reference writes
+// a double literal (2.0 * mu in the normal-stress updates and several adjoint
+// terms: Src/stress.cu:57-59,112-114,132, Src/stress_adj.cu:131-140,
+// Src/velocity_adj.cu:72,100,129), C++ promotes that sub-expression to FP64
+// and rounds once at the store; here it stays FP32 (the stencil is FP32
+// throughout), so those results can differ from the reference's by the
+// rounding of the intermediate terms. Every other float result is the
+// correctly rounded result of the reference's float expression. Two more
+// liberties: a C-PML term a * d(psi) is skipped where the profile
 // coefficient a is exactly zero, and a division by K where K is exactly one
 // (the reference multiplies by zero or divides by one there); both can
 // change the sign of an exact zero and nothing else.
