@@ -41,7 +41,7 @@ LAYERS = [
     ("dec1a", 48, 16, 1), ("dec1b", 16, 16, 1),
 ]
 IN_CHANNELS = 2       # raw gradient, log illumination
-WGRAD_PER = 8         # bricks (2 x 2 rows of up to 64 voxels) per weight-gradient partial
+WGRAD_WORKGROUPS = 768   # weight gradient: about three resident workgroups per CU
 
 
 def nn_check(rc: int, what: str):
@@ -146,7 +146,7 @@ class UNet3D:
         need = 0
         for name, cin, cout, lev in LAYERS:
             d = tuple(s >> (lev - 1) for s in dims)
-            need = max(need, int(self.lib.efwi_nn_wgrad_scratch(cout, cin, *d, WGRAD_PER)))
+            need = max(need, int(self.lib.efwi_nn_wgrad_scratch(cout, cin, *d, self._wgrad_per(cout, cin, d))))
         if self._scratch is None or self._scratch.numel() < need:
             self._scratch = torch.empty(need, device=self.dev)
         self.grads = {k: torch.zeros_like(v) for k, v in self._params().items()}
@@ -167,12 +167,22 @@ class UNet3D:
                                         cin, None if mask is None else _p(mask, moff), ldm, *dims, 0, 0,
                                         self._stream()), f"dgrad {name}")
 
+    def _wgrad_per(self, cout, cin, dims):
+        """Bricks per weight-gradient workgroup: enough workgroups to fill the
+        GPU (about WGRAD_WORKGROUPS over the (m, n) channel blocks), between 2
+        and 8 bricks each (fewer bricks per workgroup multiply the FP32
+        partials that are written and summed; measured on MI355X at the
+        U-Net's shapes)."""
+        bricks = int(self.lib.efwi_nn_wgrad_scratch(cout, cin, *dims, 1)) // (27 * cout * cin)
+        blocks = (cout // 16) * (cin // 16)
+        return int(min(8, max(2, bricks * blocks // WGRAD_WORKGROUPS)))
+
     def _wgrad(self, name, gpre, ldg, src, lds, dims, goff=0, soff=0):
         _, cin, cout, _ = next(l for l in LAYERS if l[0] == name)
         s = self._stream()
         nn_check(self.lib.efwi_nn_wgrad(_p(gpre, goff), ldg, cout, _p(src, soff), lds, cin, _p(self._scratch),
-                                        self._scratch.numel(), _p(self.grads[name + ".w"]), *dims, WGRAD_PER, s),
-                 f"wgrad {name}")
+                                        self._scratch.numel(), _p(self.grads[name + ".w"]), *dims,
+                                        self._wgrad_per(cout, cin, dims), s), f"wgrad {name}")
         nn_check(self.lib.efwi_nn_chan_sum(_p(gpre, goff), ldg, cout, dims[0] * dims[1] * dims[2],
                                            _p(self.grads[name + ".b"]), s), f"bias grad {name}")
 
