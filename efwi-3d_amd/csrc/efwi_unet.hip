@@ -109,19 +109,24 @@ __device__ __forceinline__ uint4 lds_read16(const unsigned char* base, int byte_
   return *reinterpret_cast<const uint4*>(base + byte_off);
 }
 
+// rows of a convolution brick along z (x: always 2); two wavefronts per z row
+constexpr int conv_tz(int ck, int mb) { return ck == 16 && mb == 4 ? 4 : 2; }
+
 template <int CK, int MB, int NB, int MODE>
-__global__ __launch_bounds__(256) void k_conv3(const uint16_t* __restrict__ src, int lds, int kc,
+__global__ __launch_bounds__(512) void k_conv3(const uint16_t* __restrict__ src, int lds, int kc,
                                                const uint16_t* __restrict__ w, const float* __restrict__ bias,
                                                uint16_t* __restrict__ dst, int ldd, int nc,
                                                const uint16_t* __restrict__ mask, int ldm, Vol V, int relu,
                                                int ntz, int ntx, int nty, int per_xcd, int vec_out) {
   constexpr int TY = 16 * MB, HW = TY + 2, G = CK / 8;
+  constexpr int TZ = conv_tz(CK, MB), NT = 128 * TZ, HR = (TZ + 2) * 4;  // halo rows
   constexpr int NQ = (27 * CK + 31) / 32;
-  constexpr int H_PAIRS = HW * G, H_NPASS = (H_PAIRS + 255) / 256;
+  constexpr int H_PAIRS = HW * G, NRG = HR / 8, H_NPASS = (H_PAIRS * NRG + NT - 1) / NT;
   constexpr bool WL = CK == 16 && NB == 1;
   constexpr int NBC = NB * 16;
   constexpr int VB = CK * 2;  // bytes per voxel in the halo
-  __shared__ __align__(16) unsigned char halo[16 * HW * VB];
+  constexpr bool TILE_OK = 2 * TZ * TY * NBC * 2 <= HR * HW * VB;  // output tile fits the halo space
+  __shared__ __align__(16) unsigned char halo[HR * HW * VB];
   __shared__ __align__(16) unsigned char wl[WL ? 27 * NBC * VB : 16];
   const int tid = threadIdx.x, lane = tid & 63, wv = tid >> 6;
   // blocks b, b + 8, b + 16, ... run on one XCD: give each XCD a contiguous
@@ -132,8 +137,8 @@ __global__ __launch_bounds__(256) void k_conv3(const uint16_t* __restrict__ src,
   const int yt = (int)(brick % nty);
   const long r2 = brick / nty;
   const int xt = (int)(r2 % ntx), zt = (int)(r2 / ntx);
-  const int z0 = 2 * zt, x0 = 2 * xt, y0 = yt * TY;
-  const bool interior = z0 >= 1 && z0 + 2 < V.nz && x0 >= 1 && x0 + 2 < V.nx && y0 >= 1 && y0 + TY < V.ny;
+  const int z0 = TZ * zt, x0 = 2 * xt, y0 = yt * TY;
+  const bool interior = z0 >= 1 && z0 + TZ < V.nz && x0 >= 1 && x0 + 2 < V.nx && y0 >= 1 && y0 + TY < V.ny;
   const int wz = wv >> 1, wx = wv & 1;
   const int i = lane & 15, g = lane >> 4;
   const int hi = CK == 16 ? g >> 1 : 0;    // CK = 16: upper lane half takes the odd tap
@@ -154,42 +159,41 @@ __global__ __launch_bounds__(256) void k_conv3(const uint16_t* __restrict__ src,
     for (int c = 0; c < NB; c++) acc[m][c] = f4{0.f, 0.f, 0.f, 0.f};
   for (int c0 = 0; c0 < kc; c0 += CK) {
     // halo rows r = 4 hz + hx at (z0 - 1 + hz, x0 - 1 + hx), voxels hv at
-    // y0 - 1 + hv. A thread keeps one (hv, granule) pair and walks the 16
-    // rows (row offsets are uniform), eight loads in flight before the
+    // y0 - 1 + hv. An item is one (hv, granule) pair over 8 rows (two z
+    // planes; row offsets uniform), its eight loads in flight before the
     // stores.
 #pragma unroll
     for (int v = 0; v < H_NPASS; v++) {
-      const int pidx = tid + 256 * v;
-      if (H_PAIRS % 256 == 0 || pidx < H_PAIRS) {
+      const int item = tid + NT * v;
+      if (item < H_PAIRS * NRG) {
+        const int pidx = item % H_PAIRS, rg = item / H_PAIRS;
         const int gr = pidx % G, hv = pidx / G;
         const int ys = y0 - 1 + hv;
         const bool y_ok = interior || (ys >= 0 && ys < V.ny);
-        const uint16_t* tp = src + (corner + hv) * lds + c0 + 8 * gr;
-        unsigned char* lp = halo + hv * VB + 16 * (gr ^ granule_swizzle<CK>(hv));
+        const uint16_t* tp = src + (corner + (long)(2 * rg) * V.nx * V.ny + hv) * lds + c0 + 8 * gr;
+        unsigned char* lp = halo + (8 * rg * HW + hv) * VB + 16 * (gr ^ granule_swizzle<CK>(hv));
+        uint4 hq[8];
 #pragma unroll
-        for (int r0 = 0; r0 < 16; r0 += 8) {
-          uint4 hq[8];
-#pragma unroll
-          for (int rr = 0; rr < 8; rr++) {
-            const int hz = (r0 + rr) >> 2, hx = (r0 + rr) & 3;
-            const bool ok = interior || (y_ok && (unsigned)(z0 - 1 + hz) < (unsigned)V.nz &&
-                                         (unsigned)(x0 - 1 + hx) < (unsigned)V.nx);
-            hq[rr] = uint4{0u, 0u, 0u, 0u};
-            if (ok) hq[rr] = *reinterpret_cast<const uint4*>(tp + ((long)hz * V.nx + hx) * V.ny * lds);
-          }
-#pragma unroll
-          for (int rr = 0; rr < 8; rr++) *reinterpret_cast<uint4*>(lp + (r0 + rr) * HW * VB) = hq[rr];
+        for (int rr = 0; rr < 8; rr++) {
+          const int hz = 2 * rg + (rr >> 2), hx = rr & 3;
+          const bool ok = interior || (y_ok && (unsigned)(z0 - 1 + hz) < (unsigned)V.nz &&
+                                       (unsigned)(x0 - 1 + hx) < (unsigned)V.nx);
+          hq[rr] = uint4{0u, 0u, 0u, 0u};
+          if (ok) hq[rr] = *reinterpret_cast<const uint4*>(tp + ((long)(rr >> 2) * V.nx + hx) * V.ny * lds);
         }
+#pragma unroll
+        for (int rr = 0; rr < 8; rr++) *reinterpret_cast<uint4*>(lp + rr * HW * VB) = hq[rr];
       }
     }
     if (WL) {
-      // thread: (n, granule) = (tid % 16, tid / 16 % 2), taps tid / 32 + 8 u
+      // thread: (n, granule) = (tid % 16, tid / 16 % 2), taps tid / 32 + TS u
+      constexpr int TS = NT / 32, WU = (27 + TS - 1) / TS;
       const int nl = tid & 15, gr = (tid >> 4) & 1, t0 = tid >> 5;
       const int n = nb0 * 16 + nl;
-      uint4 wq[4];
+      uint4 wq[WU];
 #pragma unroll
-      for (int u = 0; u < 4; u++) {
-        const int t = t0 + 8 * u;
+      for (int u = 0; u < WU; u++) {
+        const int t = t0 + TS * u;
         wq[u] = uint4{0u, 0u, 0u, 0u};
         if (t < 27) {
           if (MODE == 0) {
@@ -205,8 +209,8 @@ __global__ __launch_bounds__(256) void k_conv3(const uint16_t* __restrict__ src,
       }
       unsigned char* lp = wl + (t0 * NBC + nl) * VB + 16 * (gr ^ granule_swizzle<CK>(nl));
 #pragma unroll
-      for (int u = 0; u < 4; u++)
-        if (t0 + 8 * u < 27) *reinterpret_cast<uint4*>(lp + 8 * u * NBC * VB) = wq[u];
+      for (int u = 0; u < WU; u++)
+        if (t0 + TS * u < 27) *reinterpret_cast<uint4*>(lp + TS * u * NBC * VB) = wq[u];
     }
     __syncthreads();
     // weights in LDS: unrolled (the lane halves' taps differ; their offsets
@@ -264,10 +268,9 @@ __global__ __launch_bounds__(256) void k_conv3(const uint16_t* __restrict__ src,
   const int z = z0 + wz, x = x0 + wx;
   const bool row_ok = z < V.nz && x < V.nx;
   const long vrow = V.vox(row_ok ? z : 0, row_ok ? x : 0, y0);
-  if (vec_out) {
+  if (TILE_OK && vec_out) {
     // through LDS (the halo's space): the wavefront's TY x NBC tile as BF16,
     // then 16-byte granules (8 channels of one voxel) to dst, masked
-    static_assert(4 * TY * NBC * 2 <= 16 * HW * VB, "output tile exceeds the halo space");
     uint16_t* tile = reinterpret_cast<uint16_t*>(halo) + wv * TY * NBC;
 #pragma unroll
     for (int c = 0; c < NB; c++) {
@@ -968,16 +971,17 @@ int efwi_nn_conv3(const uint16_t* src, int lds, int kc, const uint16_t* w, const
   const int MB = ny >= 64 ? 4 : ny >= 32 ? 2 : 1;
   const int nbt = nc / 16;
   const int NB = nbt % 4 == 0 ? 4 : nbt % 3 == 0 ? 3 : nbt % 2 == 0 ? 2 : 1;
-  const int ntz = (nz + 1) / 2, ntx = (nx + 1) / 2, nty = (ny + 16 * MB - 1) / (16 * MB);
+  const bool ck32 = kc % 32 == 0;
+  const int TZ = conv_tz(ck32 ? 32 : 16, MB);
+  const int ntz = (nz + TZ - 1) / TZ, ntx = (nx + 1) / 2, nty = (ny + 16 * MB - 1) / (16 * MB);
   const long bricks = (long)ntz * ntx * nty;
   const int per_xcd = (int)((bricks + 7) / 8);
   const dim3 grid((unsigned)(8L * per_xcd), (unsigned)(nbt / NB));
-  const bool ck32 = kc % 32 == 0;
   // outputs (and mask) as 16-byte granules where the pointers allow
   const int vec_out = (uintptr_t)dst % 16 == 0 && (!mask || (uintptr_t)mask % 16 == 0);
 #define EFWI_NN_CONV_LAUNCH(CK, M, B, MODE)                                                                       \
-  hipLaunchKernelGGL((k_conv3<CK, M, B, MODE>), grid, dim3(256), 0, s, src, lds, kc, w, bias, dst, ldd, nc, mask, \
-                     ldm, V, relu, ntz, ntx, nty, per_xcd, vec_out)
+  hipLaunchKernelGGL((k_conv3<CK, M, B, MODE>), grid, dim3(128 * conv_tz(CK, M)), 0, s, src, lds, kc, w, bias, dst,  \
+                     ldd, nc, mask, ldm, V, relu, ntz, ntx, nty, per_xcd, vec_out)
 #define EFWI_NN_CONV_MODE(CK, M, B)                         \
   if (mode == 0) { EFWI_NN_CONV_LAUNCH(CK, M, B, 0); }    \
   else { EFWI_NN_CONV_LAUNCH(CK, M, B, 1); }
