@@ -75,14 +75,14 @@ __device__ __forceinline__ void tap(int t, int& dz, int& dx, int& dy) {
 }
 
 // 3x3x3 convolution as implicit GEMM on v_mfma_f32_16x16x32_bf16, the input
-// staged through LDS. A workgroup of four wavefronts owns a brick of 2 (z) x
+// staged through LDS. A workgroup of 2 TZ wavefronts owns a brick of TZ (z) x
 // 2 (x) rows of TY = 16 MB voxels along y and NB blocks of 16 output
 // channels; wavefront w takes row (z0 + w / 2, x0 + w % 2), MB row blocks of
 // the A operand (16 voxels each) against NB column blocks. Per chunk of CK
-// input channels the block copies the brick's halo (4 x 4 rows of TY + 2
-// voxels, zero outside the volume) into LDS with 16-byte loads: a thread
-// keeps one (voxel, granule) pair and walks the 16 rows, whose offsets are
-// uniform. Every tap then reads its A fragments from there with one 16-byte
+// input channels the block copies the brick's halo ((TZ + 2) x 4 rows of TY
+// + 2 voxels, zero outside the volume) into LDS with 16-byte loads: an item
+// is one (voxel, granule) pair over the 8 rows of two z planes, whose offsets
+// are uniform. Every tap then reads its A fragments from there with one 16-byte
 // LDS read per lane (the 8-channel granules of a voxel are XOR-swizzled with
 // its position so that each 8-lane phase of a read hits distinct banks). The
 // K dimension runs over (tap, channel) of the chunk in steps of 32: lane group
@@ -109,8 +109,10 @@ __device__ __forceinline__ uint4 lds_read16(const unsigned char* base, int byte_
   return *reinterpret_cast<const uint4*>(base + byte_off);
 }
 
-// rows of a convolution brick along z (x: always 2); two wavefronts per z row
-constexpr int conv_tz(int ck, int mb) { return ck == 16 && mb == 4 ? 4 : 2; }
+// rows of a convolution brick along z (x: always 2), two wavefronts per z
+// row: 4 for rows of 64 voxels (halo rows per output row 3 instead of 4),
+// 2 for the shorter rows of the small volumes (more bricks to fill the GPU)
+constexpr int conv_tz(int ck, int mb) { return (void)ck, mb == 4 ? 4 : 2; }
 
 template <int CK, int MB, int NB, int MODE>
 __global__ __launch_bounds__(512) void k_conv3(const uint16_t* __restrict__ src, int lds, int kc,
