@@ -112,7 +112,7 @@ __device__ __forceinline__ uint4 lds_read16(const unsigned char* base, int byte_
 // rows of a convolution brick along z (x: always 2), two wavefronts per z
 // row: 4 for rows of 64 voxels (halo rows per output row 3 instead of 4),
 // 2 for the shorter rows of the small volumes (more bricks to fill the GPU)
-constexpr int conv_tz(int ck, int mb) { return (void)ck, mb == 4 ? 4 : 2; }
+constexpr int conv_tz(int mb) { return mb == 4 ? 4 : 2; }
 
 template <int CK, int MB, int NB, int MODE>
 __global__ __launch_bounds__(512) void k_conv3(const uint16_t* __restrict__ src, int lds, int kc,
@@ -121,7 +121,7 @@ __global__ __launch_bounds__(512) void k_conv3(const uint16_t* __restrict__ src,
                                                const uint16_t* __restrict__ mask, int ldm, Vol V, int relu,
                                                int ntz, int ntx, int nty, int per_xcd, int vec_out) {
   constexpr int TY = 16 * MB, HW = TY + 2, G = CK / 8;
-  constexpr int TZ = conv_tz(CK, MB), NT = 128 * TZ, HR = (TZ + 2) * 4;  // halo rows
+  constexpr int TZ = conv_tz(MB), NT = 128 * TZ, HR = (TZ + 2) * 4;  // halo rows
   constexpr int NQ = (27 * CK + 31) / 32;
   constexpr int H_PAIRS = HW * G, NRG = HR / 8, H_NPASS = (H_PAIRS * NRG + NT - 1) / NT;
   constexpr bool WL = CK == 16 && NB == 1;
@@ -974,7 +974,7 @@ int efwi_nn_conv3(const uint16_t* src, int lds, int kc, const uint16_t* w, const
   const int nbt = nc / 16;
   const int NB = nbt % 4 == 0 ? 4 : nbt % 3 == 0 ? 3 : nbt % 2 == 0 ? 2 : 1;
   const bool ck32 = kc % 32 == 0;
-  const int TZ = conv_tz(ck32 ? 32 : 16, MB);
+  const int TZ = conv_tz(MB);
   const int ntz = (nz + TZ - 1) / TZ, ntx = (nx + 1) / 2, nty = (ny + 16 * MB - 1) / (16 * MB);
   const long bricks = (long)ntz * ntx * nty;
   const int per_xcd = (int)((bricks + 7) / 8);
@@ -982,7 +982,7 @@ int efwi_nn_conv3(const uint16_t* src, int lds, int kc, const uint16_t* w, const
   // outputs (and mask) as 16-byte granules where the pointers allow
   const int vec_out = (uintptr_t)dst % 16 == 0 && (!mask || (uintptr_t)mask % 16 == 0);
 #define EFWI_NN_CONV_LAUNCH(CK, M, B, MODE)                                                                       \
-  hipLaunchKernelGGL((k_conv3<CK, M, B, MODE>), grid, dim3(128 * conv_tz(CK, M)), 0, s, src, lds, kc, w, bias, dst,  \
+  hipLaunchKernelGGL((k_conv3<CK, M, B, MODE>), grid, dim3(128 * conv_tz(M)), 0, s, src, lds, kc, w, bias, dst,  \
                      ldd, nc, mask, ldm, V, relu, ntz, ntx, nty, per_xcd, vec_out)
 #define EFWI_NN_CONV_MODE(CK, M, B)                         \
   if (mode == 0) { EFWI_NN_CONV_LAUNCH(CK, M, B, 0); }    \
