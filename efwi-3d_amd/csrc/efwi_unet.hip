@@ -6,12 +6,14 @@
 // ld (the channel stride) a multiple of 16, so a tensor can be a channel slice
 // of a wider one (the U-Net's skip concatenations are written in place).
 //
-// Convolution as implicit GEMM on v_mfma_f32_16x16x16_bf16 (D = A B + C,
-// 16x16x16, one wavefront). Lane l holds A[l % 16][4 (l / 16) + j],
-// B[4 (l / 16) + j][l % 16] and D[4 (l / 16) + j][l % 16], j = 0..3. A
-// wavefront owns 16 consecutive voxels along y: A rows are those voxels, the
-// K dimension runs over the 27 taps x input channels in chunks of 16, and B
-// columns are output channels, 16 per accumulator block.
+// Convolutions and weight gradients as implicit GEMMs on
+// v_mfma_f32_16x16x32_bf16 (D = A B + C, 16 x 16 x 32, one wavefront). Lane
+// l holds 8 K entries of row l % 16 of A and of column l % 16 of B, the K
+// entries of lane group l / 16 being the same for A and B (that is all the
+// contraction needs), and D[4 (l / 16) + j][l % 16], j = 0..3. In the
+// convolution the A rows are 16 consecutive voxels along y, the K dimension
+// runs over (tap, input channel) and the B columns are output channels; in
+// the weight gradient K runs over voxels.
 //   forward:  out[v][n] = sum_t sum_k in[v + d_t][k] w[t][n][k]      (+ bias, ReLU)
 //   dgrad:    din[v][n] = sum_t sum_k dout[v - d_t][k] w[t][k][n]    (x ReLU mask)
 //   wgrad:    dw[t][m][n] = sum_v dpre[v][m] in[v + d_t][n]          (K = voxels)
