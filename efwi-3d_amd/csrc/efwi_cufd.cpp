@@ -40,9 +40,11 @@ struct JVal {
 };
 
 struct JParser {
+  static constexpr int kMaxDepth = 64;  // nesting of the files written by json.dump: 2
   const char* p;
   const char* end;
   std::string err;
+  int depth = 0;
   void ws() {
     while (p < end && std::isspace((unsigned char)*p)) p++;
   }
@@ -50,8 +52,13 @@ struct JParser {
     ws();
     if (p >= end) return fail("unexpected end of input");
     char c = *p;
-    if (c == '{') return obj(v);
-    if (c == '[') return arr(v);
+    if (c == '{' || c == '[') {
+      if (depth >= kMaxDepth) return fail("nesting deeper than " + std::to_string(kMaxDepth));
+      depth++;
+      const bool ok = c == '{' ? obj(v) : arr(v);
+      depth--;
+      return ok;
+    }
     if (c == '"') {
       v.kind = JVal::STR;
       return string(v.str);
@@ -159,11 +166,16 @@ int read_json(const std::string& fname, JVal& out) {
   return EFWI_OK;
 }
 
+// an integer-valued number in int range (no silent truncation or overflow)
+bool as_int(const JVal& x, int* v) {
+  if (x.kind != JVal::NUM || !(x.num >= -2147483648.0 && x.num <= 2147483647.0) || x.num != std::floor(x.num))
+    return false;
+  *v = (int)x.num;
+  return true;
+}
 bool get_int(const JVal& o, const char* k, int* v) {
   const JVal* x = o.get(k);
-  if (!x || x->kind != JVal::NUM) return false;
-  *v = (int)x->num;
-  return true;
+  return x && as_int(*x, v);
 }
 bool get_num(const JVal& o, const char* k, double* v) {
   const JVal* x = o.get(k);
@@ -182,8 +194,9 @@ bool get_ints(const JVal& o, const char* k, std::vector<int>& v) {
   if (!x || x->kind != JVal::ARR) return false;
   v.clear();
   for (auto& e : x->arr) {
-    if (e.kind != JVal::NUM) return false;
-    v.push_back((int)e.num);
+    int i;
+    if (!as_int(e, &i)) return false;
+    v.push_back(i);
   }
   return true;
 }
@@ -269,6 +282,7 @@ int read_shot(const JVal& sv, int id, int npml, bool if_win, ShotDesc& s) {
   if (!get_int(*o, "z_src", &s.sz) || !get_int(*o, "x_src", &s.sx) || !get_int(*o, "y_src", &s.sy) ||
       !get_int(*o, "nrec", &nrec))
     return fail(EFWI_ERR_IO, key + ": missing z_src/x_src/y_src/nrec");
+  if (nrec < 0) return fail(EFWI_ERR_IO, key + ": negative nrec");
   std::vector<int> zr, xr, yr;
   if (!get_ints(*o, "z_rec", zr) || !get_ints(*o, "x_rec", xr) || !get_ints(*o, "y_rec", yr))
     return fail(EFWI_ERR_IO, key + ": missing z_rec/x_rec/y_rec");

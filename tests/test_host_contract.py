@@ -109,6 +109,26 @@ def test_cufd_reports_errors_instead_of_exiting(tmp_path):
     (tmp_path / "bad.json").write_text("{\"nz\": 16, ")
     rc = lib.efwi_cufd(C.byref(m), fp, fp, fp, fp, fp, fp, fp, 1, 0, 1, ip, str(tmp_path / "bad.json").encode())
     assert rc == -3 and b"malformed" in lib.efwi_last_error()
+    # runaway nesting is refused, not recursed into
+    (tmp_path / "deep.json").write_text("{\"a\": " + "[" * 100000 + "]" * 100000 + "}")
+    rc = lib.efwi_cufd(C.byref(m), fp, fp, fp, fp, fp, fp, fp, 1, 0, 1, ip, str(tmp_path / "deep.json").encode())
+    assert rc == -3 and b"nesting" in lib.efwi_last_error()
+    # integer keys must hold integers in int range
+    base = {"nz": n, "nx": n, "ny": n, "dz": 10.0, "dx": 10.0, "dy": 10.0, "nSteps": 10, "dt": 0.001, "f0": 10.0,
+            "nPoints_pml": 4, "survey_fname": str(tmp_path / "survey.json"), "data_dir_name": str(tmp_path)}
+    for bad in ({"nz": 16.5}, {"nx": 1e30}, {"nSteps": -3e12}):
+        para.write_text(json.dumps({**base, **bad}))
+        rc = lib.efwi_cufd(C.byref(m), fp, fp, fp, fp, fp, fp, fp, 1, 0, 1, ip, str(para).encode())
+        assert rc == -3 and b"missing one of" in lib.efwi_last_error(), bad
+    # a survey shot with a negative receiver count or a fractional coordinate
+    para.write_text(json.dumps(base))
+    shot = {"z_src": 2, "x_src": 8, "y_src": 8, "nrec": -1, "z_rec": [1], "x_rec": [1], "y_rec": [1]}
+    (tmp_path / "survey.json").write_text(json.dumps({"shot0": shot}))
+    rc = lib.efwi_cufd(C.byref(m), fp, fp, fp, fp, fp, fp, fp, 1, 0, 1, ip, str(para).encode())
+    assert rc == -3 and b"negative nrec" in lib.efwi_last_error()
+    (tmp_path / "survey.json").write_text(json.dumps({"shot0": {**shot, "nrec": 1, "x_rec": [1.5]}}))
+    rc = lib.efwi_cufd(C.byref(m), fp, fp, fp, fp, fp, fp, fp, 1, 0, 1, ip, str(para).encode())
+    assert rc == -3 and b"z_rec/x_rec/y_rec" in lib.efwi_last_error()
 
 
 def test_create_without_device_fails_cleanly():
