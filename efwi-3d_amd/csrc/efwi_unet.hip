@@ -373,7 +373,8 @@ __device__ __forceinline__ void store_quads(uint16_t* dst, int pitch, const uint
 template <int TY>
 __global__ __launch_bounds__(256) void k_wgrad(const uint16_t* __restrict__ dpre, int ldp, int mc,
                                                const uint16_t* __restrict__ src, int lds, int kc,
-                                               float* __restrict__ part, Vol V, int ntz, int ntx, int nty, int per) {
+                                               float* __restrict__ part, float* __restrict__ dw, Vol V, int ntz,
+                                               int ntx, int nty, int per) {
   constexpr int HW = TY + 2, P = TY + 8, NK = TY / 32;
   constexpr int A_BYTES = 4 * 16 * P * 2, B_BYTES = 16 * 16 * P * 2, R_BYTES = 9 * 4 * 256 * 4;
   constexpr int S_BYTES = A_BYTES + (B_BYTES > R_BYTES ? B_BYTES : R_BYTES);
@@ -478,6 +479,9 @@ __global__ __launch_bounds__(256) void k_wgrad(const uint16_t* __restrict__ dpre
       const float s = red[(tt * 4 + 0) * 256 + tid] + red[(tt * 4 + 1) * 256 + tid] + red[(tt * 4 + 2) * 256 + tid] +
                       red[(tt * 4 + 3) * 256 + tid];
       part[(((long)blockIdx.x * 27 + 9 * p + tt) * mc + m) * kc + n] = s;
+      // dw is summed from the partials by atomics after this kernel: the
+      // first chunk's blocks clear it
+      if (blockIdx.x == 0) dw[((long)(9 * p + tt) * mc + m) * kc + n] = 0.f;
     }
     __syncthreads();
   }
@@ -1089,12 +1093,12 @@ static int wgrad_launch(const uint16_t* dpre, int ldp, int mc, const uint16_t* s
   const int chunks = (int)((wgrad_bricks(nz, nx, ny) + per - 1) / per);
   const dim3 grid((unsigned)chunks, (unsigned)(mc / 16), (unsigned)(kc / 16));
   if (ty == 64)
-    hipLaunchKernelGGL((k_wgrad<64>), grid, dim3(256), 0, s, dpre, ldp, mc, src, lds, kc, scratch, V, ntz, ntx, nty, per);
+    hipLaunchKernelGGL((k_wgrad<64>), grid, dim3(256), 0, s, dpre, ldp, mc, src, lds, kc, scratch, dw, V, ntz, ntx, nty,
+                       per);
   else
-    hipLaunchKernelGGL((k_wgrad<32>), grid, dim3(256), 0, s, dpre, ldp, mc, src, lds, kc, scratch, V, ntz, ntx, nty, per);
+    hipLaunchKernelGGL((k_wgrad<32>), grid, dim3(256), 0, s, dpre, ldp, mc, src, lds, kc, scratch, dw, V, ntz, ntx, nty,
+                       per);
   const long width = 27L * mc * kc;
-  hipLaunchKernelGGL(k_zero, dim3((unsigned)((width + 255) / 256)), dim3(256), 0, s, reinterpret_cast<unsigned*>(dw),
-                     (int)width);
   hipLaunchKernelGGL(k_sum_chunks, dim3(grid_for(width, 256, 1L << 30), (chunks + kChunkGroup - 1) / kChunkGroup),
                      dim3(256), 0, s, scratch, width, chunks, dw);
   return nn_check(s);
