@@ -311,9 +311,38 @@ def bench_learned(args):
         "roofline": {"bound": "mfma", "kernel": "U-Net training step (conv3 forward/input/weight gradients)",
                      "achieved": round(tflops, 2), "peak": 2500.0, "unit": "TFLOP/s",
                      "frac": round(tflops / 2500.0, 5), "traffic": None, "step_ms": round(step_ms, 2)},
-        "cpu_baseline": None,
+        "cpu_baseline": None if args.no_cpu_baseline else unet_cpu_baseline(),
     }
     print(json.dumps(out), flush=True)
+
+
+def unet_cpu_baseline(dims=(96, 128, 128), cin=16, cout=16):
+    """The CPU oracle (oracle/nn_oracle.py, numpy in float64) on a
+    bounded sample of the U-Net's work: the forward, input-gradient and
+    weight-gradient convolutions of one 16 -> 16 layer at the full 96x128x128,
+    reported in the roofline's unit (TFLOP/s of matrix-core-equivalent work)."""
+    import time
+    import numpy as np
+    sys.path.insert(0, os.path.join(ROOT, "oracle"))
+    import nn_oracle as O
+    rng = np.random.default_rng(0)
+    x = rng.standard_normal((*dims, cin))
+    w = rng.standard_normal((27, cout, cin)) * 0.1
+    g = rng.standard_normal((*dims, cout))
+    t0 = time.perf_counter()
+    O.conv3(x, w)
+    O.conv3_dgrad(g, w)
+    O.conv3_wgrad(g, x)
+    sec = time.perf_counter() - t0
+    flops = 3 * 2.0 * 27 * cin * cout * dims[0] * dims[1] * dims[2]
+    try:  # the matrix products run on numpy's BLAS threads
+        from threadpoolctl import threadpool_info
+        cores = max([i.get("num_threads", 1) for i in threadpool_info() if i.get("user_api") == "blas"] or [1])
+    except ImportError:
+        cores = 1
+    return {"value": flops / sec / 1e12, "unit": "TFLOP/s", "cores": cores, "cpu_model": cpu_model(), "kind": "port",
+            "sample": f"nn_oracle forward + input gradient + weight gradient of one {cin}->{cout} 3x3x3 convolution "
+                      f"at {dims[0]}x{dims[1]}x{dims[2]} voxels (float64, numpy matrix products), {sec:.1f} s"}
 
 
 def main():
