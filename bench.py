@@ -466,7 +466,9 @@ def main():
 
     if g.memory_mode == 1:
         seg = g.checkpoint_every or int(np.ceil(np.sqrt(g.nt - 1)))
-        recon = (f"sqrt(nt) checkpoints (segments of {seg} steps, boundary ring rebuilt per segment); "
+        res, nseg = P.resident_segments()
+        recon = (f"sqrt(nt) checkpoints (segments of {seg} steps; the last {res} of {nseg} segments keep the "
+                 f"forward pass's boundary ring, the others rebuild it by replay); "
                  f"the ring of all steps would need {ring_bytes(g) / 1e9:.0f} GB")
     else:
         recon = "ring reconstruction"

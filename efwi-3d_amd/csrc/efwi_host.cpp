@@ -342,12 +342,34 @@ int Ctx::plan_memory() {
     return fail(EFWI_ERR_ARG, "memory_mode must be 0 (ring) or 1 (checkpointed ring)");
   }
   nseg = (steps + seg_len - 1) / seg_len;
+  return cfg.memory_mode == 1 ? choose_resident() : (res_seg = 1, EFWI_OK);
+}
+
+// memory_mode 1: how many of the last segments keep the ring the forward pass
+// writes instead of being replayed. Each resident segment costs seg_len ring
+// slots and saves one checkpoint and one replay of seg_len forward steps;
+// automatic: as many as 85 % of the free device memory holds (at least one).
+int Ctx::choose_resident() {
+  if (res_req > 0) {
+    res_seg = std::min(res_req, nseg);
+    return EFWI_OK;
+  }
+  size_t free_b = 0, total_b = 0;
+  HIP_TRY(hipMemGetInfo(&free_b, &total_b));
+  // memory already held for the ring and checkpoints counts as available
+  const double avail = 0.85 * (double)(free_b + ring_bytes + ckpt_bytes) -
+                       (F2.w ? 0.0 : (double)G.fs * (F_WAVE_COUNT + P_PSI_COUNT));
+  const double seg_b = (double)B.n * 9 * seg_len * sizeof(float);
+  const double ck_b = (double)ckpt_floats_needed() * sizeof(float);
+  // R resident segments: R seg_b + (nseg - R) ck_b <= avail
+  const double r = seg_b > ck_b ? (avail - nseg * ck_b) / (seg_b - ck_b) : nseg;
+  res_seg = (int)std::max(1.0, std::min((double)nseg, std::floor(r)));
   return EFWI_OK;
 }
 
 int Ctx::ensure_ring() {
-  const size_t need = (size_t)B.n * 9 * (size_t)seg_len * sizeof(float);
-  if (ring && ring_bytes >= need) return EFWI_OK;
+  const size_t need = (size_t)B.n * 9 * (size_t)seg_len * res_seg * sizeof(float);
+  if (ring && ring_bytes >= need) return cfg.memory_mode == 1 ? ensure_checkpoints() : EFWI_OK;
   if (ring) {
     hipFree(ring);
     bytes_held -= (long long)ring_bytes;
@@ -359,7 +381,8 @@ int Ctx::ensure_ring() {
     ring_bytes = 0;
     return fail(EFWI_ERR_NOMEM, "boundary ring of " + std::to_string(need) +
                                     " B does not fit in device memory: " + hipGetErrorString(e) +
-                                    " (memory_mode 1 keeps only sqrt(nt) steps of it)");
+                                    " (memory_mode 1 keeps only sqrt(nt) steps of it; "
+                                    "efwi_set_resident_segments lowers the resident part)");
   }
   ring_bytes = need;
   bytes_held += (long long)need;
@@ -381,11 +404,18 @@ size_t Ctx::slab_floats(int axis) const {
   return (size_t)cfg.nz * cfg.nx * 2 * p;
 }
 
-int Ctx::ensure_checkpoints() {
+// Floats of one checkpoint: v and s in full, and of each psi array its
+// C-PML slabs (ensure_checkpoints).
+size_t Ctx::ckpt_floats_needed() const {
   size_t per = 9 * (size_t)N;
   for (int k = 0; k < 18; k++) per += slab_floats(kPsiAxis[k]);
+  return per;
+}
+
+int Ctx::ensure_checkpoints() {
+  const size_t per = ckpt_floats_needed();
   ckpt_floats = per;
-  const size_t need = per * (size_t)nseg * sizeof(float);
+  const size_t need = per * (size_t)std::max(first_res(), 1) * sizeof(float);
   int rc;
   if (!F2.w) {
     const size_t fs = G.fs, fsf = fs / sizeof(float);
@@ -659,15 +689,16 @@ int Ctx::run_shot(const efwi_shot* sh, int calc_id, const float* obs, float* syn
   Source Sf;
   Sf.cell = src_stencil ? src_cell : -1;
   Sf.stf = stf_dev;
-  const int last_seg = nseg - 1;
+  const int last_seg = nseg - 1, res0 = first_res();
   for (int it = 0; it <= nt - 2; it++) {
     const int seg = it / seg_len;
-    if (grad && cfg.memory_mode == 1 && it % seg_len == 0 && seg < last_seg)
+    if (grad && cfg.memory_mode == 1 && it % seg_len == 0 && seg < res0)
       if ((rc = checkpoint(seg, W, P, true, s))) return rc;
-    // The ring slots of the last segment are filled by this pass directly:
-    // slot it - a holds the state before step it (a: the segment's first
-    // step), the first by a pass of its own, the others by the step before.
-    const int a = last_seg * seg_len;
+    // The ring slots of the resident segments are filled by this pass
+    // directly: slot it - a holds the state before step it (a: the first
+    // resident step), the first by a pass of its own, the others by the step
+    // before.
+    const int a = res0 * seg_len;
     if (grad && it == a) {
       fields9(W, F9);
       prof.begin(K_RING, s);
@@ -716,7 +747,11 @@ int Ctx::run_shot(const efwi_shot* sh, int calc_id, const float* obs, float* syn
     fields9(W2, F9r);
     for (int seg = last_seg; seg >= 0; seg--) {
       const int a = seg * seg_len, b = std::min(nt - 2, a + seg_len - 1);
-      if (seg < last_seg) {
+      // slots of a resident segment count from the first resident step; a
+      // replayed one fills slots 0 .. seg_len - 1 (those of the first
+      // resident segment, consumed by then)
+      const int a_slot = seg >= res0 ? res0 * seg_len : a;
+      if (seg < res0) {
         // replay the segment from its checkpoint to rebuild its ring
         if ((rc = checkpoint(seg, W2, P2, false, s))) return rc;
         prof.begin(K_RING, s);
@@ -725,9 +760,9 @@ int Ctx::run_shot(const efwi_shot* sh, int calc_id, const float* obs, float* syn
         for (int it = a; it <= b; it++) forward_step(F2, W2, Sf, it, false, s, it < b ? it + 1 - a : -1);
       }
       for (int it = b; it >= a; it--) {
-        // the restores of slot it - a, folded into the kernels before them
-        // where those can take them
-        const long slot = it - a;
+        // the restores of slot it - a_slot, folded into the kernels before
+        // them where those can take them
+        const long slot = it - a_slot;
         RingFold rf{};
         rf.B = B;
         rf.slot = ring + slot * 9 * B.n;
@@ -986,6 +1021,23 @@ int efwi_set_split(efwi_ctx* ctx, int on) {
 int efwi_get_split(const efwi_ctx* ctx, int* on) {
   if (!ctx || !on) return fail(EFWI_ERR_ARG, "NULL argument");
   *on = reinterpret_cast<const Ctx*>(ctx)->split_bwd;
+  return EFWI_OK;
+}
+
+int efwi_set_resident_segments(efwi_ctx* ctx, int segments) {
+  if (!ctx) return fail(EFWI_ERR_ARG, "NULL context");
+  if (segments < 0) return fail(EFWI_ERR_ARG, "segments must be >= 0 (0: automatic)");
+  Ctx* c = reinterpret_cast<Ctx*>(ctx);
+  HIP_TRY(hipSetDevice(c->cfg.device));
+  c->res_req = segments;
+  return c->cfg.memory_mode == 1 ? c->choose_resident() : EFWI_OK;
+}
+
+int efwi_get_resident_segments(const efwi_ctx* ctx, int* segments, int* total) {
+  if (!ctx || !segments || !total) return fail(EFWI_ERR_ARG, "NULL argument");
+  const Ctx* c = reinterpret_cast<const Ctx*>(ctx);
+  *segments = c->res_seg;
+  *total = c->nseg;
   return EFWI_OK;
 }
 

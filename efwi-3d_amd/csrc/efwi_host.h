@@ -90,11 +90,18 @@ struct Ctx {
   Band B{};
   float *lam = nullptr, *mu = nullptr, *byc = nullptr;
   bool model_set = false;
-  // ring of the reverse reconstruction: [seg_len][9][B.n]
+  // ring of the reverse reconstruction: [res_seg * seg_len][9][B.n]. The
+  // last res_seg segments of the time loop keep the ring the forward pass
+  // writes (all of it in memory_mode 0, where there is one segment); in
+  // memory_mode 1 the earlier segments are replayed from checkpoints into
+  // the first seg_len slots once the resident segments are consumed.
   float* ring = nullptr;
   size_t ring_bytes = 0;
   int seg_len = 0, nseg = 1;
-  // memory_mode 1: checkpoints [nseg][ckpt_floats] and the replay state
+  int res_seg = 1;  // resident segments (1 <= res_seg <= nseg)
+  int res_req = 0;  // requested resident segments, 0 = automatic (from the free device memory)
+  int first_res() const { return nseg - res_seg; }
+  // memory_mode 1: checkpoints [nseg - res_seg][ckpt_floats] and the replay state
   float* ckpt = nullptr;
   size_t ckpt_bytes = 0, ckpt_floats = 0;
   Wave W2{};
@@ -131,6 +138,8 @@ struct Ctx {
   int copy_out(float* user, const float* field, hipMemcpyKind k, hipStream_t s);
   int init();
   int plan_memory();
+  size_t ckpt_floats_needed() const;
+  int choose_resident();
   int ensure_ring();
   int ensure_checkpoints();
   size_t slab_floats(int axis) const;

@@ -64,6 +64,8 @@ SIGNATURES = {
     "efwi_dot64": (C.c_int, [_vp, _vp, C.c_longlong, _dp, _vp]),
     "efwi_set_tile": (C.c_int, [_vp, C.c_int, C.c_int, C.c_int]),
     "efwi_set_fusion": (C.c_int, [_vp, C.c_int, C.c_int, C.c_int]),
+    "efwi_set_resident_segments": (C.c_int, [_vp, C.c_int]),
+    "efwi_get_resident_segments": (C.c_int, [_vp, _vp, _vp]),
     "efwi_set_split": (C.c_int, [_vp, C.c_int]),
     "efwi_get_split": (C.c_int, [_vp, _ip]),
     "efwi_get_tile": (C.c_int, [_vp, C.c_int, _ip, _ip]),

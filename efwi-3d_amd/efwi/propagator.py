@@ -151,6 +151,19 @@ class Propagator:
         N.check(N.lib().efwi_get_split(self._h, C.byref(v)), "efwi_get_split")
         return bool(v.value)
 
+    def set_resident_segments(self, segments: int = 0):
+        """memory_mode 1: the time loop's last `segments` segments keep the
+        forward pass's boundary ring instead of being replayed from
+        checkpoints (0: as many as the free device memory holds; results do
+        not depend on it)."""
+        N.check(N.lib().efwi_set_resident_segments(self._h, int(segments)), "efwi_set_resident_segments")
+
+    def resident_segments(self) -> tuple:
+        """(resident segments, segments of the time loop)."""
+        r, n = C.c_int(), C.c_int()
+        N.check(N.lib().efwi_get_resident_segments(self._h, C.byref(r), C.byref(n)), "efwi_get_resident_segments")
+        return r.value, n.value
+
     def tiles(self) -> dict:
         out = {}
         for k in N.TUNED_KERNELS:

@@ -158,6 +158,17 @@ int efwi_autotune(efwi_ctx* ctx, int reps, double* report, void* stream);
 int efwi_set_split(efwi_ctx* ctx, int on);
 int efwi_get_split(const efwi_ctx* ctx, int* on);
 
+/* memory_mode 1: how many of the time loop's last segments keep the boundary
+ * ring the forward pass writes (segments >= 1; 0 = automatic, the default: as
+ * many as 85 % of the free device memory holds). Only the earlier segments
+ * are checkpointed and replayed, so each resident segment saves a replay of
+ * checkpoint_every forward steps; the gradient is bit-identical whatever the
+ * number. efwi_get_resident_segments reports it and the number of segments.
+ * No reference counterpart (the reference keeps the whole ring in device
+ * memory: Src/libCUFD.cu:86, Src/Boundary.cu:24). */
+int efwi_set_resident_segments(efwi_ctx* ctx, int segments);
+int efwi_get_resident_segments(const efwi_ctx* ctx, int* segments, int* total);
+
 /* Fused time steps. pass 0: the forward step's stress and velocity updates in
  * one launch that marches z over workgroups of 64 x (rows + 4) threads, each
  * owning `rows` x rows (4, 8 or 12) of 60 y cells and `zchunk` z planes and

@@ -18,13 +18,15 @@ pytestmark = pytest.mark.gpu
 TOL = 1e-5
 
 
-def _gpu_run(w, model, calc_id, obs=None, tiles=None):
+def _gpu_run(w, model, calc_id, obs=None, tiles=None, resident=None):
     from efwi.propagator import Propagator
     print(f"[gpu] {w.name} grid={w.grid.shape} nt={w.grid.nt} form={w.grid.formulation} calc={calc_id} "
-          f"tiles={tiles}", flush=True)
+          f"tiles={tiles} resident={resident}", flush=True)
     with Propagator(w.grid) as P:
         for k, (r, z) in (tiles or {}).items():
             P.set_tile(k, r, z)
+        if resident is not None:
+            P.set_resident_segments(resident)
         P.set_model(*(torch.from_numpy(a) for a in model))
         P.zero_gradient()
         total = 0.0
@@ -150,16 +152,42 @@ def test_source_near_the_edge_is_bit_identical(orc, src_z):
 
 @pytest.mark.parametrize("every", [7, 0])
 def test_checkpointed_ring_is_bit_identical(orc, small_case, every):
-    """memory_mode 1 (checkpoints every `every` steps, replayed per segment)
-    gives the same gradient as the full ring, bit for bit."""
+    """memory_mode 1 (checkpoints every `every` steps, every segment but the
+    last replayed) gives the same gradient as the full ring, bit for bit."""
     import dataclasses
     from efwi import synth
     w0, obs = small_case
     w = synth.small(n=32, npml=6, nt=140)
     _, _, _, g_ring = _gpu_run(dataclasses.replace(w, grid=dataclasses.replace(w.grid, memory_mode=0)), w.model, 1, obs)
     wc = dataclasses.replace(w, grid=dataclasses.replace(w.grid, memory_mode=1, checkpoint_every=every))
-    j, _, _, g_ck = _gpu_run(wc, w.model, 1, obs)
+    j, _, _, g_ck = _gpu_run(wc, w.model, 1, obs, resident=1)
     for a, b in zip(g_ck, g_ring):
+        assert np.array_equal(a, b)
+
+
+@pytest.mark.parametrize("resident", [2, 11, 19, 20, 0])
+def test_resident_segments_are_bit_identical(orc, small_case, resident):
+    """memory_mode 1 with the last `resident` of 20 segments (139 steps,
+    segments of 7) keeping the forward pass's ring and the earlier ones
+    replayed into the first segment's slots; 0 = as many as the device holds
+    (all of them on this grid): the full ring's misfit, seismograms and
+    gradients, bit for bit."""
+    import dataclasses
+    from efwi import synth
+    from efwi.propagator import Propagator
+    w0, obs = small_case
+    w = synth.small(n=32, npml=6, nt=140)
+    ref = _gpu_run(dataclasses.replace(w, grid=dataclasses.replace(w.grid, memory_mode=0)), w.model, 1, obs)
+    wc = dataclasses.replace(w, grid=dataclasses.replace(w.grid, memory_mode=1, checkpoint_every=7))
+    got = _gpu_run(wc, w.model, 1, obs, resident=resident)
+    with Propagator(wc.grid) as P:
+        P.set_resident_segments(resident)
+        r, n = P.resident_segments()
+    assert n == 20 and r == (resident or 20)
+    assert got[0] == ref[0]
+    for a, b in zip(got[1], ref[1]):
+        assert np.array_equal(a, b)
+    for a, b in zip(got[3], ref[3]):
         assert np.array_equal(a, b)
 
 
