@@ -948,6 +948,15 @@ int blocks_per_cu(K kernel) {
   return n;
 }
 
+// compute units of the current device (256 if the query fails)
+int device_cus() {
+  int dev = 0, n = 0;
+  if (hipGetDevice(&dev) != hipSuccess ||
+      hipDeviceGetAttribute(&n, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess || n < 1)
+    n = 256;
+  return n;
+}
+
 // a channel slice that the 16-byte granule kernels can take
 inline bool granules_ok(int c, const void* p, int ld) {
   return c % 8 == 0 && c <= 2048 && ld % 8 == 0 && (uintptr_t)p % 16 == 0;
@@ -978,11 +987,15 @@ int efwi_nn_conv3(const uint16_t* src, int lds, int kc, const uint16_t* w, const
   // per wavefront (a divisor of the block count, so no block is partial)
   const int MB = ny >= 64 ? 4 : ny >= 32 ? 2 : 1;
   const int nbt = nc / 16;
-  const int NB = nbt % 4 == 0 ? 4 : nbt % 3 == 0 ? 3 : nbt % 2 == 0 ? 2 : 1;
   const bool ck32 = kc % 32 == 0;
   const int TZ = conv_tz(MB);
   const int ntz = (nz + TZ - 1) / TZ, ntx = (nx + 1) / 2, nty = (ny + 16 * MB - 1) / (16 * MB);
   const long bricks = (long)ntz * ntx * nty;
+  // where the bricks times the output-block groups would leave CUs without a
+  // workgroup (the coarsest levels), one block per workgroup: more
+  // workgroups, whose wavefronts hide each other's weight-load latency
+  int NB = nbt % 4 == 0 ? 4 : nbt % 3 == 0 ? 3 : nbt % 2 == 0 ? 2 : 1;
+  if (bricks * (nbt / NB) < device_cus()) NB = 1;
   const int per_xcd = (int)((bricks + 7) / 8);
   const dim3 grid((unsigned)(8L * per_xcd), (unsigned)(nbt / NB));
   // outputs (and mask) as 16-byte granules where the pointers allow

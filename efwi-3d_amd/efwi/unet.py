@@ -76,15 +76,39 @@ class UNet3D:
         self.dev = torch.device(f"cuda:{device}")
         self.lib = N.lib()
         w, b, wh, bh = init_params(seed)
-        self.w = {k: v.to(self.dev) for k, v in w.items()}
-        self.b = {k: v.to(self.dev) for k, v in b.items()}
-        self.w16 = {}
-        self.wt16 = {}
-        self.wh = wh.to(self.dev)
-        self.bh = bh.to(self.dev)
+        init = []
+        for name, *_ in LAYERS:
+            init += [(name + ".w", w[name]), (name + ".b", b[name])]
+        init += [("head.w", wh), ("head.b", bh)]
+        # One flat buffer per role (FP32 weights, gradients, Adam moments, the
+        # BF16 copy): a single Adam launch updates every parameter. The
+        # per-parameter tensors are views; their offsets stay multiples of 8
+        # elements (16-byte aligned BF16 weights), as every weight tensor holds
+        # 27 cout cin values, every bias a multiple of 16 and the head is last.
+        n = sum(v.numel() for _, v in init)
+        self._flat = torch.empty(n, device=self.dev)
+        self._gflat = torch.zeros(n, device=self.dev)
+        self._m1flat = torch.zeros(n, device=self.dev)
+        self._m2flat = torch.zeros(n, device=self.dev)
+        self._flat16 = torch.empty(n, dtype=torch.bfloat16, device=self.dev)
+        self._offsets = {}
+        off = 0
+        for k, v in init:
+            self._offsets[k] = (off, v.shape)
+            off += v.numel()
+        view = lambda flat, k: flat[self._offsets[k][0]:self._offsets[k][0] + self._offsets[k][1].numel()].view(
+            self._offsets[k][1])
+        self._flat.copy_(torch.cat([v.reshape(-1) for _, v in init]).to(self.dev))
+        self.w = {name: view(self._flat, name + ".w") for name, *_ in LAYERS}
+        self.b = {name: view(self._flat, name + ".b") for name, *_ in LAYERS}
+        self.wh = view(self._flat, "head.w")
+        self.bh = view(self._flat, "head.b")
+        self.grads = {k: view(self._gflat, k) for k, _ in init}
+        self.m1 = {k: view(self._m1flat, k) for k, _ in init}
+        self.m2 = {k: view(self._m2flat, k) for k, _ in init}
+        self.w16 = {name: view(self._flat16, name + ".w") for name, *_ in LAYERS}
+        self.wt16 = {name: torch.empty_like(self.w16[name]) for name, *_ in LAYERS}
         self._refresh_bf16()
-        self.m1 = {k: torch.zeros_like(v) for k, v in self._params().items()}
-        self.m2 = {k: torch.zeros_like(v) for k, v in self._params().items()}
         self.steps = 0
         self._bufs = {}
         self._scratch = None
@@ -100,9 +124,7 @@ class UNet3D:
         return out
 
     def _refresh_bf16(self):
-        for name, *_ in LAYERS:
-            self.w16[name] = self.w[name].to(torch.bfloat16)
-            self.wt16[name] = torch.empty_like(self.w16[name])
+        self._flat16.copy_(self._flat)
         self._flip()
 
     def _flip(self):
@@ -149,7 +171,6 @@ class UNet3D:
             need = max(need, int(self.lib.efwi_nn_wgrad_scratch(cout, cin, *d, self._wgrad_per(cout, cin, d))))
         if self._scratch is None or self._scratch.numel() < need:
             self._scratch = torch.empty(need, device=self.dev)
-        self.grads = {k: torch.zeros_like(v) for k, v in self._params().items()}
         return B
 
     def _stream(self):
@@ -272,13 +293,9 @@ class UNet3D:
 
     def adam(self, lr=1e-3, b1=0.9, b2=0.999, eps=1e-8):
         self.steps += 1
-        s = self._stream()
-        for k, w in self._params().items():
-            name = k.split(".")[0]
-            w16 = self.w16[name] if k.endswith(".w") and name in self.w16 else None
-            nn_check(self.lib.efwi_nn_adam(_p(w), _p(self.grads[k]), _p(self.m1[k]), _p(self.m2[k]), w.numel(),
-                                           lr, b1, b2, eps, self.steps, None if w16 is None else _p(w16), s),
-                     f"adam {k}")
+        nn_check(self.lib.efwi_nn_adam(_p(self._flat), _p(self._gflat), _p(self._m1flat), _p(self._m2flat),
+                                       self._flat.numel(), lr, b1, b2, eps, self.steps, _p(self._flat16),
+                                       self._stream()), "adam")
         self._flip()
 
     def train_step(self, feat, target, mask, count, dims, lr=1e-3, log_scaling=False):
