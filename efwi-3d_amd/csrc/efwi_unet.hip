@@ -356,7 +356,10 @@ __global__ void k_flip_weights(const uint16_t* __restrict__ w, int kc, int nc, u
 // 16-byte granules of the halo row from which the three dy shifts of the B
 // fragment (B[k][n] = src[voxel k + d_t][n]) follow by dword selection and
 // byte alignment. The four wavefronts' sums meet in LDS, and the block writes
-// part[chunk][t][m][n] for its m and n blocks.
+// part[chunk][t][m][n] for its m and n blocks. With db, the blocks of the first
+// n block also sum the A fragments they read (dpre over their voxels: the bias
+// gradient of their m block) into part[chunk][27 mc kc + m]; a chunk's
+// partials are 27 mc kc + mc floats.
 // Transposed store of 4 voxels x 8 channels (q[k] = channels of voxel k):
 // channel j's four values at dst[j * pitch .. + 3], one 8-byte store each.
 __device__ __forceinline__ void store_quads(uint16_t* dst, int pitch, const uint4 (&q)[4]) {
@@ -373,8 +376,8 @@ __device__ __forceinline__ void store_quads(uint16_t* dst, int pitch, const uint
 template <int TY>
 __global__ __launch_bounds__(256) void k_wgrad(const uint16_t* __restrict__ dpre, int ldp, int mc,
                                                const uint16_t* __restrict__ src, int lds, int kc,
-                                               float* __restrict__ part, float* __restrict__ dw, Vol V, int ntz,
-                                               int ntx, int nty, int per) {
+                                               float* __restrict__ part, float* __restrict__ dw,
+                                               float* __restrict__ db, Vol V, int ntz, int ntx, int nty, int per) {
   constexpr int HW = TY + 2, P = TY + 8, NK = TY / 32;
   constexpr int A_BYTES = 4 * 16 * P * 2, B_BYTES = 16 * 16 * P * 2, R_BYTES = 9 * 4 * 256 * 4;
   constexpr int S_BYTES = A_BYTES + (B_BYTES > R_BYTES ? B_BYTES : R_BYTES);
@@ -388,6 +391,8 @@ __global__ __launch_bounds__(256) void k_wgrad(const uint16_t* __restrict__ dpre
   const int mb = blockIdx.y, nb = blockIdx.z;
   const long nbricks = (long)ntz * ntx * nty;
   const long b0 = (long)blockIdx.x * per, b1 = std::min(nbricks, b0 + per);
+  const bool bias = db != nullptr && nb == 0;
+  float bsum = 0.f;  // lane (i, g): dpre channel mb * 16 + i over its voxels
   f4 acc[27];
 #pragma unroll
   for (int t = 0; t < 27; t++) acc[t] = f4{0.f, 0.f, 0.f, 0.f};
@@ -443,6 +448,11 @@ __global__ __launch_bounds__(256) void k_wgrad(const uint16_t* __restrict__ dpre
 #pragma unroll 1
     for (int c = 0; c < NK; c++) {
       const uint4 a = *reinterpret_cast<const uint4*>(&Ah[(wv * 16 + i) * P + 32 * c + 8 * g]);
+      if (bias) {
+        const unsigned u[4] = {a.x, a.y, a.z, a.w};
+#pragma unroll
+        for (int j = 0; j < 4; j++) bsum += __uint_as_float(u[j] << 16) + __uint_as_float(u[j] & 0xFFFF0000u);
+      }
 #pragma unroll
       for (int dzdx = 0; dzdx < 9; dzdx++) {
         const int hr = (wz + dzdx / 3) * 4 + (wx + dzdx % 3);
@@ -465,6 +475,7 @@ __global__ __launch_bounds__(256) void k_wgrad(const uint16_t* __restrict__ dpre
   }
   // sum of the four wavefronts, 9 taps at a time; lane (i, g) holds
   // D[m = 4 g + j][n = i] of each tap
+  const long W = 27L * mc * kc, stride = W + mc;
 #pragma unroll
   for (int p = 0; p < 3; p++) {
 #pragma unroll
@@ -478,25 +489,38 @@ __global__ __launch_bounds__(256) void k_wgrad(const uint16_t* __restrict__ dpre
     for (int tt = 0; tt < 9; tt++) {
       const float s = red[(tt * 4 + 0) * 256 + tid] + red[(tt * 4 + 1) * 256 + tid] + red[(tt * 4 + 2) * 256 + tid] +
                       red[(tt * 4 + 3) * 256 + tid];
-      part[(((long)blockIdx.x * 27 + 9 * p + tt) * mc + m) * kc + n] = s;
+      part[(long)blockIdx.x * stride + ((long)(9 * p + tt) * mc + m) * kc + n] = s;
       // dw is summed from the partials by atomics after this kernel: the
       // first chunk's blocks clear it
       if (blockIdx.x == 0) dw[((long)(9 * p + tt) * mc + m) * kc + n] = 0.f;
     }
     __syncthreads();
   }
+  if (bias) {
+    red[tid] = bsum;  // tid = 64 wv + 16 g + i
+    __syncthreads();
+    if (tid < 16) {
+      float s = 0.f;
+#pragma unroll
+      for (int q = 0; q < 16; q++) s += red[16 * q + tid];
+      part[(long)blockIdx.x * stride + W + mb * 16 + tid] = s;
+      if (blockIdx.x == 0) db[mb * 16 + tid] = 0.f;
+    }
+  }
 }
 
-// out[k] += sum over chunks [blockIdx.y * kChunkGroup, +kChunkGroup) of part[chunk][k]
-// (out cleared beforehand; one atomic per element per chunk group)
+// out[k] (k < width) or out2[k - width] (width <= k < total) += sum over the
+// chunks [blockIdx.y * kChunkGroup, +kChunkGroup) of part[chunk * stride + k]
+// (outputs cleared beforehand; one atomic per element per chunk group)
 constexpr int kChunkGroup = 32;
-__global__ void k_sum_chunks(const float* __restrict__ part, long width, int nchunks, float* __restrict__ out) {
+__global__ void k_sum_chunks(const float* __restrict__ part, long stride, long width, long total, int nchunks,
+                             float* __restrict__ out, float* __restrict__ out2) {
   const long k = (long)blockIdx.x * blockDim.x + threadIdx.x;
-  if (k >= width) return;
+  if (k >= total) return;
   const int c0 = blockIdx.y * kChunkGroup, c1 = min(nchunks, c0 + kChunkGroup);
   float s = 0.f;
-  for (int c = c0; c < c1; c++) s += part[(long)c * width + k];
-  atomicAdd(out + k, s);
+  for (int c = c0; c < c1; c++) s += part[(long)c * stride + k];
+  atomicAdd(k < width ? out + k : out2 + (k - width), s);
 }
 
 // out[c] += sum_v x[v][c] (bias gradients; out cleared beforehand). blockDim
@@ -1042,7 +1066,7 @@ static inline long wgrad_bricks(int nz, int nx, int ny) {
 long efwi_nn_wgrad_scratch(int mc, int kc, int nz, int nx, int ny, int per) {
   if (per <= 0 || nz <= 0 || nx <= 0 || ny <= 0) return 0;
   const long chunks = (wgrad_bricks(nz, nx, ny) + per - 1) / per;
-  return chunks * 27L * mc * kc;
+  return chunks * (27L * mc * kc + mc);
 }
 
 static bool wgrad_args_ok(const uint16_t* dpre, int ldp, int mc, const uint16_t* src, int lds, int kc, float* dw,
@@ -1052,13 +1076,13 @@ static bool wgrad_args_ok(const uint16_t* dpre, int ldp, int mc, const uint16_t*
          ((uintptr_t)dpre | (uintptr_t)src) % 16 == 0;
 }
 static int wgrad_launch(const uint16_t* dpre, int ldp, int mc, const uint16_t* src, int lds, int kc, float* scratch,
-                        float* dw, int nz, int nx, int ny, int per, hipStream_t s);
+                        float* dw, float* db, int nz, int nx, int ny, int per, hipStream_t s);
 
 int efwi_nn_wgrad(const uint16_t* dpre, int ldp, int mc, const uint16_t* src, int lds, int kc, float* scratch,
-                  long scratch_floats, float* dw, int nz, int nx, int ny, int per, void* stream) {
+                  long scratch_floats, float* dw, float* db, int nz, int nx, int ny, int per, void* stream) {
   if (!scratch || !wgrad_args_ok(dpre, ldp, mc, src, lds, kc, dw, nz, nx, ny, per)) return EFWI_NN_ERR_ARG;
   if (efwi_nn_wgrad_scratch(mc, kc, nz, nx, ny, per) > scratch_floats) return EFWI_NN_ERR_ARG;
-  return wgrad_launch(dpre, ldp, mc, src, lds, kc, scratch, dw, nz, nx, ny, per, (hipStream_t)stream);
+  return wgrad_launch(dpre, ldp, mc, src, lds, kc, scratch, dw, db, nz, nx, ny, per, (hipStream_t)stream);
 }
 
 // The partial sums of efwi_nn_wgrad_noalloc: one array per GPU, for the
@@ -1089,31 +1113,31 @@ static float* static_scratch(int* err) {
 }
 
 int efwi_nn_wgrad_noalloc(const uint16_t* dpre, int ldp, int mc, const uint16_t* src, int lds, int kc, float* dw,
-                          int nz, int nx, int ny, int per, void* stream) {
+                          float* db, int nz, int nx, int ny, int per, void* stream) {
   if (!wgrad_args_ok(dpre, ldp, mc, src, lds, kc, dw, nz, nx, ny, per)) return EFWI_NN_ERR_ARG;
   if (efwi_nn_wgrad_scratch(mc, kc, nz, nx, ny, per) > EFWI_NN_WGRAD_STATIC_FLOATS) return EFWI_NN_ERR_SCRATCH;
   int err = 0;
   float* ptr = static_scratch(&err);
   if (!ptr) return err;
-  return wgrad_launch(dpre, ldp, mc, src, lds, kc, ptr, dw, nz, nx, ny, per, (hipStream_t)stream);
+  return wgrad_launch(dpre, ldp, mc, src, lds, kc, ptr, dw, db, nz, nx, ny, per, (hipStream_t)stream);
 }
 
 static int wgrad_launch(const uint16_t* dpre, int ldp, int mc, const uint16_t* src, int lds, int kc, float* scratch,
-                        float* dw, int nz, int nx, int ny, int per, hipStream_t s) {
+                        float* dw, float* db, int nz, int nx, int ny, int per, hipStream_t s) {
   const Vol V{nz, nx, ny};
   const int ty = wgrad_ty(ny);
   const int ntz = (nz + 1) / 2, ntx = (nx + 1) / 2, nty = (ny + ty - 1) / ty;
   const int chunks = (int)((wgrad_bricks(nz, nx, ny) + per - 1) / per);
   const dim3 grid((unsigned)chunks, (unsigned)(mc / 16), (unsigned)(kc / 16));
   if (ty == 64)
-    hipLaunchKernelGGL((k_wgrad<64>), grid, dim3(256), 0, s, dpre, ldp, mc, src, lds, kc, scratch, dw, V, ntz, ntx, nty,
-                       per);
+    hipLaunchKernelGGL((k_wgrad<64>), grid, dim3(256), 0, s, dpre, ldp, mc, src, lds, kc, scratch, dw, db, V, ntz, ntx,
+                       nty, per);
   else
-    hipLaunchKernelGGL((k_wgrad<32>), grid, dim3(256), 0, s, dpre, ldp, mc, src, lds, kc, scratch, dw, V, ntz, ntx, nty,
-                       per);
-  const long width = 27L * mc * kc;
-  hipLaunchKernelGGL(k_sum_chunks, dim3(grid_for(width, 256, 1L << 30), (chunks + kChunkGroup - 1) / kChunkGroup),
-                     dim3(256), 0, s, scratch, width, chunks, dw);
+    hipLaunchKernelGGL((k_wgrad<32>), grid, dim3(256), 0, s, dpre, ldp, mc, src, lds, kc, scratch, dw, db, V, ntz, ntx,
+                       nty, per);
+  const long width = 27L * mc * kc, total = width + (db ? mc : 0);
+  hipLaunchKernelGGL(k_sum_chunks, dim3(grid_for(total, 256, 1L << 30), (chunks + kChunkGroup - 1) / kChunkGroup),
+                     dim3(256), 0, s, scratch, width + mc, width, total, chunks, dw, db);
   return nn_check(s);
 }
 

@@ -85,9 +85,9 @@ SIGNATURES = {
                                 C.c_int, C.c_int, C.c_int, C.c_int, C.c_int, _vp]),
     "efwi_nn_flip_weights": (C.c_int, [_vp, C.c_int, C.c_int, _vp, _vp]),
     "efwi_nn_wgrad_scratch": (C.c_long, [C.c_int, C.c_int, C.c_int, C.c_int, C.c_int, C.c_int]),
-    "efwi_nn_wgrad": (C.c_int, [_vp, C.c_int, C.c_int, _vp, C.c_int, C.c_int, _vp, C.c_long, _vp,
+    "efwi_nn_wgrad": (C.c_int, [_vp, C.c_int, C.c_int, _vp, C.c_int, C.c_int, _vp, C.c_long, _vp, _vp,
                                 C.c_int, C.c_int, C.c_int, C.c_int, _vp]),
-    "efwi_nn_wgrad_noalloc": (C.c_int, [_vp, C.c_int, C.c_int, _vp, C.c_int, C.c_int, _vp, C.c_int, C.c_int,
+    "efwi_nn_wgrad_noalloc": (C.c_int, [_vp, C.c_int, C.c_int, _vp, C.c_int, C.c_int, _vp, _vp, C.c_int, C.c_int,
                                         C.c_int, C.c_int, _vp]),
     "efwi_nn_chan_sum": (C.c_int, [_vp, C.c_int, C.c_int, C.c_long, _vp, _vp]),
     "efwi_nn_pool2": (C.c_int, [_vp, C.c_int, _vp, C.c_int, C.c_int, C.c_int, C.c_int, C.c_int, _vp]),

@@ -194,7 +194,7 @@ class UNet3D:
         and 8 bricks each (fewer bricks per workgroup multiply the FP32
         partials that are written and summed; measured on MI355X at the
         U-Net's shapes)."""
-        bricks = int(self.lib.efwi_nn_wgrad_scratch(cout, cin, *dims, 1)) // (27 * cout * cin)
+        bricks = int(self.lib.efwi_nn_wgrad_scratch(cout, cin, *dims, 1)) // (27 * cout * cin + cout)
         blocks = (cout // 16) * (cin // 16)
         return int(min(8, max(2, bricks * blocks // WGRAD_WORKGROUPS)))
 
@@ -202,10 +202,9 @@ class UNet3D:
         _, cin, cout, _ = next(l for l in LAYERS if l[0] == name)
         s = self._stream()
         nn_check(self.lib.efwi_nn_wgrad(_p(gpre, goff), ldg, cout, _p(src, soff), lds, cin, _p(self._scratch),
-                                        self._scratch.numel(), _p(self.grads[name + ".w"]), *dims,
-                                        self._wgrad_per(cout, cin, dims), s), f"wgrad {name}")
-        nn_check(self.lib.efwi_nn_chan_sum(_p(gpre, goff), ldg, cout, dims[0] * dims[1] * dims[2],
-                                           _p(self.grads[name + ".b"]), s), f"bias grad {name}")
+                                        self._scratch.numel(), _p(self.grads[name + ".w"]),
+                                        _p(self.grads[name + ".b"]), *dims, self._wgrad_per(cout, cin, dims), s),
+                 f"wgrad {name}")
 
     # ---- forward / backward
     def forward(self, feat, dims):

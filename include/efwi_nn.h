@@ -49,16 +49,17 @@ int efwi_nn_conv3(const uint16_t* src, int lds, int kc, const uint16_t* w, const
 int efwi_nn_flip_weights(const uint16_t* w, int kc, int nc, uint16_t* wt, void* stream);
 
 /* Weight gradient of a 3x3x3 convolution, dw[t][m][n] = sum_v dpre[v][m]
- * src[v + d_t][n] (FP32 result), dpre = gradient of the layer's
- * pre-activation (mc channels), src = its input (kc channels); dpre and src
- * 16-byte aligned (a channel slice starts at a multiple of 8 channels). The
- * volume is cut into bricks of 2 x 2 rows of 64 voxels along y (32 where ny
- * <= 32); each workgroup reduces `per` bricks for one 16 x 16 block of (m, n)
- * on v_mfma_f32_16x16x32_bf16 into FP32 partials in `scratch`
+ * src[v + d_t][n] (FP32 result), and, when db is not NULL, its bias gradient
+ * db[m] = sum_v dpre[v][m]; dpre = gradient of the layer's pre-activation (mc
+ * channels), src = its input (kc channels); dpre and src 16-byte aligned (a
+ * channel slice starts at a multiple of 8 channels). The volume is cut into
+ * bricks of 2 x 2 rows of 64 voxels along y (32 where ny <= 32); each
+ * workgroup reduces `per` bricks for one 16 x 16 block of (m, n) on
+ * v_mfma_f32_16x16x32_bf16 into FP32 partials in `scratch`
  * (efwi_nn_wgrad_scratch floats), which are then summed. */
 long efwi_nn_wgrad_scratch(int mc, int kc, int nz, int nx, int ny, int per);
 int efwi_nn_wgrad(const uint16_t* dpre, int ldp, int mc, const uint16_t* src, int lds, int kc, float* scratch,
-                  long scratch_floats, float* dw, int nz, int nx, int ny, int per, void* stream);
+                  long scratch_floats, float* dw, float* db, int nz, int nx, int ny, int per, void* stream);
 
 /* efwi_nn_wgrad without a workspace argument and without any allocation:
  * the partial sums live in a static device array of
@@ -73,9 +74,9 @@ int efwi_nn_wgrad(const uint16_t* dpre, int ldp, int mc, const uint16_t* src, in
  * memory (code under graph capture, training loops that forbid it). */
 #define EFWI_NN_WGRAD_STATIC_FLOATS (4L << 20)
 int efwi_nn_wgrad_noalloc(const uint16_t* dpre, int ldp, int mc, const uint16_t* src, int lds, int kc, float* dw,
-                          int nz, int nx, int ny, int per, void* stream);
+                          float* db, int nz, int nx, int ny, int per, void* stream);
 
-/* out[c] = sum_v x[v][c] (bias gradients), c divides 256, ld >= c. */
+/* out[c] = sum_v x[v][c] (a bias gradient on its own), c divides 256, ld >= c. */
 int efwi_nn_chan_sum(const uint16_t* x, int ld, int c, long nvox, float* out, void* stream);
 
 /* 2x2x2 average pooling of a (nz, nx, ny) volume (even sizes) and its

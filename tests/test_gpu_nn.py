@@ -124,14 +124,23 @@ def test_conv3_weight_gradient(dims, mc, kc):
     # (every device operand is held in a variable: a temporary would be freed
     # before the asynchronous kernel reads it)
     dd, xd = _dev(dpre.reshape(V, mc)), _dev(x.reshape(V, kc))
-    nn_check(N.lib().efwi_nn_wgrad(_p(dd), mc, mc, _p(xd), kc, kc, _p(scratch), need, _p(dw), nz, nx, ny, per, _s()),
-             "wgrad")
+    db = torch.full((mc,), 7.0, device=DEV)
+    nn_check(N.lib().efwi_nn_wgrad(_p(dd), mc, mc, _p(xd), kc, kc, _p(scratch), need, _p(dw), _p(db), nz, nx, ny, per,
+                                   _s()), "wgrad")
     ref = O.conv3_wgrad(dpre, x)
     got = dw.cpu().numpy()
     assert np.linalg.norm(got - ref) <= 1e-5 * np.linalg.norm(ref)
+    # the bias gradient summed on the side: sum over the voxels of dpre
+    ref_b = dpre.reshape(V, mc).astype(np.float64).sum(axis=0)
+    assert np.linalg.norm(db.cpu().numpy() - ref_b) <= 1e-5 * np.linalg.norm(ref_b)
+    # without a bias output the weight gradient is the same
+    dw2 = torch.full_like(dw, 3.0)
+    nn_check(N.lib().efwi_nn_wgrad(_p(dd), mc, mc, _p(xd), kc, kc, _p(scratch), need, _p(dw2), None, nz, nx, ny, per,
+                                   _s()), "wgrad without bias")
+    assert torch.equal(dw2, dw)
     # too small a scratch is refused
-    assert N.lib().efwi_nn_wgrad(_p(dd), mc, mc, _p(xd), kc, kc, _p(scratch), need - 1, _p(dw), nz, nx, ny, per,
-                                 _s()) == 1
+    assert N.lib().efwi_nn_wgrad(_p(dd), mc, mc, _p(xd), kc, kc, _p(scratch), need - 1, _p(dw), _p(db), nz, nx, ny,
+                                 per, _s()) == 1
 
 
 @pytest.mark.parametrize("c,ld,off,nvox", [(16, 48, 32, 12345), (64, 64, 0, 3000), (32, 96, 64, 777), (4, 16, 2, 999)])

@@ -1,9 +1,9 @@
 """efwi_nn_wgrad_noalloc (partial sums in a static device array, no
 allocation) against efwi_nn_wgrad (partial sums in a caller's workspace),
-through the C ABI: the same weight gradients (up to the order of the atomic
-FP32 additions both entry points make) on two calls in a row, the second on a
-smaller volume, and a clear refusal when the partial sums would not fit the
-static array."""
+through the C ABI: the same weight and bias gradients (up to the order of the
+atomic FP32 additions both entry points make) on two calls in a row, the second
+on a smaller volume, and a clear refusal when the partial sums would not fit
+the static array."""
 import ctypes as C
 
 import numpy as np
@@ -36,11 +36,16 @@ def _both(lib, dims, mc, kc, per, rng):
     ws = torch.empty(need, device=DEV)
     ref = torch.full((27, mc, kc), np.nan, device=DEV)
     got = torch.full((27, mc, kc), np.nan, device=DEV)
+    ref_b = torch.full((mc,), np.nan, device=DEV)
+    got_b = torch.full((mc,), np.nan, device=DEV)
     print(f"[gpu] wgrad noalloc {dims} {mc}x{kc} per={per} partials={need}", flush=True)
-    assert lib.efwi_nn_wgrad(_p(dpre), mc, mc, _p(src), kc, kc, _p(ws), need, _p(ref), nz, nx, ny, per, _s()) == 0
-    assert lib.efwi_nn_wgrad_noalloc(_p(dpre), mc, mc, _p(src), kc, kc, _p(got), nz, nx, ny, per, _s()) == 0
+    assert lib.efwi_nn_wgrad(_p(dpre), mc, mc, _p(src), kc, kc, _p(ws), need, _p(ref), _p(ref_b), nz, nx, ny, per,
+                             _s()) == 0
+    assert lib.efwi_nn_wgrad_noalloc(_p(dpre), mc, mc, _p(src), kc, kc, _p(got), _p(got_b), nz, nx, ny, per,
+                                     _s()) == 0
     torch.cuda.synchronize()
-    return ref.cpu().numpy(), got.cpu().numpy()
+    return (np.concatenate([ref.cpu().numpy().ravel(), ref_b.cpu().numpy()]),
+            np.concatenate([got.cpu().numpy().ravel(), got_b.cpu().numpy()]))
 
 
 def test_noalloc_wgrad_equals_workspace_wgrad():
@@ -72,6 +77,6 @@ def test_noalloc_wgrad_refuses_partials_beyond_its_array():
     src = torch.zeros(16, kc, dtype=torch.bfloat16, device=DEV)
     dw = torch.zeros(27, mc, kc, device=DEV)
     # EFWI_NN_ERR_SCRATCH; nothing is launched (the sizes are checked first)
-    assert lib.efwi_nn_wgrad_noalloc(_p(dpre), mc, mc, _p(src), kc, kc, _p(dw), nz, nx, ny, per, _s()) == 3
+    assert lib.efwi_nn_wgrad_noalloc(_p(dpre), mc, mc, _p(src), kc, kc, _p(dw), None, nz, nx, ny, per, _s()) == 3
     # invalid arguments stay EFWI_NN_ERR_ARG
-    assert lib.efwi_nn_wgrad_noalloc(_p(dpre), mc, mc, _p(src), kc, 8, _p(dw), 1, 1, 16, 1, _s()) == 1
+    assert lib.efwi_nn_wgrad_noalloc(_p(dpre), mc, mc, _p(src), kc, 8, _p(dw), None, 1, 1, 16, 1, _s()) == 1
