@@ -599,9 +599,10 @@ __global__ void k_pool2(const uint16_t* __restrict__ src, int lds, uint16_t* __r
     dst[Cv.vox(z, x, y) * ldd + ch] = f2bf(0.125f * s);
   }
 }
-// gradient of the pooling: d_fine[v][c] (=|+=) d_coarse[parent][c] / 8
+// gradient of the pooling: d_fine[v][c] (=|+=) d_coarse[parent][c] / 8, then
+// zeroed where mask[v][c] <= 0 when a mask is given (rounded to BF16 first)
 __global__ void k_pool2_bwd(const uint16_t* __restrict__ dc, int ldc, uint16_t* __restrict__ df, int ldf, int c, Vol F,
-                            int accumulate) {
+                            int accumulate, const uint16_t* __restrict__ mask, int ldm) {
   const long n = (long)F.nz * F.nx * F.ny * c;
   for (long k = (long)blockIdx.x * blockDim.x + threadIdx.x; k < n; k += (long)gridDim.x * blockDim.x) {
     const int ch = (int)(k % c);
@@ -613,7 +614,9 @@ __global__ void k_pool2_bwd(const uint16_t* __restrict__ dc, int ldc, uint16_t* 
     float g = 0.125f * bf2f(dc[Cv.vox(z / 2, x / 2, y / 2) * ldc + ch]);
     const long o = F.vox(z, x, y) * ldf + ch;
     if (accumulate) g += bf2f(df[o]);
-    df[o] = f2bf(g);
+    uint16_t r = f2bf(g);
+    if (mask && !(bf2f(mask[F.vox(z, x, y) * ldm + ch]) > 0.f)) r = 0;
+    df[o] = r;
   }
 }
 // nearest upsampling x2: fine[v][c] = coarse[parent][c]
@@ -701,7 +704,7 @@ __global__ void k_pool2_v(const uint16_t* __restrict__ src, int lds, uint16_t* _
   }
 }
 __global__ void k_pool2_bwd_v(const uint16_t* __restrict__ dc, int ldc, uint16_t* __restrict__ df, int ldf, int c, Vol F,
-                              int accumulate) {
+                              int accumulate, const uint16_t* __restrict__ mask, int ldm) {
   const Vol Cv{F.nz / 2, F.nx / 2, F.ny / 2};
   const long n = (long)F.nz * F.nx * F.ny;
   GranuleWalk gw(c);
@@ -718,6 +721,15 @@ __global__ void k_pool2_bwd_v(const uint16_t* __restrict__ dc, int ldc, uint16_t
       unpack8(ld16(df + v * ldf + gw.g8), a);
 #pragma unroll
       for (int j = 0; j < 8; j++) gv[j] += a[j];
+    }
+    if (mask) {
+      // the BF16 rounding of the sum, then the mask (exact: a zero or the
+      // rounded value), as a separate masking pass would give
+      float m[8];
+      unpack8(ld16(mask + v * ldm + gw.g8), m);
+#pragma unroll
+      for (int j = 0; j < 8; j++)
+        if (!(m[j] > 0.f)) gv[j] = 0.f;
     }
     st16(df + v * ldf + gw.g8, pack8(gv));
   }
@@ -1165,15 +1177,15 @@ int efwi_nn_pool2(const uint16_t* src, int lds, uint16_t* dst, int ldd, int c, i
 }
 
 int efwi_nn_pool2_bwd(const uint16_t* dc, int ldc, uint16_t* df, int ldf, int c, int nz, int nx, int ny, int accumulate,
-                      void* stream) {
-  if (!dc || !df || nz % 2 || nx % 2 || ny % 2 || c <= 0) return EFWI_NN_ERR_ARG;
+                      const uint16_t* mask, int ldm, void* stream) {
+  if (!dc || !df || nz % 2 || nx % 2 || ny % 2 || c <= 0 || (mask && ldm < c)) return EFWI_NN_ERR_ARG;
   hipStream_t s = (hipStream_t)stream;
-  if (granules_ok(c, dc, ldc) && granules_ok(c, df, ldf))
+  if (granules_ok(c, dc, ldc) && granules_ok(c, df, ldf) && (!mask || granules_ok(c, mask, ldm)))
     hipLaunchKernelGGL(k_pool2_bwd_v, dim3(grid_for((long)nz * nx * ny, 256 / (c / 8))), dim3(256 / (c / 8) * (c / 8)), 0, s, dc, ldc, df, ldf, c,
-                       Vol{nz, nx, ny}, accumulate);
+                       Vol{nz, nx, ny}, accumulate, mask, ldm);
   else
     hipLaunchKernelGGL(k_pool2_bwd, dim3(grid_for((long)nz * nx * ny * c, 256)), dim3(256), 0, s, dc, ldc, df, ldf, c,
-                       Vol{nz, nx, ny}, accumulate);
+                       Vol{nz, nx, ny}, accumulate, mask, ldm);
   return nn_check(s);
 }
 

@@ -92,7 +92,7 @@ SIGNATURES = {
     "efwi_nn_chan_sum": (C.c_int, [_vp, C.c_int, C.c_int, C.c_long, _vp, _vp]),
     "efwi_nn_pool2": (C.c_int, [_vp, C.c_int, _vp, C.c_int, C.c_int, C.c_int, C.c_int, C.c_int, _vp]),
     "efwi_nn_pool2_bwd": (C.c_int, [_vp, C.c_int, _vp, C.c_int, C.c_int, C.c_int, C.c_int, C.c_int, C.c_int,
-                                    _vp]),
+                                    _vp, C.c_int, _vp]),
     "efwi_nn_up2": (C.c_int, [_vp, C.c_int, _vp, C.c_int, C.c_int, C.c_int, C.c_int, C.c_int, _vp]),
     "efwi_nn_up2_bwd": (C.c_int, [_vp, C.c_int, _vp, C.c_int, C.c_int, C.c_int, C.c_int, C.c_int, _vp,
                                   C.c_int, _vp]),

@@ -246,7 +246,6 @@ class UNet3D:
         d1 = self._dims
         d2, d3 = tuple(x // 2 for x in d1), tuple(x // 4 for x in d1)
         v1 = d1[0] * d1[1] * d1[2]
-        v2 = v1 // 8
         if through_gate:
             nn_check(self.lib.efwi_nn_gate_bwd(_p(B["o"]), _p(B["out"]), _p(dout), _p(B["do"]), v1, s), "gate bwd")
             dout = B["do"]
@@ -269,16 +268,16 @@ class UNet3D:
         self._dgrad("bot_b", B["g3b"], 64, B["g3"], 64, B["a3"], 64, d3)
         self._wgrad("bot_a", B["g3"], 64, B["p2"], 32, d3)
         self._dgrad("bot_a", B["g3"], 64, B["gp2"], 32, None, 0, d3)
-        # level 2 encoder: skip gradient (masked) + pooled gradient, then the ReLU mask
-        nn_check(self.lib.efwi_nn_pool2_bwd(_p(B["gp2"]), 32, _p(B["gcat2"], 64), 96, 32, *d2, 1, s), "pool2 bwd")
-        nn_check(self.lib.efwi_nn_add(_p(B["gcat2"], 64), 96, None, 0, 32, v2, _p(B["cat2"], 64), 96, s), "mask2")
+        # level 2 encoder: skip gradient + pooled gradient, then the ReLU mask
+        nn_check(self.lib.efwi_nn_pool2_bwd(_p(B["gp2"]), 32, _p(B["gcat2"], 64), 96, 32, *d2, 1,
+                                            _p(B["cat2"], 64), 96, s), "pool2 bwd")
         self._wgrad("enc2b", B["gcat2"], 96, B["a2"], 32, d2, goff=64)
         self._dgrad("enc2b", B["gcat2"], 96, B["g2"], 32, B["a2"], 32, d2, goff=64)
         self._wgrad("enc2a", B["g2"], 32, B["p1"], 16, d2)
         self._dgrad("enc2a", B["g2"], 32, B["gp1"], 16, None, 0, d2)
         # level 1 encoder
-        nn_check(self.lib.efwi_nn_pool2_bwd(_p(B["gp1"]), 16, _p(B["gcat1"], 32), 48, 16, *d1, 1, s), "pool1 bwd")
-        nn_check(self.lib.efwi_nn_add(_p(B["gcat1"], 32), 48, None, 0, 16, v1, _p(B["cat1"], 32), 48, s), "mask1")
+        nn_check(self.lib.efwi_nn_pool2_bwd(_p(B["gp1"]), 16, _p(B["gcat1"], 32), 48, 16, *d1, 1,
+                                            _p(B["cat1"], 32), 48, s), "pool1 bwd")
         self._wgrad("enc1b", B["gcat1"], 48, B["a1"], 16, d1, goff=32)
         self._dgrad("enc1b", B["gcat1"], 48, B["g1"], 16, B["a1"], 16, d1, goff=32)
         self._wgrad("enc1a", B["g1"], 16, self._feat, 16, d1)

@@ -80,10 +80,11 @@ int efwi_nn_wgrad_noalloc(const uint16_t* dpre, int ldp, int mc, const uint16_t*
 int efwi_nn_chan_sum(const uint16_t* x, int ld, int c, long nvox, float* out, void* stream);
 
 /* 2x2x2 average pooling of a (nz, nx, ny) volume (even sizes) and its
- * gradient (accumulate: add into df instead of overwriting). */
+ * gradient (accumulate: add into df instead of overwriting; zeroed where
+ * mask <= 0 when mask is given, after the BF16 rounding). */
 int efwi_nn_pool2(const uint16_t* src, int lds, uint16_t* dst, int ldd, int c, int nz, int nx, int ny, void* stream);
 int efwi_nn_pool2_bwd(const uint16_t* dc, int ldc, uint16_t* df, int ldf, int c, int nz, int nx, int ny, int accumulate,
-                      void* stream);
+                      const uint16_t* mask, int ldm, void* stream);
 
 /* Nearest-neighbour upsampling x2 to a (nz, nx, ny) volume and its gradient
  * (sum over the 8 children, zeroed where mask <= 0 when mask is given). */

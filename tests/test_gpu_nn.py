@@ -185,8 +185,15 @@ def test_pool_up_add_granule_and_scalar_paths(c, off):
     gwide[:, off:off + c] = gc.reshape(-1, c)
     gcd = _dev(gwide)
     acc = _dev(wide)
-    nn_check(lib.efwi_nn_pool2_bwd(_p(gcd, off), ld, _p(acc, off), ld, c, nz, nx, ny, 1, _s()), "pool bwd")
+    nn_check(lib.efwi_nn_pool2_bwd(_p(gcd, off), ld, _p(acc, off), ld, c, nz, nx, ny, 1, None, 0, _s()), "pool bwd")
     assert _ulps_ok(_host(acc)[:, off:off + c], (x + O.pool2_bwd(gc)).reshape(V, c)).all()
+    # with the ReLU mask of x: the same rounded sums, zero where x <= 0
+    accm = _dev(wide)
+    nn_check(lib.efwi_nn_pool2_bwd(_p(gcd, off), ld, _p(accm, off), ld, c, nz, nx, ny, 1, _p(xd, off), ld, _s()),
+             "pool bwd masked")
+    hm = _host(accm)
+    assert np.array_equal(hm[:, off:off + c], np.where(wide[:, off:off + c] > 0, _host(acc)[:, off:off + c], 0.0))
+    assert np.array_equal(hm[:, :off], wide[:, :off]) and np.array_equal(hm[:, off + c:], wide[:, off + c:])
     up = torch.zeros(V, ld, dtype=torch.bfloat16, device=DEV)
     nn_check(lib.efwi_nn_up2(_p(gcd, off), ld, _p(up, off), ld, c, nz, nx, ny, _s()), "up")
     assert np.array_equal(_host(up)[:, off:off + c], O.up2(gc).reshape(V, c))
@@ -216,7 +223,7 @@ def test_pool_up_add_head_loss_adam_pack():
     acc = _bf(rng, nz, nx, ny, c)
     accd = _dev(acc.reshape(V, c))
     acc0 = _dev(acc.reshape(V, c))
-    nn_check(lib.efwi_nn_pool2_bwd(_p(gcd), c, _p(accd), c, c, nz, nx, ny, 1, _s()), "pool bwd")
+    nn_check(lib.efwi_nn_pool2_bwd(_p(gcd), c, _p(accd), c, c, nz, nx, ny, 1, None, 0, _s()), "pool bwd")
     assert _ulps_ok(_host(accd), (acc + O.pool2_bwd(gc)).reshape(V, c)).all()
     # upsampling and its gradient with a mask
     up = torch.zeros(V, c, dtype=torch.bfloat16, device=DEV)
