@@ -346,12 +346,14 @@ __global__ void k_flip_weights(const uint16_t* __restrict__ w, int kc, int nc, u
 
 // Weight gradient partials on v_mfma_f32_16x16x32_bf16: K = voxels. A block
 // of four wavefronts takes `per` bricks of 2 (z) x 2 (x) rows of TY voxels
-// along y (the convolution's bricks), one 16-channel block of m and one of n.
-// Per brick it copies into LDS, channel-major (y contiguous, rows padded to TY
-// + 8 so that each 8-lane phase of a fragment read hits distinct banks; 8-byte
-// stores of 4 voxels per channel): dpre of the
-// brick (zero outside the volume) and the brick's src halo (4 x 4 rows of TY
-// + 2 voxels). Wavefront w takes output row w and every tap: per 32 voxels it
+// along y, consecutive along z (bricks are numbered z fastest), one 16-channel
+// block of m and one of n. Per brick it copies into LDS, channel-major (y
+// contiguous, rows padded to TY + 8 so that each 8-lane phase of a fragment
+// read hits distinct banks; 8-byte stores of 4 voxels per channel): dpre of the
+// brick (zero outside the volume) and the brick's src halo (4 planes x 4 rows
+// of TY + 2 voxels) in a ring of four plane slots (plane z in slot z mod 4):
+// a brick that follows its z neighbour in the same block keeps that
+// neighbour's last two halo planes and copies only two new ones. Wavefront w takes output row w and every tap: per 32 voxels it
 // reads one A fragment (A[m][k] = dpre[voxel k][m]) and, per (dz, dx), two
 // 16-byte granules of the halo row from which the three dy shifts of the B
 // fragment (B[k][n] = src[voxel k + d_t][n]) follow by dword selection and
@@ -397,10 +399,13 @@ __global__ __launch_bounds__(256) void k_wgrad(const uint16_t* __restrict__ dpre
 #pragma unroll
   for (int t = 0; t < 27; t++) acc[t] = f4{0.f, 0.f, 0.f, 0.f};
   for (long brick = b0; brick < b1; brick++) {
-    const int yt = (int)(brick % nty);
-    const long r2 = brick / nty;
-    const int xt = (int)(r2 % ntx), zt = (int)(r2 / ntx);
+    const int zt = (int)(brick % ntz);
+    const long r2 = brick / ntz;
+    const int xt = (int)(r2 % ntx), yt = (int)(r2 / ntx);
     const int z0 = 2 * zt, x0 = 2 * xt, y0 = yt * TY;
+    // planes z0 - 1 and z0 are in the ring when this block's previous brick
+    // was the one below in the same column
+    const int hz0 = brick > b0 && zt > 0 ? 2 : 0;
     // channel-major copies, 4 voxels along y per item: each channel's 4 values
     // go out as one 8-byte LDS store (consecutive lanes take consecutive
     // voxel quads of one channel row). Threads keep their (granule, quad,
@@ -432,6 +437,7 @@ __global__ __launch_bounds__(256) void k_wgrad(const uint16_t* __restrict__ dpre
       const long plane = (long)V.nx * V.ny;
 #pragma unroll
       for (int hz = 0; hz < 4; hz++) {
+        if (hz < hz0) continue;
         const int zs = z0 - 1 + hz;
         const bool zx_ok = x_ok && zs >= 0 && zs < V.nz;
         const uint16_t* p = src + (v0 + hz * plane) * lds + nb * 16 + 8 * gr;
@@ -441,7 +447,7 @@ __global__ __launch_bounds__(256) void k_wgrad(const uint16_t* __restrict__ dpre
           q[k] = uint4{0u, 0u, 0u, 0u};
           if (interior || (zx_ok && ys + k >= 0 && ys + k < V.ny)) q[k] = ld16(p + (long)k * lds);
         }
-        store_quads(Bh + ((hz * 4 + hx) * 16 + 8 * gr) * P + 4 * y4, P, q);
+        store_quads(Bh + (((zs & 3) * 4 + hx) * 16 + 8 * gr) * P + 4 * y4, P, q);
       }
     }
     __syncthreads();
@@ -455,7 +461,7 @@ __global__ __launch_bounds__(256) void k_wgrad(const uint16_t* __restrict__ dpre
       }
 #pragma unroll
       for (int dzdx = 0; dzdx < 9; dzdx++) {
-        const int hr = (wz + dzdx / 3) * 4 + (wx + dzdx % 3);
+        const int hr = ((z0 - 1 + wz + dzdx / 3) & 3) * 4 + (wx + dzdx % 3);
         const uint16_t* bp = &Bh[(hr * 16 + i) * P + 32 * c + 8 * g];
         const uint4 g0 = *reinterpret_cast<const uint4*>(bp);
         const uint4 g1 = *reinterpret_cast<const uint4*>(bp + 8);
