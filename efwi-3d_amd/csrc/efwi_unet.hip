@@ -344,6 +344,26 @@ __global__ void k_flip_weights(const uint16_t* __restrict__ w, int kc, int nc, u
   }
 }
 
+// The same for up to kFlipMax layers in one launch (blockIdx.y = layer)
+constexpr int kFlipMax = 16;
+struct FlipSet {
+  const uint16_t* w[kFlipMax];
+  uint16_t* wt[kFlipMax];
+  int kc[kFlipMax], nc[kFlipMax];
+};
+__global__ void k_flip_many(FlipSet S) {
+  const int l = blockIdx.y, kc = S.kc[l], nc = S.nc[l];
+  const uint16_t* __restrict__ w = S.w[l];
+  uint16_t* __restrict__ wt = S.wt[l];
+  const long n_all = 27L * kc * nc;
+  for (long e = (long)blockIdx.x * blockDim.x + threadIdx.x; e < n_all; e += (long)gridDim.x * blockDim.x) {
+    const int k = (int)(e % kc);
+    const long r = e / kc;
+    const int n = (int)(r % nc), t = (int)(r / nc);
+    wt[((long)(26 - t) * kc + k) * nc + n] = w[e];
+  }
+}
+
 // Weight gradient partials on v_mfma_f32_16x16x32_bf16: K = voxels. A block
 // of four wavefronts takes `per` bricks of 2 (z) x 2 (x) rows of TY voxels
 // along y, consecutive along z (bricks are numbered z fastest), one 16-channel
@@ -1070,6 +1090,24 @@ int efwi_nn_flip_weights(const uint16_t* w, int kc, int nc, uint16_t* wt, void* 
   if (!w || !wt || kc <= 0 || nc <= 0 || w == wt) return EFWI_NN_ERR_ARG;
   hipStream_t s = (hipStream_t)stream;
   hipLaunchKernelGGL(k_flip_weights, dim3(grid_for(27L * kc * nc, 256)), dim3(256), 0, s, w, kc, nc, wt);
+  return nn_check(s);
+}
+
+int efwi_nn_flip_weights_n(int n, const uint16_t* const* w, const int* kc, const int* nc, uint16_t* const* wt,
+                           void* stream) {
+  if (n < 1 || n > kFlipMax || !w || !kc || !nc || !wt) return EFWI_NN_ERR_ARG;
+  FlipSet S{};
+  long most = 0;
+  for (int l = 0; l < n; l++) {
+    if (!w[l] || !wt[l] || kc[l] <= 0 || nc[l] <= 0 || w[l] == wt[l]) return EFWI_NN_ERR_ARG;
+    S.w[l] = w[l];
+    S.wt[l] = wt[l];
+    S.kc[l] = kc[l];
+    S.nc[l] = nc[l];
+    most = std::max(most, 27L * kc[l] * nc[l]);
+  }
+  hipStream_t s = (hipStream_t)stream;
+  hipLaunchKernelGGL(k_flip_many, dim3(grid_for(most, 256), (unsigned)n), dim3(256), 0, s, S);
   return nn_check(s);
 }
 

@@ -84,6 +84,7 @@ SIGNATURES = {
     "efwi_nn_conv3": (C.c_int, [_vp, C.c_int, C.c_int, _vp, _vp, _vp, C.c_int, C.c_int, _vp, C.c_int,
                                 C.c_int, C.c_int, C.c_int, C.c_int, C.c_int, _vp]),
     "efwi_nn_flip_weights": (C.c_int, [_vp, C.c_int, C.c_int, _vp, _vp]),
+    "efwi_nn_flip_weights_n": (C.c_int, [C.c_int, _vp, _vp, _vp, _vp, _vp]),
     "efwi_nn_wgrad_scratch": (C.c_long, [C.c_int, C.c_int, C.c_int, C.c_int, C.c_int, C.c_int]),
     "efwi_nn_wgrad": (C.c_int, [_vp, C.c_int, C.c_int, _vp, C.c_int, C.c_int, _vp, C.c_long, _vp, _vp,
                                 C.c_int, C.c_int, C.c_int, C.c_int, _vp]),

@@ -130,10 +130,12 @@ class UNet3D:
     def _flip(self):
         """The input-gradient weights (taps mirrored, channels transposed)
         from the BF16 weights, for the mode-0 convolution of the backward."""
-        s = self._stream()
-        for name, cin, cout, _ in LAYERS:
-            nn_check(self.lib.efwi_nn_flip_weights(_p(self.w16[name]), cin, cout, _p(self.wt16[name]), s),
-                     f"flip {name}")
+        n = len(LAYERS)
+        w = (C.c_void_p * n)(*[self.w16[name].data_ptr() for name, *_ in LAYERS])
+        wt = (C.c_void_p * n)(*[self.wt16[name].data_ptr() for name, *_ in LAYERS])
+        kc = (C.c_int * n)(*[cin for _, cin, _, _ in LAYERS])
+        nc = (C.c_int * n)(*[cout for _, _, cout, _ in LAYERS])
+        nn_check(self.lib.efwi_nn_flip_weights_n(n, w, kc, nc, wt, self._stream()), "flip")
 
     def state_dict(self):
         return {k: v.detach().cpu().clone() for k, v in self._params().items()}

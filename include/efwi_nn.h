@@ -48,6 +48,11 @@ int efwi_nn_conv3(const uint16_t* src, int lds, int kc, const uint16_t* w, const
  * alias w. */
 int efwi_nn_flip_weights(const uint16_t* w, int kc, int nc, uint16_t* wt, void* stream);
 
+/* efwi_nn_flip_weights for n layers (1 <= n <= 16) in one launch: host arrays
+ * of n weight pointers, channel counts and output pointers. */
+int efwi_nn_flip_weights_n(int n, const uint16_t* const* w, const int* kc, const int* nc, uint16_t* const* wt,
+                           void* stream);
+
 /* Weight gradient of a 3x3x3 convolution, dw[t][m][n] = sum_v dpre[v][m]
  * src[v + d_t][n] (FP32 result), and, when db is not NULL, its bias gradient
  * db[m] = sum_v dpre[v][m]; dpre = gradient of the layer's pre-activation (mc

@@ -107,6 +107,30 @@ def test_conv3_forward_and_input_gradient(dims, kc, nc, relu):
     assert np.array_equal(got2[:, 4:4 + kc], got[:, 16:])
 
 
+def test_flip_weights_of_several_layers_in_one_launch():
+    import ctypes as C
+    from efwi import _native as N
+    from efwi.unet import nn_check
+    rng = np.random.default_rng(3)
+    shapes = [(16, 16), (48, 16), (32, 96)]
+    ws = [rng.standard_normal((27, nc, kc)).astype(np.float32) for kc, nc in shapes]
+    wd = [torch.from_numpy(w).to(DEV).to(torch.bfloat16) for w in ws]
+    wt = [torch.full_like(w, 5.0) for w in wd]
+    n = len(shapes)
+    arr = lambda ts: (C.c_void_p * n)(*[t.data_ptr() for t in ts])
+    kcs = (C.c_int * n)(*[kc for kc, _ in shapes])
+    ncs = (C.c_int * n)(*[nc for _, nc in shapes])
+    print("[gpu] flip_weights_n 3 layers", flush=True)
+    nn_check(N.lib().efwi_nn_flip_weights_n(n, arr(wd), kcs, ncs, arr(wt), _s()), "flip n")
+    for (kc, nc), w, t in zip(shapes, wd, wt):
+        ref = _host(w).reshape(27, nc, kc)[::-1].transpose(0, 2, 1)
+        assert np.array_equal(_host(t).reshape(27, kc, nc), ref)
+    # counts outside 1..16 and aliasing are refused before any launch
+    assert N.lib().efwi_nn_flip_weights_n(0, arr(wd), kcs, ncs, arr(wt), _s()) == 1
+    assert N.lib().efwi_nn_flip_weights_n(17, arr(wd), kcs, ncs, arr(wt), _s()) == 1
+    assert N.lib().efwi_nn_flip_weights_n(n, arr(wd), kcs, ncs, arr(wd), _s()) == 1
+
+
 @pytest.mark.parametrize("dims,mc,kc", [((6, 10, 37), 16, 16), ((4, 8, 16), 64, 32), ((4, 4, 20), 32, 96)])
 def test_conv3_weight_gradient(dims, mc, kc):
     from efwi import _native as N
