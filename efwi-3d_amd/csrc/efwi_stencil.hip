@@ -916,9 +916,11 @@ __device__ __forceinline__ void march(const Sweep& Sw, const Geom& G, const Grou
         tr = hr < 2 ? hr : R + hr;
         tc = h - hr * kTileW;
       } else {
+        // side halo of the R own rows: columns 0, 1 (y - 2, y - 1 of lane 0)
+        // and 66, 67 (y + 1, y + 2 of lane 63)
         const int e = h - 4 * kTileW;
         tr = 2 + (e >> 2);
-        tc = (e & 3) < 2 ? (e & 3) : 62 + (e & 3);
+        tc = (e & 3) < 2 ? (e & 3) : 64 + (e & 3);
       }
       const int hx = min(max(x_base - 2 + tr, 0), G.nx - 1);
       const int hy = min(max(y_base - 2 + tc, 0), G.ny - 1);

@@ -208,6 +208,26 @@ def test_tile_shapes_are_bit_identical(orc, small_case, rows, zdepth):
         assert np.array_equal(a, b)
 
 
+@pytest.mark.parametrize("rows,zdepth", [(4, 16), (8, 16), (2, 40)])
+def test_marching_shapes_on_several_y_segments(orc, rows, zdepth):
+    """The z-marching kernels on a grid 212 cells long in y (four 64-cell
+    segments: every segment edge needs its neighbour segment's cells from the
+    LDS tile's side halo), forward and gradient against the oracle bit for
+    bit."""
+    from efwi import synth
+    w = synth.small(n=20, npml=6, nt=120, ny=200)
+    obs = [s.astype(np.float32) for s in oracle_run(orc, w, model=w.true, calc_id=2).syn]
+    ref = oracle_run(orc, w, obs=obs, calc_id=1)
+    t = {k: (rows, zdepth) for k in ("stress_fwd", "velocity_fwd", "stress_bwd", "velocity_bwd")}
+    print(f"[gpu] marching shapes {rows}x{zdepth} on a 212-cell y axis", flush=True)
+    j, syn, res, grads = _gpu_run(w, w.model, 1, obs, tiles=t)
+    assert np.array_equal(syn[0], ref.syn[0].astype(np.float32))
+    assert abs(j - ref.misfit64) <= 1e-12 * abs(ref.misfit64)
+    for g, c in zip(grads, (ref.glam, ref.gmu, ref.grho)):
+        assert np.abs(c).max() > 0
+        assert np.array_equal(g, c.astype(np.float32))
+
+
 def test_autotuner_picks_a_candidate(small_case):
     import torch
     from efwi import synth
