@@ -881,7 +881,7 @@ __global__ void k_head16(const uint16_t* __restrict__ act, int ld, const float* 
 __global__ void k_head_bwd16(const uint16_t* __restrict__ act, int ld, const float* __restrict__ w,
                              const float* __restrict__ dout, uint16_t* __restrict__ dact, int ldd, float* __restrict__ dw,
                              float* __restrict__ db, long nvox) {
-  __shared__ float red[17][64];
+  __shared__ float red[17][256];
   float acc[17];
   for (int k = 0; k < 17; k++) acc[k] = 0.f;
   float wr[16];
@@ -902,12 +902,29 @@ __global__ void k_head_bwd16(const uint16_t* __restrict__ act, int ld, const flo
     }
     acc[16] += g;
   }
-  // blockDim.x == 64
+  // blockDim.x == 256: per sum, 16 partial sums of 16 threads, then their total
   for (int k = 0; k < 17; k++) red[k][threadIdx.x] = acc[k];
+  __syncthreads();
+  // 17 x 16 partial sums over 256 threads: item e = threadIdx.x + 256 r
+  float part[2] = {0.f, 0.f};
+#pragma unroll
+  for (int r = 0; r < 2; r++) {
+    const int e = (int)threadIdx.x + 256 * r;
+    if (e < 17 * 16) {
+      const int k = e >> 4, q0 = (e & 15) * 16;
+      for (int q = q0; q < q0 + 16; q++) part[r] += red[k][q];
+    }
+  }
+  __syncthreads();
+#pragma unroll
+  for (int r = 0; r < 2; r++) {
+    const int e = (int)threadIdx.x + 256 * r;
+    if (e < 17 * 16) red[e >> 4][e & 15] = part[r];
+  }
   __syncthreads();
   if (threadIdx.x < 17) {
     float t = 0.f;
-    for (int q = 0; q < 64; q++) t += red[threadIdx.x][q];
+    for (int q = 0; q < 16; q++) t += red[threadIdx.x][q];
     if (threadIdx.x < 16) atomicAdd(dw + threadIdx.x, t);
     if (threadIdx.x == 16) atomicAdd(db, t);
   }
@@ -1286,7 +1303,7 @@ int efwi_nn_head_bwd(const uint16_t* act, int ld, int c, const float* w, const f
   hipLaunchKernelGGL(k_zero, dim3(1), dim3(64), 0, s, reinterpret_cast<unsigned*>(dw), c);
   hipLaunchKernelGGL(k_zero, dim3(1), dim3(64), 0, s, reinterpret_cast<unsigned*>(db), 1);
   if (c == 16 && granules_ok(16, act, ld) && granules_ok(16, dact, ldd))
-    hipLaunchKernelGGL(k_head_bwd16, dim3(grid_for(nvox, 64, 2048)), dim3(64), 0, s, act, ld, w, dout, dact, ldd, dw, db,
+    hipLaunchKernelGGL(k_head_bwd16, dim3(grid_for(nvox, 256, 2048)), dim3(256), 0, s, act, ld, w, dout, dact, ldd, dw, db,
                        nvox);
   else
     hipLaunchKernelGGL(k_head_bwd, dim3(grid_for(nvox, 64, 2048)), dim3(64), 0, s, act, ld, c, w, dout, dact, ldd, dw, db,
