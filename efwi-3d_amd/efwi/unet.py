@@ -257,14 +257,21 @@ class UNet3D:
         self._wgrad("dec1b", B["g1b"], 16, B["d1a"], 16, d1)
         self._dgrad("dec1b", B["g1b"], 16, B["g1"], 16, B["d1a"], 16, d1)
         self._wgrad("dec1a", B["g1"], 16, B["cat1"], 48, d1)
-        self._dgrad("dec1a", B["g1"], 16, B["gcat1"], 48, B["cat1"], 48, d1)
-        nn_check(self.lib.efwi_nn_up2_bwd(_p(B["gcat1"]), 48, _p(B["g2b"]), 32, 32, *d1, None, 0, s), "up1 bwd")
+        # the ReLU derivative of cat1 is applied where each part is consumed, not
+        # here: the upsampled part's (that of d2b, shared by a parent's 8
+        # children) inside the upsampling gradient, the skip part's after the
+        # pooled gradient is added to it; either way a masked value is exactly
+        # zero and a kept one the same sum
+        self._dgrad("dec1a", B["g1"], 16, B["gcat1"], 48, None, 0, d1)
+        nn_check(self.lib.efwi_nn_up2_bwd(_p(B["gcat1"]), 48, _p(B["g2b"]), 32, 32, *d1, _p(B["d2b"]), 32, s),
+                 "up1 bwd")
         # level 2 decoder
         self._wgrad("dec2b", B["g2b"], 32, B["d2a"], 32, d2)
         self._dgrad("dec2b", B["g2b"], 32, B["g2"], 32, B["d2a"], 32, d2)
         self._wgrad("dec2a", B["g2"], 32, B["cat2"], 96, d2)
-        self._dgrad("dec2a", B["g2"], 32, B["gcat2"], 96, B["cat2"], 96, d2)
-        nn_check(self.lib.efwi_nn_up2_bwd(_p(B["gcat2"]), 96, _p(B["g3b"]), 64, 64, *d2, None, 0, s), "up2 bwd")
+        self._dgrad("dec2a", B["g2"], 32, B["gcat2"], 96, None, 0, d2)
+        nn_check(self.lib.efwi_nn_up2_bwd(_p(B["gcat2"]), 96, _p(B["g3b"]), 64, 64, *d2, _p(B["b3"]), 64, s),
+                 "up2 bwd")
         # bottleneck
         self._wgrad("bot_b", B["g3b"], 64, B["a3"], 64, d3)
         self._dgrad("bot_b", B["g3b"], 64, B["g3"], 64, B["a3"], 64, d3)
