@@ -121,7 +121,9 @@ __global__ __launch_bounds__(512) void k_conv3(const uint16_t* __restrict__ src,
                                                const uint16_t* __restrict__ w, const float* __restrict__ bias,
                                                uint16_t* __restrict__ dst, int ldd, int nc,
                                                const uint16_t* __restrict__ mask, int ldm, Vol V, int relu,
-                                               int ntz, int ntx, int nty, int per_xcd, int vec_out) {
+                                               int ntz, int ntx, int nty, int per_xcd, int vec_out,
+                                               const uint16_t* __restrict__ src2, int lds2, int kc1,
+                                               uint16_t* __restrict__ dst2, int ldd2, int nc1) {
   constexpr int TY = 16 * MB, HW = TY + 2, G = CK / 8;
   constexpr int TZ = conv_tz(MB), NT = 128 * TZ, HR = (TZ + 2) * 4;  // halo rows
   constexpr int NQ = (27 * CK + 31) / 32;
@@ -162,6 +164,12 @@ __global__ __launch_bounds__(512) void k_conv3(const uint16_t* __restrict__ src,
 #pragma unroll
     for (int c = 0; c < NB; c++) acc[m][c] = f4{0.f, 0.f, 0.f, 0.f};
   for (int c0 = 0; c0 < kc; c0 += CK) {
+    // input channels [0, kc1) come from src, [kc1, kc) from src2 (a chunk
+    // never straddles the two)
+    const bool second = c0 >= kc1;
+    const uint16_t* const sb = second ? src2 : src;
+    const long sl = second ? lds2 : lds;
+    const int cs = second ? c0 - kc1 : c0;
     // halo rows r = 4 hz + hx at (z0 - 1 + hz, x0 - 1 + hx), voxels hv at
     // y0 - 1 + hv. An item is one (hv, granule) pair over 8 rows (two z
     // planes; row offsets uniform), its eight loads in flight before the
@@ -174,7 +182,7 @@ __global__ __launch_bounds__(512) void k_conv3(const uint16_t* __restrict__ src,
         const int gr = pidx % G, hv = pidx / G;
         const int ys = y0 - 1 + hv;
         const bool y_ok = interior || (ys >= 0 && ys < V.ny);
-        const uint16_t* tp = src + (corner + (long)(2 * rg) * V.nx * V.ny + hv) * lds + c0 + 8 * gr;
+        const uint16_t* tp = sb + (corner + (long)(2 * rg) * V.nx * V.ny + hv) * sl + cs + 8 * gr;
         unsigned char* lp = halo + (8 * rg * HW + hv) * VB + 16 * (gr ^ granule_swizzle<CK>(hv));
         uint4 hq[8];
 #pragma unroll
@@ -183,7 +191,7 @@ __global__ __launch_bounds__(512) void k_conv3(const uint16_t* __restrict__ src,
           const bool ok = interior || (y_ok && (unsigned)(z0 - 1 + hz) < (unsigned)V.nz &&
                                        (unsigned)(x0 - 1 + hx) < (unsigned)V.nx);
           hq[rr] = uint4{0u, 0u, 0u, 0u};
-          if (ok) hq[rr] = *reinterpret_cast<const uint4*>(tp + ((long)(rr >> 2) * V.nx + hx) * V.ny * lds);
+          if (ok) hq[rr] = *reinterpret_cast<const uint4*>(tp + ((long)(rr >> 2) * V.nx + hx) * V.ny * sl);
         }
 #pragma unroll
         for (int rr = 0; rr < 8; rr++) *reinterpret_cast<uint4*>(lp + rr * HW * VB) = hq[rr];
@@ -307,7 +315,9 @@ __global__ __launch_bounds__(512) void k_conv3(const uint16_t* __restrict__ src,
           if (!(mk[j] > 0.f)) f[j] = 0.f;
         q = pack8(f);
       }
-      st16(dst + (vrow + vv) * ldd + nb0 * 16 + 8 * gg, q);
+      // output channels [0, nc1) go to dst, [nc1, nc) to dst2
+      const int ch = nb0 * 16 + 8 * gg;
+      st16(ch < nc1 ? dst + (vrow + vv) * ldd + ch : dst2 + (vrow + vv) * ldd2 + (ch - nc1), q);
     }
     return;
   }
@@ -316,7 +326,8 @@ __global__ __launch_bounds__(512) void k_conv3(const uint16_t* __restrict__ src,
   for (int c = 0; c < NB; c++) {
     const int n = (nb0 + c) * 16 + i;
     const float bn = bias ? bias[n] : 0.f;
-    uint16_t* out = dst + (vrow + 4 * g) * ldd + n;
+    const long od = n < nc1 ? ldd : ldd2;
+    uint16_t* out = n < nc1 ? dst + (vrow + 4 * g) * ldd + n : dst2 + (vrow + 4 * g) * ldd2 + (n - nc1);
     const uint16_t* mk = mask ? mask + (vrow + 4 * g) * ldm + n : nullptr;
 #pragma unroll
     for (int m = 0; m < MB; m++)
@@ -327,7 +338,7 @@ __global__ __launch_bounds__(512) void k_conv3(const uint16_t* __restrict__ src,
         float r = acc[m][c][jj] + bn;
         if (relu) r = fmaxf(r, 0.f);
         if (mk && !(bf2f(mk[(long)dyv * ldm]) > 0.f)) r = 0.f;
-        out[(long)dyv * ldd] = f2bf(r);
+        out[(long)dyv * od] = f2bf(r);
       }
   }
 }
@@ -399,7 +410,8 @@ template <int TY>
 __global__ __launch_bounds__(256) void k_wgrad(const uint16_t* __restrict__ dpre, int ldp, int mc,
                                                const uint16_t* __restrict__ src, int lds, int kc,
                                                float* __restrict__ part, float* __restrict__ dw,
-                                               float* __restrict__ db, Vol V, int ntz, int ntx, int nty, int per) {
+                                               float* __restrict__ db, Vol V, int ntz, int ntx, int nty, int per,
+                                               const uint16_t* __restrict__ src2, int lds2, int kc1) {
   constexpr int HW = TY + 2, P = TY + 8, NK = TY / 32;
   constexpr int A_BYTES = 4 * 16 * P * 2, B_BYTES = 16 * 16 * P * 2, R_BYTES = 9 * 4 * 256 * 4;
   constexpr int S_BYTES = A_BYTES + (B_BYTES > R_BYTES ? B_BYTES : R_BYTES);
@@ -414,6 +426,11 @@ __global__ __launch_bounds__(256) void k_wgrad(const uint16_t* __restrict__ dpre
   const long nbricks = (long)ntz * ntx * nty;
   const long b0 = (long)blockIdx.x * per, b1 = std::min(nbricks, b0 + per);
   const bool bias = db != nullptr && nb == 0;
+  // this block's 16 src channels: from src below kc1, else from src2
+  const bool second = nb * 16 >= kc1;
+  const uint16_t* const sb = second ? src2 : src;
+  const long sl = second ? lds2 : lds;
+  const int scol = second ? nb * 16 - kc1 : nb * 16;
   float bsum = 0.f;  // lane (i, g): dpre channel mb * 16 + i over its voxels
   f4 acc[27];
 #pragma unroll
@@ -460,12 +477,12 @@ __global__ __launch_bounds__(256) void k_wgrad(const uint16_t* __restrict__ dpre
         if (hz < hz0) continue;
         const int zs = z0 - 1 + hz;
         const bool zx_ok = x_ok && zs >= 0 && zs < V.nz;
-        const uint16_t* p = src + (v0 + hz * plane) * lds + nb * 16 + 8 * gr;
+        const uint16_t* p = sb + (v0 + hz * plane) * sl + scol + 8 * gr;
         uint4 q[4];
 #pragma unroll
         for (int k = 0; k < 4; k++) {
           q[k] = uint4{0u, 0u, 0u, 0u};
-          if (interior || (zx_ok && ys + k >= 0 && ys + k < V.ny)) q[k] = ld16(p + (long)k * lds);
+          if (interior || (zx_ok && ys + k >= 0 && ys + k < V.ny)) q[k] = ld16(p + (long)k * sl);
         }
         store_quads(Bh + (((zs & 3) * 4 + hx) * 16 + 8 * gr) * P + 4 * y4, P, q);
       }
@@ -1054,19 +1071,27 @@ static int nn_check(hipStream_t s) {
   return hipGetLastError() == hipSuccess ? 0 : EFWI_NN_ERR_HIP;
 }
 
-int efwi_nn_conv3(const uint16_t* src, int lds, int kc, const uint16_t* w, const float* bias, uint16_t* dst, int ldd,
-                  int nc, const uint16_t* mask, int ldm, int nz, int nx, int ny, int mode, int relu, void* stream) {
-  if (!src || !w || !dst || kc <= 0 || nc <= 0 || kc % 16 || nc % 16 || lds % 16 || ldd % 16 || lds < kc ||
-      ldd < nc || (mask && (ldm % 16 || ldm < nc)) || nz <= 0 || nx <= 0 || ny <= 0 || (mode != 0 && mode != 1) ||
+int efwi_nn_conv3_cat(const uint16_t* src, int lds, int kc1, const uint16_t* src2, int lds2, int kc2,
+                      const uint16_t* w, const float* bias, uint16_t* dst, int ldd, int nc1, uint16_t* dst2, int ldd2,
+                      int nc2, const uint16_t* mask, int ldm, int nz, int nx, int ny, int mode, int relu,
+                      void* stream) {
+  if (!src2) kc2 = 0;
+  if (!dst2) nc2 = 0;
+  const int kc = kc1 + kc2, nc = nc1 + nc2;
+  if (!src || !w || !dst || kc1 <= 0 || nc1 <= 0 || kc1 % 16 || nc1 % 16 || lds % 16 || ldd % 16 || lds < kc1 ||
+      ldd < nc1 || (mask && (ldm % 16 || ldm < nc)) || nz <= 0 || nx <= 0 || ny <= 0 || (mode != 0 && mode != 1) ||
       ((uintptr_t)src | (uintptr_t)w) % 16)
     return EFWI_NN_ERR_ARG;
+  if (src2 && (kc2 <= 0 || kc2 % 16 || lds2 % 16 || lds2 < kc2 || (uintptr_t)src2 % 16)) return EFWI_NN_ERR_ARG;
+  if (dst2 && (nc2 <= 0 || nc2 % 16 || ldd2 % 16 || ldd2 < nc2)) return EFWI_NN_ERR_ARG;
   hipStream_t s = (hipStream_t)stream;
   const Vol V{nz, nx, ny};
   // bricks of 2 x 2 rows of 64, 32 or 16 voxels along y; 1 to 4 output blocks
   // per wavefront (a divisor of the block count, so no block is partial)
   const int MB = ny >= 64 ? 4 : ny >= 32 ? 2 : 1;
   const int nbt = nc / 16;
-  const bool ck32 = kc % 32 == 0;
+  // 32-channel chunks where no chunk would straddle the two sources
+  const bool ck32 = kc % 32 == 0 && kc1 % 32 == 0;
   const int TZ = conv_tz(MB);
   const int ntz = (nz + TZ - 1) / TZ, ntx = (nx + 1) / 2, nty = (ny + 16 * MB - 1) / (16 * MB);
   const long bricks = (long)ntz * ntx * nty;
@@ -1078,10 +1103,11 @@ int efwi_nn_conv3(const uint16_t* src, int lds, int kc, const uint16_t* w, const
   const int per_xcd = (int)((bricks + 7) / 8);
   const dim3 grid((unsigned)(8L * per_xcd), (unsigned)(nbt / NB));
   // outputs (and mask) as 16-byte granules where the pointers allow
-  const int vec_out = (uintptr_t)dst % 16 == 0 && (!mask || (uintptr_t)mask % 16 == 0);
+  const int vec_out = (uintptr_t)dst % 16 == 0 && (uintptr_t)dst2 % 16 == 0 && (!mask || (uintptr_t)mask % 16 == 0);
 #define EFWI_NN_CONV_LAUNCH(CK, M, B, MODE)                                                                       \
   hipLaunchKernelGGL((k_conv3<CK, M, B, MODE>), grid, dim3(128 * conv_tz(M)), 0, s, src, lds, kc, w, bias, dst,  \
-                     ldd, nc, mask, ldm, V, relu, ntz, ntx, nty, per_xcd, vec_out)
+                     ldd, nc, mask, ldm, V, relu, ntz, ntx, nty, per_xcd, vec_out, src2, lds2, kc1, dst2, ldd2,   \
+                     nc1)
 #define EFWI_NN_CONV_MODE(CK, M, B)                         \
   if (mode == 0) { EFWI_NN_CONV_LAUNCH(CK, M, B, 0); }    \
   else { EFWI_NN_CONV_LAUNCH(CK, M, B, 1); }
@@ -1101,6 +1127,12 @@ int efwi_nn_conv3(const uint16_t* src, int lds, int kc, const uint16_t* w, const
 #undef EFWI_NN_CONV_MODE
 #undef EFWI_NN_CONV_LAUNCH
   return nn_check(s);
+}
+
+int efwi_nn_conv3(const uint16_t* src, int lds, int kc, const uint16_t* w, const float* bias, uint16_t* dst, int ldd,
+                  int nc, const uint16_t* mask, int ldm, int nz, int nx, int ny, int mode, int relu, void* stream) {
+  return efwi_nn_conv3_cat(src, lds, kc, nullptr, 0, 0, w, bias, dst, ldd, nc, nullptr, 0, 0, mask, ldm, nz, nx, ny,
+                           mode, relu, stream);
 }
 
 int efwi_nn_flip_weights(const uint16_t* w, int kc, int nc, uint16_t* wt, void* stream) {
@@ -1149,13 +1181,27 @@ static bool wgrad_args_ok(const uint16_t* dpre, int ldp, int mc, const uint16_t*
          ((uintptr_t)dpre | (uintptr_t)src) % 16 == 0;
 }
 static int wgrad_launch(const uint16_t* dpre, int ldp, int mc, const uint16_t* src, int lds, int kc, float* scratch,
-                        float* dw, float* db, int nz, int nx, int ny, int per, hipStream_t s);
+                        float* dw, float* db, int nz, int nx, int ny, int per, hipStream_t s,
+                        const uint16_t* src2 = nullptr, int lds2 = 0, int kc1 = 0);
 
 int efwi_nn_wgrad(const uint16_t* dpre, int ldp, int mc, const uint16_t* src, int lds, int kc, float* scratch,
                   long scratch_floats, float* dw, float* db, int nz, int nx, int ny, int per, void* stream) {
   if (!scratch || !wgrad_args_ok(dpre, ldp, mc, src, lds, kc, dw, nz, nx, ny, per)) return EFWI_NN_ERR_ARG;
   if (efwi_nn_wgrad_scratch(mc, kc, nz, nx, ny, per) > scratch_floats) return EFWI_NN_ERR_ARG;
   return wgrad_launch(dpre, ldp, mc, src, lds, kc, scratch, dw, db, nz, nx, ny, per, (hipStream_t)stream);
+}
+
+int efwi_nn_wgrad_cat(const uint16_t* dpre, int ldp, int mc, const uint16_t* src, int lds, int kc1, const uint16_t* src2,
+                      int lds2, int kc2, float* scratch, long scratch_floats, float* dw, float* db, int nz, int nx,
+                      int ny, int per, void* stream) {
+  const int kc = kc1 + kc2;
+  if (!dpre || !src || !src2 || !dw || !scratch || mc <= 0 || mc % 16 || kc1 <= 0 || kc1 % 16 || kc2 <= 0 ||
+      kc2 % 16 || ldp % 16 || ldp < mc || lds % 16 || lds < kc1 || lds2 % 16 || lds2 < kc2 || per <= 0 || nz <= 0 ||
+      nx <= 0 || ny <= 0 || ((uintptr_t)dpre | (uintptr_t)src | (uintptr_t)src2) % 16)
+    return EFWI_NN_ERR_ARG;
+  if (efwi_nn_wgrad_scratch(mc, kc, nz, nx, ny, per) > scratch_floats) return EFWI_NN_ERR_ARG;
+  return wgrad_launch(dpre, ldp, mc, src, lds, kc, scratch, dw, db, nz, nx, ny, per, (hipStream_t)stream, src2, lds2,
+                      kc1);
 }
 
 // The partial sums of efwi_nn_wgrad_noalloc: one array per GPU, for the
@@ -1196,18 +1242,20 @@ int efwi_nn_wgrad_noalloc(const uint16_t* dpre, int ldp, int mc, const uint16_t*
 }
 
 static int wgrad_launch(const uint16_t* dpre, int ldp, int mc, const uint16_t* src, int lds, int kc, float* scratch,
-                        float* dw, float* db, int nz, int nx, int ny, int per, hipStream_t s) {
+                        float* dw, float* db, int nz, int nx, int ny, int per, hipStream_t s,
+                        const uint16_t* src2, int lds2, int kc1) {
   const Vol V{nz, nx, ny};
+  if (!src2) kc1 = kc;
   const int ty = wgrad_ty(ny);
   const int ntz = (nz + 1) / 2, ntx = (nx + 1) / 2, nty = (ny + ty - 1) / ty;
   const int chunks = (int)((wgrad_bricks(nz, nx, ny) + per - 1) / per);
   const dim3 grid((unsigned)chunks, (unsigned)(mc / 16), (unsigned)(kc / 16));
   if (ty == 64)
     hipLaunchKernelGGL((k_wgrad<64>), grid, dim3(256), 0, s, dpre, ldp, mc, src, lds, kc, scratch, dw, db, V, ntz, ntx,
-                       nty, per);
+                       nty, per, src2, lds2, kc1);
   else
     hipLaunchKernelGGL((k_wgrad<32>), grid, dim3(256), 0, s, dpre, ldp, mc, src, lds, kc, scratch, dw, db, V, ntz, ntx,
-                       nty, per);
+                       nty, per, src2, lds2, kc1);
   const long width = 27L * mc * kc, total = width + (db ? mc : 0);
   hipLaunchKernelGGL(k_sum_chunks, dim3(grid_for(total, 256, 1L << 30), (chunks + kChunkGroup - 1) / kChunkGroup),
                      dim3(256), 0, s, scratch, width + mc, width, total, chunks, dw, db);

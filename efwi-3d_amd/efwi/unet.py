@@ -8,19 +8,22 @@ counterpart: EFWI-3D has no learned part.
 Architecture (3 levels, 16 base channels, 3x3x3 convolutions with ReLU,
 2x2x2 average pooling, nearest upsampling, skip concatenation):
 
-    enc1a 16->16, enc1b 16->16 ------------------------------ cat1[32:48]
-      pool -> enc2a 16->32, enc2b 32->32 --------------- cat2[64:96]    |
-        pool -> bot_a 32->64, bot_b 64->64 -> up -> cat2[0:64]          |
-                  dec2a 96->32, dec2b 32->32 -> up -> cat1[0:32] -------+
-                                    dec1a 48->16, dec1b 16->16 -> head 16->1 (FP32)
+    enc1a 16->16, enc1b 16->16 = e1
+      pool -> enc2a 16->32, enc2b 32->32 = e2
+        pool -> bot_a 32->64, bot_b 64->64 -> up = u2
+                  dec2a (u2 | e2) 96->32, dec2b 32->32 -> up = u1
+                            dec1a (u1 | e1) 48->16, dec1b 16->16 -> head 16->1 (FP32)
     update = g exp(clamp(head, -4, 6))   (g = input channel 0: a learned,
                             spatially varying scaling of the gradient, as a
                             preconditioner is)
 
 Weights and activations are BF16 with FP32 accumulation; FP32 master weights
 and Adam state; activation gradients BF16, weight gradients FP32. The
-skip concatenations are channel slices written in place; the ReLU derivative
-of each activation is applied as a mask inside the input-gradient kernels.
+skip concatenations stay two tensors each (the upsampled part and the skip
+activation), which the decoder convolutions read as one input and whose input
+gradients they write as two outputs; the ReLU derivative of each activation is
+applied as a mask inside the input-gradient, pooling-gradient and
+upsampling-gradient kernels.
 """
 from __future__ import annotations
 
@@ -158,13 +161,16 @@ class UNet3D:
         v1 = nz * nx * ny
         v2, v3 = v1 // 8, v1 // 64
         bf = lambda n, c: torch.zeros(n, c, dtype=torch.bfloat16, device=self.dev)
-        B = {"a1": bf(v1, 16), "cat1": bf(v1, 48), "p1": bf(v2, 16), "a2": bf(v2, 32), "cat2": bf(v2, 96),
-             "p2": bf(v3, 32), "a3": bf(v3, 64), "b3": bf(v3, 64), "d2a": bf(v2, 32), "d2b": bf(v2, 32),
-             "d1a": bf(v1, 16), "d1b": bf(v1, 16), "o": torch.zeros(v1, device=self.dev),
-             "out": torch.zeros(v1, device=self.dev), "do": torch.zeros(v1, device=self.dev),
+        # u1 | e1 and u2 | e2: the two parts of the skip concatenations
+        B = {"a1": bf(v1, 16), "e1": bf(v1, 16), "u1": bf(v1, 32), "p1": bf(v2, 16), "a2": bf(v2, 32),
+             "e2": bf(v2, 32), "u2": bf(v2, 64), "p2": bf(v3, 32), "a3": bf(v3, 64), "b3": bf(v3, 64),
+             "d2a": bf(v2, 32), "d2b": bf(v2, 32), "d1a": bf(v1, 16), "d1b": bf(v1, 16),
+             "o": torch.zeros(v1, device=self.dev), "out": torch.zeros(v1, device=self.dev),
+             "do": torch.zeros(v1, device=self.dev),
              # gradients
-             "g1": bf(v1, 16), "g1b": bf(v1, 16), "gcat1": bf(v1, 48), "gcat2": bf(v2, 96), "g2": bf(v2, 32),
-             "g2b": bf(v2, 32), "gp1": bf(v2, 16), "gp2": bf(v3, 32), "g3": bf(v3, 64), "g3b": bf(v3, 64),
+             "g1": bf(v1, 16), "g1b": bf(v1, 16), "gu1": bf(v1, 32), "ge1": bf(v1, 16), "gu2": bf(v2, 64),
+             "ge2": bf(v2, 32), "g2": bf(v2, 32), "g2b": bf(v2, 32), "gp1": bf(v2, 16), "gp2": bf(v3, 32),
+             "g3": bf(v3, 64), "g3b": bf(v3, 64),
              "dout": torch.zeros(v1, device=self.dev), "loss": torch.zeros(1, dtype=torch.float64, device=self.dev)}
         self._bufs[dims] = B
         need = 0
@@ -178,17 +184,34 @@ class UNet3D:
     def _stream(self):
         return C.c_void_p(torch.cuda.current_stream(self.dev).cuda_stream)
 
-    def _conv(self, name, src, lds, dst, ldd, dims, relu=True, soff=0, doff=0):
+    def _conv(self, name, src, lds, dst, ldd, dims, relu=True, src2=None):
+        """Forward convolution of layer `name`; with src2, its input is the
+        channels of src (lds of them) followed by those of src2."""
         _, cin, cout, _ = next(l for l in LAYERS if l[0] == name)
-        nn_check(self.lib.efwi_nn_conv3(_p(src, soff), lds, cin, _p(self.w16[name]), _p(self.b[name]),
-                                        _p(dst, doff), ldd, cout, None, 0, *dims, 0, int(relu), self._stream()),
-                 f"conv {name}")
+        if src2 is None:
+            rc = self.lib.efwi_nn_conv3(_p(src), lds, cin, _p(self.w16[name]), _p(self.b[name]), _p(dst), ldd, cout,
+                                        None, 0, *dims, 0, int(relu), self._stream())
+        else:
+            k2 = src2.shape[1]
+            rc = self.lib.efwi_nn_conv3_cat(_p(src), lds, cin - k2, _p(src2), k2, k2, _p(self.w16[name]),
+                                            _p(self.b[name]), _p(dst), ldd, cout, None, 0, 0, None, 0, *dims, 0,
+                                            int(relu), self._stream())
+        nn_check(rc, f"conv {name}")
 
-    def _dgrad(self, name, gout, ldg, gin, ldi, mask, ldm, dims, goff=0, ioff=0, moff=0):
+    def _dgrad(self, name, gout, ldg, gin, ldi, mask, ldm, dims, gin2=None):
+        """Input gradient of layer `name` (a forward convolution with the
+        flipped weights); with gin2, its first ldi channels go to gin and the
+        rest to gin2."""
         _, cin, cout, _ = next(l for l in LAYERS if l[0] == name)
-        nn_check(self.lib.efwi_nn_conv3(_p(gout, goff), ldg, cout, _p(self.wt16[name]), None, _p(gin, ioff), ldi,
-                                        cin, None if mask is None else _p(mask, moff), ldm, *dims, 0, 0,
-                                        self._stream()), f"dgrad {name}")
+        mk = None if mask is None else _p(mask)
+        if gin2 is None:
+            rc = self.lib.efwi_nn_conv3(_p(gout), ldg, cout, _p(self.wt16[name]), None, _p(gin), ldi, cin, mk, ldm,
+                                        *dims, 0, 0, self._stream())
+        else:
+            n2 = gin2.shape[1]
+            rc = self.lib.efwi_nn_conv3_cat(_p(gout), ldg, cout, None, 0, 0, _p(self.wt16[name]), None, _p(gin),
+                                            ldi, cin - n2, _p(gin2), n2, n2, mk, ldm, *dims, 0, 0, self._stream())
+        nn_check(rc, f"dgrad {name}")
 
     def _wgrad_per(self, cout, cin, dims):
         """Bricks per weight-gradient workgroup: enough workgroups to fill the
@@ -200,13 +223,21 @@ class UNet3D:
         blocks = (cout // 16) * (cin // 16)
         return int(min(8, max(2, bricks * blocks // WGRAD_WORKGROUPS)))
 
-    def _wgrad(self, name, gpre, ldg, src, lds, dims, goff=0, soff=0):
+    def _wgrad(self, name, gpre, ldg, src, lds, dims, src2=None):
+        """Weight and bias gradients of layer `name`; with src2, its input is
+        the channels of src (lds of them) followed by those of src2."""
         _, cin, cout, _ = next(l for l in LAYERS if l[0] == name)
         s = self._stream()
-        nn_check(self.lib.efwi_nn_wgrad(_p(gpre, goff), ldg, cout, _p(src, soff), lds, cin, _p(self._scratch),
-                                        self._scratch.numel(), _p(self.grads[name + ".w"]),
-                                        _p(self.grads[name + ".b"]), *dims, self._wgrad_per(cout, cin, dims), s),
-                 f"wgrad {name}")
+        per = self._wgrad_per(cout, cin, dims)
+        gw, gb = _p(self.grads[name + ".w"]), _p(self.grads[name + ".b"])
+        if src2 is None:
+            rc = self.lib.efwi_nn_wgrad(_p(gpre), ldg, cout, _p(src), lds, cin, _p(self._scratch),
+                                        self._scratch.numel(), gw, gb, *dims, per, s)
+        else:
+            k2 = src2.shape[1]
+            rc = self.lib.efwi_nn_wgrad_cat(_p(gpre), ldg, cout, _p(src), lds, cin - k2, _p(src2), k2, k2,
+                                            _p(self._scratch), self._scratch.numel(), gw, gb, *dims, per, s)
+        nn_check(rc, f"wgrad {name}")
 
     # ---- forward / backward
     def forward(self, feat, dims):
@@ -217,18 +248,18 @@ class UNet3D:
         d3 = tuple(x // 4 for x in dims)
         v1 = d1[0] * d1[1] * d1[2]
         self._conv("enc1a", feat, 16, B["a1"], 16, d1)
-        self._conv("enc1b", B["a1"], 16, B["cat1"], 48, d1, doff=32)
-        nn_check(self.lib.efwi_nn_pool2(_p(B["cat1"], 32), 48, _p(B["p1"]), 16, 16, *d1, s), "pool1")
+        self._conv("enc1b", B["a1"], 16, B["e1"], 16, d1)
+        nn_check(self.lib.efwi_nn_pool2(_p(B["e1"]), 16, _p(B["p1"]), 16, 16, *d1, s), "pool1")
         self._conv("enc2a", B["p1"], 16, B["a2"], 32, d2)
-        self._conv("enc2b", B["a2"], 32, B["cat2"], 96, d2, doff=64)
-        nn_check(self.lib.efwi_nn_pool2(_p(B["cat2"], 64), 96, _p(B["p2"]), 32, 32, *d2, s), "pool2")
+        self._conv("enc2b", B["a2"], 32, B["e2"], 32, d2)
+        nn_check(self.lib.efwi_nn_pool2(_p(B["e2"]), 32, _p(B["p2"]), 32, 32, *d2, s), "pool2")
         self._conv("bot_a", B["p2"], 32, B["a3"], 64, d3)
         self._conv("bot_b", B["a3"], 64, B["b3"], 64, d3)
-        nn_check(self.lib.efwi_nn_up2(_p(B["b3"]), 64, _p(B["cat2"]), 96, 64, *d2, s), "up2")
-        self._conv("dec2a", B["cat2"], 96, B["d2a"], 32, d2)
+        nn_check(self.lib.efwi_nn_up2(_p(B["b3"]), 64, _p(B["u2"]), 64, 64, *d2, s), "up2")
+        self._conv("dec2a", B["u2"], 64, B["d2a"], 32, d2, src2=B["e2"])
         self._conv("dec2b", B["d2a"], 32, B["d2b"], 32, d2)
-        nn_check(self.lib.efwi_nn_up2(_p(B["d2b"]), 32, _p(B["cat1"]), 48, 32, *d1, s), "up1")
-        self._conv("dec1a", B["cat1"], 48, B["d1a"], 16, d1)
+        nn_check(self.lib.efwi_nn_up2(_p(B["d2b"]), 32, _p(B["u1"]), 32, 32, *d1, s), "up1")
+        self._conv("dec1a", B["u1"], 32, B["d1a"], 16, d1, src2=B["e1"])
         self._conv("dec1b", B["d1a"], 16, B["d1b"], 16, d1)
         nn_check(self.lib.efwi_nn_head(_p(B["d1b"]), 16, 16, _p(self.wh), _p(self.bh), _p(B["o"]), v1, s), "head")
         nn_check(self.lib.efwi_nn_gate(_p(feat), 16, _p(B["o"]), _p(B["out"]), v1, s), "gate")
@@ -256,21 +287,21 @@ class UNet3D:
         # level 1 decoder
         self._wgrad("dec1b", B["g1b"], 16, B["d1a"], 16, d1)
         self._dgrad("dec1b", B["g1b"], 16, B["g1"], 16, B["d1a"], 16, d1)
-        self._wgrad("dec1a", B["g1"], 16, B["cat1"], 48, d1)
-        # the ReLU derivative of cat1 is applied where each part is consumed, not
-        # here: the upsampled part's (that of d2b, shared by a parent's 8
-        # children) inside the upsampling gradient, the skip part's after the
-        # pooled gradient is added to it; either way a masked value is exactly
-        # zero and a kept one the same sum
-        self._dgrad("dec1a", B["g1"], 16, B["gcat1"], 48, None, 0, d1)
-        nn_check(self.lib.efwi_nn_up2_bwd(_p(B["gcat1"]), 48, _p(B["g2b"]), 32, 32, *d1, _p(B["d2b"]), 32, s),
+        self._wgrad("dec1a", B["g1"], 16, B["u1"], 32, d1, src2=B["e1"])
+        # the ReLU derivative of u1 | e1 is applied where each part is consumed,
+        # not here: u1's (that of d2b, shared by a parent's 8 children) inside
+        # the upsampling gradient, e1's after the pooled gradient is added to
+        # it; either way a masked value is exactly zero and a kept one the same
+        # sum
+        self._dgrad("dec1a", B["g1"], 16, B["gu1"], 32, None, 0, d1, gin2=B["ge1"])
+        nn_check(self.lib.efwi_nn_up2_bwd(_p(B["gu1"]), 32, _p(B["g2b"]), 32, 32, *d1, _p(B["d2b"]), 32, s),
                  "up1 bwd")
         # level 2 decoder
         self._wgrad("dec2b", B["g2b"], 32, B["d2a"], 32, d2)
         self._dgrad("dec2b", B["g2b"], 32, B["g2"], 32, B["d2a"], 32, d2)
-        self._wgrad("dec2a", B["g2"], 32, B["cat2"], 96, d2)
-        self._dgrad("dec2a", B["g2"], 32, B["gcat2"], 96, None, 0, d2)
-        nn_check(self.lib.efwi_nn_up2_bwd(_p(B["gcat2"]), 96, _p(B["g3b"]), 64, 64, *d2, _p(B["b3"]), 64, s),
+        self._wgrad("dec2a", B["g2"], 32, B["u2"], 64, d2, src2=B["e2"])
+        self._dgrad("dec2a", B["g2"], 32, B["gu2"], 64, None, 0, d2, gin2=B["ge2"])
+        nn_check(self.lib.efwi_nn_up2_bwd(_p(B["gu2"]), 64, _p(B["g3b"]), 64, 64, *d2, _p(B["b3"]), 64, s),
                  "up2 bwd")
         # bottleneck
         self._wgrad("bot_b", B["g3b"], 64, B["a3"], 64, d3)
@@ -278,17 +309,17 @@ class UNet3D:
         self._wgrad("bot_a", B["g3"], 64, B["p2"], 32, d3)
         self._dgrad("bot_a", B["g3"], 64, B["gp2"], 32, None, 0, d3)
         # level 2 encoder: skip gradient + pooled gradient, then the ReLU mask
-        nn_check(self.lib.efwi_nn_pool2_bwd(_p(B["gp2"]), 32, _p(B["gcat2"], 64), 96, 32, *d2, 1,
-                                            _p(B["cat2"], 64), 96, s), "pool2 bwd")
-        self._wgrad("enc2b", B["gcat2"], 96, B["a2"], 32, d2, goff=64)
-        self._dgrad("enc2b", B["gcat2"], 96, B["g2"], 32, B["a2"], 32, d2, goff=64)
+        nn_check(self.lib.efwi_nn_pool2_bwd(_p(B["gp2"]), 32, _p(B["ge2"]), 32, 32, *d2, 1, _p(B["e2"]), 32, s),
+                 "pool2 bwd")
+        self._wgrad("enc2b", B["ge2"], 32, B["a2"], 32, d2)
+        self._dgrad("enc2b", B["ge2"], 32, B["g2"], 32, B["a2"], 32, d2)
         self._wgrad("enc2a", B["g2"], 32, B["p1"], 16, d2)
         self._dgrad("enc2a", B["g2"], 32, B["gp1"], 16, None, 0, d2)
         # level 1 encoder
-        nn_check(self.lib.efwi_nn_pool2_bwd(_p(B["gp1"]), 16, _p(B["gcat1"], 32), 48, 16, *d1, 1,
-                                            _p(B["cat1"], 32), 48, s), "pool1 bwd")
-        self._wgrad("enc1b", B["gcat1"], 48, B["a1"], 16, d1, goff=32)
-        self._dgrad("enc1b", B["gcat1"], 48, B["g1"], 16, B["a1"], 16, d1, goff=32)
+        nn_check(self.lib.efwi_nn_pool2_bwd(_p(B["gp1"]), 16, _p(B["ge1"]), 16, 16, *d1, 1, _p(B["e1"]), 16, s),
+                 "pool1 bwd")
+        self._wgrad("enc1b", B["ge1"], 16, B["a1"], 16, d1)
+        self._dgrad("enc1b", B["ge1"], 16, B["g1"], 16, B["a1"], 16, d1)
         self._wgrad("enc1a", B["g1"], 16, self._feat, 16, d1)
         return self.grads
 

@@ -42,6 +42,18 @@ int efwi_nn_conv3(const uint16_t* src, int lds, int kc, const uint16_t* w, const
                   int ldd, int nc, const uint16_t* mask, int ldm, int nz, int nx, int ny, int mode, int relu,
                   void* stream);
 
+/* efwi_nn_conv3 on an input given as two channel ranges and an output split
+ * in two: input channels [0, kc1) from src (stride lds), [kc1, kc1 + kc2)
+ * from src2 (stride lds2); output channels [0, nc1) to dst (stride ldd),
+ * [nc1, nc1 + nc2) to dst2 (stride ldd2). src2 or dst2 may be NULL (one
+ * range). Weights, bias and mask (stride ldm >= nc1 + nc2) address the joined
+ * channels. Results equal efwi_nn_conv3 on the joined tensors bit for bit. For
+ * a skip concatenation kept as two separate tensors. */
+int efwi_nn_conv3_cat(const uint16_t* src, int lds, int kc1, const uint16_t* src2, int lds2, int kc2,
+                      const uint16_t* w, const float* bias, uint16_t* dst, int ldd, int nc1, uint16_t* dst2, int ldd2,
+                      int nc2, const uint16_t* mask, int ldm, int nz, int nx, int ny, int mode, int relu,
+                      void* stream);
+
 /* wt[t][k][n] = w[26 - t][n][k] for a layer's weights w[27][nc][kc]: the
  * input gradient of that layer is the mode-0 convolution of the output
  * gradient (kc' = nc input channels, nc' = kc outputs) with wt. wt must not
@@ -65,6 +77,14 @@ int efwi_nn_flip_weights_n(int n, const uint16_t* const* w, const int* kc, const
 long efwi_nn_wgrad_scratch(int mc, int kc, int nz, int nx, int ny, int per);
 int efwi_nn_wgrad(const uint16_t* dpre, int ldp, int mc, const uint16_t* src, int lds, int kc, float* scratch,
                   long scratch_floats, float* dw, float* db, int nz, int nx, int ny, int per, void* stream);
+
+/* efwi_nn_wgrad with src given as two channel ranges: [0, kc1) from src
+ * (stride lds), [kc1, kc1 + kc2) from src2 (stride lds2); dw[t][m][n] over the
+ * joined kc1 + kc2 channels, equal to efwi_nn_wgrad on the joined tensor.
+ * scratch as efwi_nn_wgrad_scratch(mc, kc1 + kc2, ...). */
+int efwi_nn_wgrad_cat(const uint16_t* dpre, int ldp, int mc, const uint16_t* src, int lds, int kc1, const uint16_t* src2,
+                      int lds2, int kc2, float* scratch, long scratch_floats, float* dw, float* db, int nz, int nx,
+                      int ny, int per, void* stream);
 
 /* efwi_nn_wgrad without a workspace argument and without any allocation:
  * the partial sums live in a static device array of

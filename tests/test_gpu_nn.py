@@ -135,6 +135,61 @@ def test_flip_weights_of_several_layers_in_one_launch():
     assert N.lib().efwi_nn_flip_weights_n(n, arr(wd), kcs, ncs, arr(wd), _s()) == 1
 
 
+@pytest.mark.parametrize("dims,k1,k2,n1,n2", [((6, 10, 70), 32, 16, 16, 0), ((5, 8, 40), 64, 32, 32, 0),
+                                               ((4, 6, 70), 16, 32, 32, 16), ((6, 8, 33), 16, 0, 32, 64),
+                                               ((16, 128, 64), 32, 16, 32, 16)])
+def test_convolutions_on_split_channels(dims, k1, k2, n1, n2):
+    """efwi_nn_conv3_cat and efwi_nn_wgrad_cat on an input kept as two
+    tensors (and an output written as two) give efwi_nn_conv3's and
+    efwi_nn_wgrad's results on the joined tensors bit for bit, with and
+    without a mask."""
+    from efwi import _native as N
+    from efwi.unet import nn_check
+    lib = N.lib()
+    rng = np.random.default_rng(k1 * 7 + k2 + n1 + n2)
+    nz, nx, ny = dims
+    V = nz * nx * ny
+    kc, nc = k1 + k2, n1 + n2
+    x = _dev(_bf(rng, nz, nx, ny, kc).reshape(V, kc))
+    x1 = x[:, :k1].contiguous()
+    x2 = x[:, k1:].contiguous() if k2 else None
+    w = _dev(_bf(rng, 27, nc, kc, scale=0.1))
+    b = torch.from_numpy(rng.standard_normal(nc).astype(np.float32)).to(DEV)
+    m = _dev(_bf(rng, nz, nx, ny, nc).reshape(V, nc))
+    print(f"[gpu] split-channel conv {dims} {k1}+{k2} -> {n1}+{n2}", flush=True)
+    for mask in (None, m):
+        ref = torch.zeros(V, nc, dtype=torch.bfloat16, device=DEV)
+        mk = None if mask is None else _p(mask)
+        nn_check(lib.efwi_nn_conv3(_p(x), kc, kc, _p(w), _p(b), _p(ref), nc, nc, mk, nc, nz, nx, ny, 0, 1, _s()),
+                 "conv")
+        o1 = torch.full((V, n1), 3.0, dtype=torch.bfloat16, device=DEV)
+        o2 = torch.full((V, n2), 3.0, dtype=torch.bfloat16, device=DEV) if n2 else None
+        nn_check(lib.efwi_nn_conv3_cat(_p(x1), k1, k1, None if x2 is None else _p(x2), k2, k2, _p(w), _p(b),
+                                       _p(o1), n1, n1, None if o2 is None else _p(o2), n2, n2, mk, nc, nz, nx, ny,
+                                       0, 1, _s()), "conv cat")
+        assert torch.equal(o1, ref[:, :n1])
+        if n2:
+            assert torch.equal(o2, ref[:, n1:])
+    if k2:
+        g = _dev(_bf(rng, nz, nx, ny, nc).reshape(V, nc))
+        per = 2
+        need = int(lib.efwi_nn_wgrad_scratch(nc, kc, nz, nx, ny, per))
+        scr = torch.empty(need, device=DEV)
+        dw, db = torch.zeros(27, nc, kc, device=DEV), torch.zeros(nc, device=DEV)
+        dw2, db2 = torch.full_like(dw, 5.0), torch.full_like(db, 5.0)
+        nn_check(lib.efwi_nn_wgrad(_p(g), nc, nc, _p(x), kc, kc, _p(scr), need, _p(dw), _p(db), nz, nx, ny, per,
+                                   _s()), "wgrad")
+        nn_check(lib.efwi_nn_wgrad_cat(_p(g), nc, nc, _p(x1), k1, k1, _p(x2), k2, k2, _p(scr), need, _p(dw2),
+                                       _p(db2), nz, nx, ny, per, _s()), "wgrad cat")
+        torch.cuda.synchronize()
+        # the atomic sums of the partial groups may add in another order
+        assert torch.allclose(dw2, dw, rtol=1e-6, atol=1e-6 * float(dw.abs().max()))
+        assert torch.allclose(db2, db, rtol=1e-6, atol=1e-6 * float(db.abs().max()))
+        # a chunk may not straddle the two inputs: k1 not a multiple of 16 is refused
+        assert lib.efwi_nn_wgrad_cat(_p(g), nc, nc, _p(x1), k1, 8, _p(x2), k2, k2, _p(scr), need, _p(dw2), None,
+                                     nz, nx, ny, per, _s()) == 1
+
+
 @pytest.mark.parametrize("dims,mc,kc", [((6, 10, 37), 16, 16), ((4, 8, 16), 64, 32), ((4, 4, 20), 32, 96)])
 def test_conv3_weight_gradient(dims, mc, kc):
     from efwi import _native as N
@@ -338,15 +393,15 @@ def _oracle_network(net, feat, dims, target, mask, count):
 
 
 def _layer_inputs(net, feat, dims):
-    """(input tensor, channel offset, channels, output tensor, channel offset,
+    """(input tensors, which are joined along channels, output tensor,
     volume) of every convolution of the network's last forward."""
     B = net._bufs[dims]
     d1, d2, d3 = dims, tuple(x // 2 for x in dims), tuple(x // 4 for x in dims)
-    return {"enc1a": (feat, 0, 16, B["a1"], 0, d1), "enc1b": (B["a1"], 0, 16, B["cat1"], 32, d1),
-            "enc2a": (B["p1"], 0, 16, B["a2"], 0, d2), "enc2b": (B["a2"], 0, 32, B["cat2"], 64, d2),
-            "bot_a": (B["p2"], 0, 32, B["a3"], 0, d3), "bot_b": (B["a3"], 0, 64, B["b3"], 0, d3),
-            "dec2a": (B["cat2"], 0, 96, B["d2a"], 0, d2), "dec2b": (B["d2a"], 0, 32, B["d2b"], 0, d2),
-            "dec1a": (B["cat1"], 0, 48, B["d1a"], 0, d1), "dec1b": (B["d1a"], 0, 16, B["d1b"], 0, d1)}
+    return {"enc1a": ((feat,), B["a1"], d1), "enc1b": ((B["a1"],), B["e1"], d1),
+            "enc2a": ((B["p1"],), B["a2"], d2), "enc2b": ((B["a2"],), B["e2"], d2),
+            "bot_a": ((B["p2"],), B["a3"], d3), "bot_b": ((B["a3"],), B["b3"], d3),
+            "dec2a": ((B["u2"], B["e2"]), B["d2a"], d2), "dec2b": ((B["d2a"],), B["d2b"], d2),
+            "dec1a": ((B["u1"], B["e1"]), B["d1a"], d1), "dec1b": ((B["d1a"],), B["d1b"], d1)}
 
 
 def _network_case(dims, seed=9):
@@ -377,11 +432,11 @@ def test_unet_layers_on_network_activations_match_oracle(dims):
     torch.cuda.synchronize()
     spec = _layer_inputs(net, feat, dims)
     for name, cin, cout, _ in LAYERS:
-        src, so, kc, dst, do, d = spec[name]
-        x = _host(src)[:, so:so + kc].reshape(*d, kc)
+        srcs, dst, d = spec[name]
+        x = np.concatenate([_host(t) for t in srcs], axis=1).reshape(*d, cin)
         w = _host(net.w16[name]).reshape(27, cout, cin)
         ref = O.conv3(x, w, bias=net.b[name].cpu().numpy(), relu=True).reshape(-1, cout)
-        got = _host(dst)[:, do:do + cout]
+        got = _host(dst)
         assert _ulps_ok(got, ref).all(), name
         assert np.mean(got != O.bf16_round(ref.astype(np.float32))) <= 1e-3, name
 
