@@ -4,7 +4,8 @@
 // Tensors are channels-last volumes: element (voxel v, channel c) at
 // p[v * ld + c], voxel v = (z * nx + x) * ny + y (y fastest, as the FWI grid),
 // ld (the channel stride) a multiple of 16, so a tensor can be a channel slice
-// of a wider one (the U-Net's skip concatenations are written in place).
+// of a wider one; a convolution can also take its input from two tensors and
+// write its output to two (a skip concatenation kept as its two parts).
 //
 // Convolutions and weight gradients as implicit GEMMs on
 // v_mfma_f32_16x16x32_bf16 (D = A B + C, 16 x 16 x 32, one wavefront). Lane
@@ -1090,11 +1091,6 @@ int efwi_nn_conv3_cat(const uint16_t* src, int lds, int kc1, const uint16_t* src
   // per wavefront (a divisor of the block count, so no block is partial)
   const int MB = ny >= 64 ? 4 : ny >= 32 ? 2 : 1;
   const int nbt = nc / 16;
-  // 32-channel chunks where no chunk would straddle the two sources, and only
-  // for bricks of 2 z-rows: with 4 z-rows a 32-channel halo (101 KB of LDS)
-  // leaves one workgroup per CU, a 16-channel one three, which measured faster
-  // (48x64x64 voxels, 32->32 29 -> 27 us, 96->32 93 -> 77 us)
-  const bool ck32 = kc % 32 == 0 && kc1 % 32 == 0 && MB < 4;
   const int TZ = conv_tz(MB);
   const int ntz = (nz + TZ - 1) / TZ, ntx = (nx + 1) / 2, nty = (ny + 16 * MB - 1) / (16 * MB);
   const long bricks = (long)ntz * ntx * nty;
@@ -1123,8 +1119,11 @@ int efwi_nn_conv3_cat(const uint16_t* src, int lds, int kc1, const uint16_t* src
   if (MB == 4) { EFWI_NN_CONV_NB(CK, 4) }                  \
   else if (MB == 2) { EFWI_NN_CONV_NB(CK, 2) }             \
   else { EFWI_NN_CONV_NB(CK, 1) }
-  if (ck32) { EFWI_NN_CONV_MB(32) }
-  else { EFWI_NN_CONV_MB(16) }
+  // 16-channel chunks (the kernel also takes CK = 32): their halos leave room
+  // for more workgroups per CU than 32-channel ones (with 4 z-rows 51 against
+  // 101 KB of LDS), which measured faster at every level (48x64x64 voxels,
+  // 32->32 29 -> 27 us, 96->32 93 -> 77 us; 24x32x32, 64->64 26.5 -> 18.8 us)
+  EFWI_NN_CONV_MB(16)
 #undef EFWI_NN_CONV_MB
 #undef EFWI_NN_CONV_NB
 #undef EFWI_NN_CONV_MODE
