@@ -216,12 +216,14 @@ class UNet3D:
     def _wgrad_per(self, cout, cin, dims):
         """Bricks per weight-gradient workgroup: enough workgroups to fill the
         GPU (about WGRAD_WORKGROUPS over the (m, n) channel blocks), between 2
-        and 8 bricks each (fewer bricks per workgroup multiply the FP32
-        partials that are written and summed; measured on MI355X at the
-        U-Net's shapes)."""
+        and 24 bricks each (fewer bricks per workgroup multiply the FP32
+        partials that are written and summed, more keep more halo planes in
+        the workgroup's z ring; measured on MI355X at the U-Net's shapes:
+        48->16 at 96x128x128 per 8 205 us, per 24 172 us; 96->32 at 48x64x64
+        per 8 112 us, per 12 96 us)."""
         bricks = int(self.lib.efwi_nn_wgrad_scratch(cout, cin, *dims, 1)) // (27 * cout * cin + cout)
         blocks = (cout // 16) * (cin // 16)
-        return int(min(8, max(2, bricks * blocks // WGRAD_WORKGROUPS)))
+        return int(min(24, max(2, bricks * blocks // WGRAD_WORKGROUPS)))
 
     def _wgrad(self, name, gpre, ldg, src, lds, dims, src2=None):
         """Weight and bias gradients of layer `name`; with src2, its input is
