@@ -170,6 +170,16 @@ def test_convolutions_on_split_channels(dims, k1, k2, n1, n2):
         assert torch.equal(o1, ref[:, :n1])
         if n2:
             assert torch.equal(o2, ref[:, n1:])
+            # the second output as a slice 4 channels into a wider tensor (8-byte
+            # aligned: both outputs then leave channel by channel)
+            wide = torch.full((V, n2 + 16), 3.0, dtype=torch.bfloat16, device=DEV)
+            o1.fill_(3.0)
+            nn_check(lib.efwi_nn_conv3_cat(_p(x1), k1, k1, None if x2 is None else _p(x2), k2, k2, _p(w), _p(b),
+                                           _p(o1), n1, n1, _p(wide, 4), n2 + 16, n2, mk, nc, nz, nx, ny, 0, 1,
+                                           _s()), "conv cat into a slice")
+            assert torch.equal(o1, ref[:, :n1])
+            assert torch.equal(wide[:, 4:4 + n2], ref[:, n1:])
+            assert bool((wide[:, :4] == 3.0).all()) and bool((wide[:, 4 + n2:] == 3.0).all())
     if k2:
         g = _dev(_bf(rng, nz, nx, ny, nc).reshape(V, nc))
         per = 2
