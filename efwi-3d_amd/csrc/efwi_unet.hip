@@ -563,6 +563,8 @@ __global__ void k_sum_chunks(const float* __restrict__ part, long stride, long w
   if (k >= total) return;
   const int c0 = blockIdx.y * kChunkGroup, c1 = min(nchunks, c0 + kChunkGroup);
   float s = 0.f;
+  // the loads of 8 chunks in flight at a time; the sum keeps chunk order
+#pragma unroll 8
   for (int c = c0; c < c1; c++) s += part[(long)c * stride + k];
   atomicAdd(k < width ? out + k : out2 + (k - width), s);
 }
