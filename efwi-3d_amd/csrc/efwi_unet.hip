@@ -124,7 +124,7 @@ __global__ __launch_bounds__(512) void k_conv3(const uint16_t* __restrict__ src,
                                                const uint16_t* __restrict__ mask, int ldm, Vol V, int relu,
                                                int ntz, int ntx, int nty, int per_xcd, int vec_out,
                                                const uint16_t* __restrict__ src2, int lds2, int kc1,
-                                               uint16_t* __restrict__ dst2, int ldd2, int nc1) {
+                                               uint16_t* __restrict__ dst2, int ldd2, int nc1, int up1) {
   constexpr int TY = 16 * MB, HW = TY + 2, G = CK / 8;
   constexpr int TZ = conv_tz(MB), NT = 128 * TZ, HR = (TZ + 2) * 4;  // halo rows
   constexpr int NQ = (27 * CK + 31) / 32;
@@ -171,6 +171,9 @@ __global__ __launch_bounds__(512) void k_conv3(const uint16_t* __restrict__ src,
     const uint16_t* const sb = second ? src2 : src;
     const long sl = second ? lds2 : lds;
     const int cs = second ? c0 - kc1 : c0;
+    // up1: src is the coarse (nz/2, nx/2, ny/2) tensor whose nearest
+    // upsampling is the input; a voxel reads its parent
+    const bool up = up1 && !second;
     // halo rows r = 4 hz + hx at (z0 - 1 + hz, x0 - 1 + hx), voxels hv at
     // y0 - 1 + hv. An item is one (hv, granule) pair over 8 rows (two z
     // planes; row offsets uniform), its eight loads in flight before the
@@ -192,7 +195,14 @@ __global__ __launch_bounds__(512) void k_conv3(const uint16_t* __restrict__ src,
           const bool ok = interior || (y_ok && (unsigned)(z0 - 1 + hz) < (unsigned)V.nz &&
                                        (unsigned)(x0 - 1 + hx) < (unsigned)V.nx);
           hq[rr] = uint4{0u, 0u, 0u, 0u};
-          if (ok) hq[rr] = *reinterpret_cast<const uint4*>(tp + ((long)(rr >> 2) * V.nx + hx) * V.ny * sl);
+          if (up) {
+            if (ok)
+              hq[rr] = *reinterpret_cast<const uint4*>(
+                  sb + ((((long)(z0 - 1 + hz) >> 1) * (V.nx >> 1) + ((x0 - 1 + hx) >> 1)) * (V.ny >> 1) + (ys >> 1)) * sl +
+                  cs + 8 * gr);
+          } else if (ok) {
+            hq[rr] = *reinterpret_cast<const uint4*>(tp + ((long)(rr >> 2) * V.nx + hx) * V.ny * sl);
+          }
         }
 #pragma unroll
         for (int rr = 0; rr < 8; rr++) *reinterpret_cast<uint4*>(lp + rr * HW * VB) = hq[rr];
@@ -412,7 +422,7 @@ __global__ __launch_bounds__(256) void k_wgrad(const uint16_t* __restrict__ dpre
                                                const uint16_t* __restrict__ src, int lds, int kc,
                                                float* __restrict__ part, float* __restrict__ dw,
                                                float* __restrict__ db, Vol V, int ntz, int ntx, int nty, int per,
-                                               const uint16_t* __restrict__ src2, int lds2, int kc1) {
+                                               const uint16_t* __restrict__ src2, int lds2, int kc1, int up1) {
   constexpr int HW = TY + 2, P = TY + 8, NK = TY / 32;
   constexpr int A_BYTES = 4 * 16 * P * 2, B_BYTES = 16 * 16 * P * 2, R_BYTES = 9 * 4 * 256 * 4;
   constexpr int S_BYTES = A_BYTES + (B_BYTES > R_BYTES ? B_BYTES : R_BYTES);
@@ -432,6 +442,8 @@ __global__ __launch_bounds__(256) void k_wgrad(const uint16_t* __restrict__ dpre
   const uint16_t* const sb = second ? src2 : src;
   const long sl = second ? lds2 : lds;
   const int scol = second ? nb * 16 - kc1 : nb * 16;
+  // up1: src is the coarse tensor whose nearest upsampling is the input
+  const bool up = up1 && !second;
   float bsum = 0.f;  // lane (i, g): dpre channel mb * 16 + i over its voxels
   f4 acc[27];
 #pragma unroll
@@ -479,11 +491,15 @@ __global__ __launch_bounds__(256) void k_wgrad(const uint16_t* __restrict__ dpre
         const int zs = z0 - 1 + hz;
         const bool zx_ok = x_ok && zs >= 0 && zs < V.nz;
         const uint16_t* p = sb + (v0 + hz * plane) * sl + scol + 8 * gr;
+        // coarse row of (zs, xs) when up (read only where zx_ok)
+        const uint16_t* pc =
+            sb + (((long)(zs >> 1) * (V.nx >> 1) + (xs >> 1)) * (V.ny >> 1)) * sl + scol + 8 * gr;
         uint4 q[4];
 #pragma unroll
         for (int k = 0; k < 4; k++) {
           q[k] = uint4{0u, 0u, 0u, 0u};
-          if (interior || (zx_ok && ys + k >= 0 && ys + k < V.ny)) q[k] = ld16(p + (long)k * sl);
+          if (interior || (zx_ok && ys + k >= 0 && ys + k < V.ny))
+            q[k] = up ? ld16(pc + (long)((ys + k) >> 1) * sl) : ld16(p + (long)k * sl);
         }
         store_quads(Bh + (((zs & 3) * 4 + hx) * 16 + 8 * gr) * P + 4 * y4, P, q);
       }
@@ -1074,10 +1090,11 @@ static int nn_check(hipStream_t s) {
   return hipGetLastError() == hipSuccess ? 0 : EFWI_NN_ERR_HIP;
 }
 
-int efwi_nn_conv3_cat(const uint16_t* src, int lds, int kc1, const uint16_t* src2, int lds2, int kc2,
+int efwi_nn_conv3_cat(const uint16_t* src, int lds, int kc1, int up1, const uint16_t* src2, int lds2, int kc2,
                       const uint16_t* w, const float* bias, uint16_t* dst, int ldd, int nc1, uint16_t* dst2, int ldd2,
                       int nc2, const uint16_t* mask, int ldm, int nz, int nx, int ny, int mode, int relu,
                       void* stream) {
+  if (up1 && (nz % 2 || nx % 2 || ny % 2)) return EFWI_NN_ERR_ARG;
   if (!src2) kc2 = 0;
   if (!dst2) nc2 = 0;
   const int kc = kc1 + kc2, nc = nc1 + nc2;
@@ -1108,7 +1125,7 @@ int efwi_nn_conv3_cat(const uint16_t* src, int lds, int kc1, const uint16_t* src
 #define EFWI_NN_CONV_LAUNCH(CK, M, B, MODE)                                                                       \
   hipLaunchKernelGGL((k_conv3<CK, M, B, MODE>), grid, dim3(128 * conv_tz(M)), 0, s, src, lds, kc, w, bias, dst,  \
                      ldd, nc, mask, ldm, V, relu, ntz, ntx, nty, per_xcd, vec_out, src2, lds2, kc1, dst2, ldd2,   \
-                     nc1)
+                     nc1, up1 ? 1 : 0)
 #define EFWI_NN_CONV_MODE(CK, M, B)                         \
   if (mode == 0) { EFWI_NN_CONV_LAUNCH(CK, M, B, 0); }    \
   else { EFWI_NN_CONV_LAUNCH(CK, M, B, 1); }
@@ -1135,8 +1152,8 @@ int efwi_nn_conv3_cat(const uint16_t* src, int lds, int kc1, const uint16_t* src
 
 int efwi_nn_conv3(const uint16_t* src, int lds, int kc, const uint16_t* w, const float* bias, uint16_t* dst, int ldd,
                   int nc, const uint16_t* mask, int ldm, int nz, int nx, int ny, int mode, int relu, void* stream) {
-  return efwi_nn_conv3_cat(src, lds, kc, nullptr, 0, 0, w, bias, dst, ldd, nc, nullptr, 0, 0, mask, ldm, nz, nx, ny,
-                           mode, relu, stream);
+  return efwi_nn_conv3_cat(src, lds, kc, 0, nullptr, 0, 0, w, bias, dst, ldd, nc, nullptr, 0, 0, mask, ldm, nz, nx,
+                           ny, mode, relu, stream);
 }
 
 int efwi_nn_flip_weights(const uint16_t* w, int kc, int nc, uint16_t* wt, void* stream) {
@@ -1186,7 +1203,7 @@ static bool wgrad_args_ok(const uint16_t* dpre, int ldp, int mc, const uint16_t*
 }
 static int wgrad_launch(const uint16_t* dpre, int ldp, int mc, const uint16_t* src, int lds, int kc, float* scratch,
                         float* dw, float* db, int nz, int nx, int ny, int per, hipStream_t s,
-                        const uint16_t* src2 = nullptr, int lds2 = 0, int kc1 = 0);
+                        const uint16_t* src2 = nullptr, int lds2 = 0, int kc1 = 0, int up1 = 0);
 
 int efwi_nn_wgrad(const uint16_t* dpre, int ldp, int mc, const uint16_t* src, int lds, int kc, float* scratch,
                   long scratch_floats, float* dw, float* db, int nz, int nx, int ny, int per, void* stream) {
@@ -1195,9 +1212,10 @@ int efwi_nn_wgrad(const uint16_t* dpre, int ldp, int mc, const uint16_t* src, in
   return wgrad_launch(dpre, ldp, mc, src, lds, kc, scratch, dw, db, nz, nx, ny, per, (hipStream_t)stream);
 }
 
-int efwi_nn_wgrad_cat(const uint16_t* dpre, int ldp, int mc, const uint16_t* src, int lds, int kc1, const uint16_t* src2,
-                      int lds2, int kc2, float* scratch, long scratch_floats, float* dw, float* db, int nz, int nx,
-                      int ny, int per, void* stream) {
+int efwi_nn_wgrad_cat(const uint16_t* dpre, int ldp, int mc, const uint16_t* src, int lds, int kc1, int up1,
+                      const uint16_t* src2, int lds2, int kc2, float* scratch, long scratch_floats, float* dw,
+                      float* db, int nz, int nx, int ny, int per, void* stream) {
+  if (up1 && (nz % 2 || nx % 2 || ny % 2)) return EFWI_NN_ERR_ARG;
   const int kc = kc1 + kc2;
   if (!dpre || !src || !src2 || !dw || !scratch || mc <= 0 || mc % 16 || kc1 <= 0 || kc1 % 16 || kc2 <= 0 ||
       kc2 % 16 || ldp % 16 || ldp < mc || lds % 16 || lds < kc1 || lds2 % 16 || lds2 < kc2 || per <= 0 || nz <= 0 ||
@@ -1205,7 +1223,7 @@ int efwi_nn_wgrad_cat(const uint16_t* dpre, int ldp, int mc, const uint16_t* src
     return EFWI_NN_ERR_ARG;
   if (efwi_nn_wgrad_scratch(mc, kc, nz, nx, ny, per) > scratch_floats) return EFWI_NN_ERR_ARG;
   return wgrad_launch(dpre, ldp, mc, src, lds, kc, scratch, dw, db, nz, nx, ny, per, (hipStream_t)stream, src2, lds2,
-                      kc1);
+                      kc1, up1 ? 1 : 0);
 }
 
 // The partial sums of efwi_nn_wgrad_noalloc: one array per GPU, for the
@@ -1247,7 +1265,7 @@ int efwi_nn_wgrad_noalloc(const uint16_t* dpre, int ldp, int mc, const uint16_t*
 
 static int wgrad_launch(const uint16_t* dpre, int ldp, int mc, const uint16_t* src, int lds, int kc, float* scratch,
                         float* dw, float* db, int nz, int nx, int ny, int per, hipStream_t s,
-                        const uint16_t* src2, int lds2, int kc1) {
+                        const uint16_t* src2, int lds2, int kc1, int up1) {
   const Vol V{nz, nx, ny};
   if (!src2) kc1 = kc;
   const int ty = wgrad_ty(ny);
@@ -1256,10 +1274,10 @@ static int wgrad_launch(const uint16_t* dpre, int ldp, int mc, const uint16_t* s
   const dim3 grid((unsigned)chunks, (unsigned)(mc / 16), (unsigned)(kc / 16));
   if (ty == 64)
     hipLaunchKernelGGL((k_wgrad<64>), grid, dim3(256), 0, s, dpre, ldp, mc, src, lds, kc, scratch, dw, db, V, ntz, ntx,
-                       nty, per, src2, lds2, kc1);
+                       nty, per, src2, lds2, kc1, up1);
   else
     hipLaunchKernelGGL((k_wgrad<32>), grid, dim3(256), 0, s, dpre, ldp, mc, src, lds, kc, scratch, dw, db, V, ntz, ntx,
-                       nty, per, src2, lds2, kc1);
+                       nty, per, src2, lds2, kc1, up1);
   const long width = 27L * mc * kc, total = width + (db ? mc : 0);
   hipLaunchKernelGGL(k_sum_chunks, dim3(grid_for(total, 256, 1L << 30), (chunks + kChunkGroup - 1) / kChunkGroup),
                      dim3(256), 0, s, scratch, width + mc, width, total, chunks, dw, db);

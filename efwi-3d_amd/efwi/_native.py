@@ -83,11 +83,11 @@ SIGNATURES = {
     # learned preconditioner (include/efwi_nn.h)
     "efwi_nn_conv3": (C.c_int, [_vp, C.c_int, C.c_int, _vp, _vp, _vp, C.c_int, C.c_int, _vp, C.c_int,
                                 C.c_int, C.c_int, C.c_int, C.c_int, C.c_int, _vp]),
-    "efwi_nn_conv3_cat": (C.c_int, [_vp, C.c_int, C.c_int, _vp, C.c_int, C.c_int, _vp, _vp, _vp, C.c_int, C.c_int,
-                                    _vp, C.c_int, C.c_int, _vp, C.c_int, C.c_int, C.c_int, C.c_int, C.c_int,
-                                    C.c_int, _vp]),
-    "efwi_nn_wgrad_cat": (C.c_int, [_vp, C.c_int, C.c_int, _vp, C.c_int, C.c_int, _vp, C.c_int, C.c_int, _vp,
-                                    C.c_long, _vp, _vp, C.c_int, C.c_int, C.c_int, C.c_int, _vp]),
+    "efwi_nn_conv3_cat": (C.c_int, [_vp, C.c_int, C.c_int, C.c_int, _vp, C.c_int, C.c_int, _vp, _vp, _vp, C.c_int,
+                                    C.c_int, _vp, C.c_int, C.c_int, _vp, C.c_int, C.c_int, C.c_int, C.c_int,
+                                    C.c_int, C.c_int, _vp]),
+    "efwi_nn_wgrad_cat": (C.c_int, [_vp, C.c_int, C.c_int, _vp, C.c_int, C.c_int, C.c_int, _vp, C.c_int, C.c_int,
+                                    _vp, C.c_long, _vp, _vp, C.c_int, C.c_int, C.c_int, C.c_int, _vp]),
     "efwi_nn_flip_weights": (C.c_int, [_vp, C.c_int, C.c_int, _vp, _vp]),
     "efwi_nn_flip_weights_n": (C.c_int, [C.c_int, _vp, _vp, _vp, _vp, _vp]),
     "efwi_nn_wgrad_scratch": (C.c_long, [C.c_int, C.c_int, C.c_int, C.c_int, C.c_int, C.c_int]),

@@ -10,18 +10,19 @@ Architecture (3 levels, 16 base channels, 3x3x3 convolutions with ReLU,
 
     enc1a 16->16, enc1b 16->16 = e1
       pool -> enc2a 16->32, enc2b 32->32 = e2
-        pool -> bot_a 32->64, bot_b 64->64 -> up = u2
-                  dec2a (u2 | e2) 96->32, dec2b 32->32 -> up = u1
-                            dec1a (u1 | e1) 48->16, dec1b 16->16 -> head 16->1 (FP32)
+        pool -> bot_a 32->64, bot_b 64->64 = b3
+                  dec2a (up(b3) | e2) 96->32, dec2b 32->32 = d2b
+                            dec1a (up(d2b) | e1) 48->16, dec1b 16->16 -> head 16->1 (FP32)
     update = g exp(clamp(head, -4, 6))   (g = input channel 0: a learned,
                             spatially varying scaling of the gradient, as a
                             preconditioner is)
 
 Weights and activations are BF16 with FP32 accumulation; FP32 master weights
 and Adam state; activation gradients BF16, weight gradients FP32. The
-skip concatenations stay two tensors each (the upsampled part and the skip
-activation), which the decoder convolutions read as one input and whose input
-gradients they write as two outputs; the ReLU derivative of each activation is
+skip concatenations are never formed: the decoder convolutions read the coarse
+tensor through its nearest upsampling and the skip activation as one input,
+and write their input gradients as two outputs; the ReLU derivative of each
+activation is
 applied as a mask inside the input-gradient, pooling-gradient and
 upsampling-gradient kernels.
 """
@@ -161,9 +162,8 @@ class UNet3D:
         v1 = nz * nx * ny
         v2, v3 = v1 // 8, v1 // 64
         bf = lambda n, c: torch.zeros(n, c, dtype=torch.bfloat16, device=self.dev)
-        # u1 | e1 and u2 | e2: the two parts of the skip concatenations
-        B = {"a1": bf(v1, 16), "e1": bf(v1, 16), "u1": bf(v1, 32), "p1": bf(v2, 16), "a2": bf(v2, 32),
-             "e2": bf(v2, 32), "u2": bf(v2, 64), "p2": bf(v3, 32), "a3": bf(v3, 64), "b3": bf(v3, 64),
+        B = {"a1": bf(v1, 16), "e1": bf(v1, 16), "p1": bf(v2, 16), "a2": bf(v2, 32),
+             "e2": bf(v2, 32), "p2": bf(v3, 32), "a3": bf(v3, 64), "b3": bf(v3, 64),
              "d2a": bf(v2, 32), "d2b": bf(v2, 32), "d1a": bf(v1, 16), "d1b": bf(v1, 16),
              "o": torch.zeros(v1, device=self.dev), "out": torch.zeros(v1, device=self.dev),
              "do": torch.zeros(v1, device=self.dev),
@@ -186,14 +186,15 @@ class UNet3D:
 
     def _conv(self, name, src, lds, dst, ldd, dims, relu=True, src2=None):
         """Forward convolution of layer `name`; with src2, its input is the
-        channels of src (lds of them) followed by those of src2."""
+        nearest upsampling of the half-size src (lds channels) followed by the
+        channels of src2."""
         _, cin, cout, _ = next(l for l in LAYERS if l[0] == name)
         if src2 is None:
             rc = self.lib.efwi_nn_conv3(_p(src), lds, cin, _p(self.w16[name]), _p(self.b[name]), _p(dst), ldd, cout,
                                         None, 0, *dims, 0, int(relu), self._stream())
         else:
             k2 = src2.shape[1]
-            rc = self.lib.efwi_nn_conv3_cat(_p(src), lds, cin - k2, _p(src2), k2, k2, _p(self.w16[name]),
+            rc = self.lib.efwi_nn_conv3_cat(_p(src), lds, cin - k2, 1, _p(src2), k2, k2, _p(self.w16[name]),
                                             _p(self.b[name]), _p(dst), ldd, cout, None, 0, 0, None, 0, *dims, 0,
                                             int(relu), self._stream())
         nn_check(rc, f"conv {name}")
@@ -209,7 +210,7 @@ class UNet3D:
                                         *dims, 0, 0, self._stream())
         else:
             n2 = gin2.shape[1]
-            rc = self.lib.efwi_nn_conv3_cat(_p(gout), ldg, cout, None, 0, 0, _p(self.wt16[name]), None, _p(gin),
+            rc = self.lib.efwi_nn_conv3_cat(_p(gout), ldg, cout, 0, None, 0, 0, _p(self.wt16[name]), None, _p(gin),
                                             ldi, cin - n2, _p(gin2), n2, n2, mk, ldm, *dims, 0, 0, self._stream())
         nn_check(rc, f"dgrad {name}")
 
@@ -227,7 +228,8 @@ class UNet3D:
 
     def _wgrad(self, name, gpre, ldg, src, lds, dims, src2=None):
         """Weight and bias gradients of layer `name`; with src2, its input is
-        the channels of src (lds of them) followed by those of src2."""
+        the nearest upsampling of the half-size src (lds channels) followed by
+        the channels of src2."""
         _, cin, cout, _ = next(l for l in LAYERS if l[0] == name)
         s = self._stream()
         per = self._wgrad_per(cout, cin, dims)
@@ -237,7 +239,7 @@ class UNet3D:
                                         self._scratch.numel(), gw, gb, *dims, per, s)
         else:
             k2 = src2.shape[1]
-            rc = self.lib.efwi_nn_wgrad_cat(_p(gpre), ldg, cout, _p(src), lds, cin - k2, _p(src2), k2, k2,
+            rc = self.lib.efwi_nn_wgrad_cat(_p(gpre), ldg, cout, _p(src), lds, cin - k2, 1, _p(src2), k2, k2,
                                             _p(self._scratch), self._scratch.numel(), gw, gb, *dims, per, s)
         nn_check(rc, f"wgrad {name}")
 
@@ -257,11 +259,9 @@ class UNet3D:
         nn_check(self.lib.efwi_nn_pool2(_p(B["e2"]), 32, _p(B["p2"]), 32, 32, *d2, s), "pool2")
         self._conv("bot_a", B["p2"], 32, B["a3"], 64, d3)
         self._conv("bot_b", B["a3"], 64, B["b3"], 64, d3)
-        nn_check(self.lib.efwi_nn_up2(_p(B["b3"]), 64, _p(B["u2"]), 64, 64, *d2, s), "up2")
-        self._conv("dec2a", B["u2"], 64, B["d2a"], 32, d2, src2=B["e2"])
+        self._conv("dec2a", B["b3"], 64, B["d2a"], 32, d2, src2=B["e2"])
         self._conv("dec2b", B["d2a"], 32, B["d2b"], 32, d2)
-        nn_check(self.lib.efwi_nn_up2(_p(B["d2b"]), 32, _p(B["u1"]), 32, 32, *d1, s), "up1")
-        self._conv("dec1a", B["u1"], 32, B["d1a"], 16, d1, src2=B["e1"])
+        self._conv("dec1a", B["d2b"], 32, B["d1a"], 16, d1, src2=B["e1"])
         self._conv("dec1b", B["d1a"], 16, B["d1b"], 16, d1)
         nn_check(self.lib.efwi_nn_head(_p(B["d1b"]), 16, 16, _p(self.wh), _p(self.bh), _p(B["o"]), v1, s), "head")
         nn_check(self.lib.efwi_nn_gate(_p(feat), 16, _p(B["o"]), _p(B["out"]), v1, s), "gate")
@@ -289,19 +289,19 @@ class UNet3D:
         # level 1 decoder
         self._wgrad("dec1b", B["g1b"], 16, B["d1a"], 16, d1)
         self._dgrad("dec1b", B["g1b"], 16, B["g1"], 16, B["d1a"], 16, d1)
-        self._wgrad("dec1a", B["g1"], 16, B["u1"], 32, d1, src2=B["e1"])
-        # the ReLU derivative of u1 | e1 is applied where each part is consumed,
-        # not here: u1's (that of d2b, shared by a parent's 8 children) inside
-        # the upsampling gradient, e1's after the pooled gradient is added to
-        # it; either way a masked value is exactly zero and a kept one the same
-        # sum
+        self._wgrad("dec1a", B["g1"], 16, B["d2b"], 32, d1, src2=B["e1"])
+        # the ReLU derivative of up(d2b) | e1 is applied where each part is
+        # consumed, not here: the upsampled part's (that of d2b, shared by a
+        # parent's 8 children) inside the upsampling gradient, e1's after the
+        # pooled gradient is added to it; either way a masked value is exactly
+        # zero and a kept one the same sum
         self._dgrad("dec1a", B["g1"], 16, B["gu1"], 32, None, 0, d1, gin2=B["ge1"])
         nn_check(self.lib.efwi_nn_up2_bwd(_p(B["gu1"]), 32, _p(B["g2b"]), 32, 32, *d1, _p(B["d2b"]), 32, s),
                  "up1 bwd")
         # level 2 decoder
         self._wgrad("dec2b", B["g2b"], 32, B["d2a"], 32, d2)
         self._dgrad("dec2b", B["g2b"], 32, B["g2"], 32, B["d2a"], 32, d2)
-        self._wgrad("dec2a", B["g2"], 32, B["u2"], 64, d2, src2=B["e2"])
+        self._wgrad("dec2a", B["g2"], 32, B["b3"], 64, d2, src2=B["e2"])
         self._dgrad("dec2a", B["g2"], 32, B["gu2"], 64, None, 0, d2, gin2=B["ge2"])
         nn_check(self.lib.efwi_nn_up2_bwd(_p(B["gu2"]), 64, _p(B["g3b"]), 64, 64, *d2, _p(B["b3"]), 64, s),
                  "up2 bwd")

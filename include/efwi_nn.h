@@ -46,10 +46,12 @@ int efwi_nn_conv3(const uint16_t* src, int lds, int kc, const uint16_t* w, const
  * in two: input channels [0, kc1) from src (stride lds), [kc1, kc1 + kc2)
  * from src2 (stride lds2); output channels [0, nc1) to dst (stride ldd),
  * [nc1, nc1 + nc2) to dst2 (stride ldd2). src2 or dst2 may be NULL (one
- * range). Weights, bias and mask (stride ldm >= nc1 + nc2) address the joined
- * channels. Results equal efwi_nn_conv3 on the joined tensors bit for bit. For
- * a skip concatenation kept as two separate tensors. */
-int efwi_nn_conv3_cat(const uint16_t* src, int lds, int kc1, const uint16_t* src2, int lds2, int kc2,
+ * range). With up1, src is a (nz/2, nx/2, ny/2) volume (even sizes) whose
+ * nearest upsampling is read: voxel (z, x, y) takes src's (z/2, x/2, y/2).
+ * Weights, bias and mask (stride ldm >= nc1 + nc2) address the joined
+ * channels. Results equal efwi_nn_conv3 on the joined (and upsampled) tensors
+ * bit for bit. For a skip concatenation kept as two separate tensors. */
+int efwi_nn_conv3_cat(const uint16_t* src, int lds, int kc1, int up1, const uint16_t* src2, int lds2, int kc2,
                       const uint16_t* w, const float* bias, uint16_t* dst, int ldd, int nc1, uint16_t* dst2, int ldd2,
                       int nc2, const uint16_t* mask, int ldm, int nz, int nx, int ny, int mode, int relu,
                       void* stream);
@@ -79,12 +81,13 @@ int efwi_nn_wgrad(const uint16_t* dpre, int ldp, int mc, const uint16_t* src, in
                   long scratch_floats, float* dw, float* db, int nz, int nx, int ny, int per, void* stream);
 
 /* efwi_nn_wgrad with src given as two channel ranges: [0, kc1) from src
- * (stride lds), [kc1, kc1 + kc2) from src2 (stride lds2); dw[t][m][n] over the
- * joined kc1 + kc2 channels, equal to efwi_nn_wgrad on the joined tensor.
- * scratch as efwi_nn_wgrad_scratch(mc, kc1 + kc2, ...). */
-int efwi_nn_wgrad_cat(const uint16_t* dpre, int ldp, int mc, const uint16_t* src, int lds, int kc1, const uint16_t* src2,
-                      int lds2, int kc2, float* scratch, long scratch_floats, float* dw, float* db, int nz, int nx,
-                      int ny, int per, void* stream);
+ * (stride lds; with up1 a half-size volume read through nearest upsampling,
+ * as in efwi_nn_conv3_cat), [kc1, kc1 + kc2) from src2 (stride lds2);
+ * dw[t][m][n] over the joined kc1 + kc2 channels, equal to efwi_nn_wgrad on
+ * the joined tensor. scratch as efwi_nn_wgrad_scratch(mc, kc1 + kc2, ...). */
+int efwi_nn_wgrad_cat(const uint16_t* dpre, int ldp, int mc, const uint16_t* src, int lds, int kc1, int up1,
+                      const uint16_t* src2, int lds2, int kc2, float* scratch, long scratch_floats, float* dw,
+                      float* db, int nz, int nx, int ny, int per, void* stream);
 
 /* efwi_nn_wgrad without a workspace argument and without any allocation:
  * the partial sums live in a static device array of

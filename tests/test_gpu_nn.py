@@ -135,24 +135,33 @@ def test_flip_weights_of_several_layers_in_one_launch():
     assert N.lib().efwi_nn_flip_weights_n(n, arr(wd), kcs, ncs, arr(wd), _s()) == 1
 
 
-@pytest.mark.parametrize("dims,k1,k2,n1,n2", [((6, 10, 70), 32, 16, 16, 0), ((5, 8, 40), 64, 32, 32, 0),
-                                               ((4, 6, 70), 16, 32, 32, 16), ((6, 8, 33), 16, 0, 32, 64),
-                                               ((16, 128, 64), 32, 16, 32, 16)])
-def test_convolutions_on_split_channels(dims, k1, k2, n1, n2):
+@pytest.mark.parametrize("dims,k1,k2,n1,n2,up", [((6, 10, 70), 32, 16, 16, 0, 0), ((5, 8, 40), 64, 32, 32, 0, 0),
+                                                  ((4, 6, 70), 16, 32, 32, 16, 0), ((6, 8, 33), 16, 0, 32, 64, 0),
+                                                  ((16, 128, 64), 32, 16, 32, 16, 0), ((6, 10, 70), 32, 16, 16, 0, 1),
+                                                  ((8, 12, 130), 64, 32, 32, 0, 1), ((16, 128, 64), 32, 16, 16, 0, 1)])
+def test_convolutions_on_split_channels(dims, k1, k2, n1, n2, up):
     """efwi_nn_conv3_cat and efwi_nn_wgrad_cat on an input kept as two
-    tensors (and an output written as two) give efwi_nn_conv3's and
-    efwi_nn_wgrad's results on the joined tensors bit for bit, with and
-    without a mask."""
+    tensors (the first, with up, a half-size volume read through its nearest
+    upsampling), and an output written as two, give efwi_nn_conv3's and
+    efwi_nn_wgrad's results on the joined tensors bit for bit, with and without
+    a mask."""
     from efwi import _native as N
     from efwi.unet import nn_check
     lib = N.lib()
-    rng = np.random.default_rng(k1 * 7 + k2 + n1 + n2)
+    rng = np.random.default_rng(k1 * 7 + k2 + n1 + n2 + up)
     nz, nx, ny = dims
     V = nz * nx * ny
     kc, nc = k1 + k2, n1 + n2
-    x = _dev(_bf(rng, nz, nx, ny, kc).reshape(V, kc))
-    x1 = x[:, :k1].contiguous()
-    x2 = x[:, k1:].contiguous() if k2 else None
+    if up:
+        coarse = _bf(rng, nz // 2, nx // 2, ny // 2, k1)
+        x1 = _dev(coarse.reshape(-1, k1))
+        x2h = _bf(rng, nz, nx, ny, k2).reshape(V, k2)
+        x = _dev(np.concatenate([O.up2(coarse).reshape(V, k1), x2h], axis=1))
+        x2 = _dev(x2h)
+    else:
+        x = _dev(_bf(rng, nz, nx, ny, kc).reshape(V, kc))
+        x1 = x[:, :k1].contiguous()
+        x2 = x[:, k1:].contiguous() if k2 else None
     w = _dev(_bf(rng, 27, nc, kc, scale=0.1))
     b = torch.from_numpy(rng.standard_normal(nc).astype(np.float32)).to(DEV)
     m = _dev(_bf(rng, nz, nx, ny, nc).reshape(V, nc))
@@ -164,7 +173,7 @@ def test_convolutions_on_split_channels(dims, k1, k2, n1, n2):
                  "conv")
         o1 = torch.full((V, n1), 3.0, dtype=torch.bfloat16, device=DEV)
         o2 = torch.full((V, n2), 3.0, dtype=torch.bfloat16, device=DEV) if n2 else None
-        nn_check(lib.efwi_nn_conv3_cat(_p(x1), k1, k1, None if x2 is None else _p(x2), k2, k2, _p(w), _p(b),
+        nn_check(lib.efwi_nn_conv3_cat(_p(x1), k1, k1, up, None if x2 is None else _p(x2), k2, k2, _p(w), _p(b),
                                        _p(o1), n1, n1, None if o2 is None else _p(o2), n2, n2, mk, nc, nz, nx, ny,
                                        0, 1, _s()), "conv cat")
         assert torch.equal(o1, ref[:, :n1])
@@ -174,9 +183,9 @@ def test_convolutions_on_split_channels(dims, k1, k2, n1, n2):
             # aligned: both outputs then leave channel by channel)
             wide = torch.full((V, n2 + 16), 3.0, dtype=torch.bfloat16, device=DEV)
             o1.fill_(3.0)
-            nn_check(lib.efwi_nn_conv3_cat(_p(x1), k1, k1, None if x2 is None else _p(x2), k2, k2, _p(w), _p(b),
-                                           _p(o1), n1, n1, _p(wide, 4), n2 + 16, n2, mk, nc, nz, nx, ny, 0, 1,
-                                           _s()), "conv cat into a slice")
+            nn_check(lib.efwi_nn_conv3_cat(_p(x1), k1, k1, up, None if x2 is None else _p(x2), k2, k2, _p(w),
+                                           _p(b), _p(o1), n1, n1, _p(wide, 4), n2 + 16, n2, mk, nc, nz, nx, ny, 0,
+                                           1, _s()), "conv cat into a slice")
             assert torch.equal(o1, ref[:, :n1])
             assert torch.equal(wide[:, 4:4 + n2], ref[:, n1:])
             assert bool((wide[:, :4] == 3.0).all()) and bool((wide[:, 4 + n2:] == 3.0).all())
@@ -189,15 +198,15 @@ def test_convolutions_on_split_channels(dims, k1, k2, n1, n2):
         dw2, db2 = torch.full_like(dw, 5.0), torch.full_like(db, 5.0)
         nn_check(lib.efwi_nn_wgrad(_p(g), nc, nc, _p(x), kc, kc, _p(scr), need, _p(dw), _p(db), nz, nx, ny, per,
                                    _s()), "wgrad")
-        nn_check(lib.efwi_nn_wgrad_cat(_p(g), nc, nc, _p(x1), k1, k1, _p(x2), k2, k2, _p(scr), need, _p(dw2),
+        nn_check(lib.efwi_nn_wgrad_cat(_p(g), nc, nc, _p(x1), k1, k1, up, _p(x2), k2, k2, _p(scr), need, _p(dw2),
                                        _p(db2), nz, nx, ny, per, _s()), "wgrad cat")
         torch.cuda.synchronize()
         # the atomic sums of the partial groups may add in another order
         assert torch.allclose(dw2, dw, rtol=1e-6, atol=1e-6 * float(dw.abs().max()))
         assert torch.allclose(db2, db, rtol=1e-6, atol=1e-6 * float(db.abs().max()))
         # a chunk may not straddle the two inputs: k1 not a multiple of 16 is refused
-        assert lib.efwi_nn_wgrad_cat(_p(g), nc, nc, _p(x1), k1, 8, _p(x2), k2, k2, _p(scr), need, _p(dw2), None,
-                                     nz, nx, ny, per, _s()) == 1
+        assert lib.efwi_nn_wgrad_cat(_p(g), nc, nc, _p(x1), k1, 8, up, _p(x2), k2, k2, _p(scr), need, _p(dw2),
+                                     None, nz, nx, ny, per, _s()) == 1
 
 
 @pytest.mark.parametrize("dims,mc,kc", [((6, 10, 37), 16, 16), ((4, 8, 16), 64, 32), ((4, 4, 20), 32, 96)])
@@ -410,8 +419,21 @@ def _layer_inputs(net, feat, dims):
     return {"enc1a": ((feat,), B["a1"], d1), "enc1b": ((B["a1"],), B["e1"], d1),
             "enc2a": ((B["p1"],), B["a2"], d2), "enc2b": ((B["a2"],), B["e2"], d2),
             "bot_a": ((B["p2"],), B["a3"], d3), "bot_b": ((B["a3"],), B["b3"], d3),
-            "dec2a": ((B["u2"], B["e2"]), B["d2a"], d2), "dec2b": ((B["d2a"],), B["d2b"], d2),
-            "dec1a": ((B["u1"], B["e1"]), B["d1a"], d1), "dec1b": ((B["d1a"],), B["d1b"], d1)}
+            "dec2a": ((("up", B["b3"], d3), B["e2"]), B["d2a"], d2), "dec2b": ((B["d2a"],), B["d2b"], d2),
+            "dec1a": ((("up", B["d2b"], d2), B["e1"]), B["d1a"], d1), "dec1b": ((B["d1a"],), B["d1b"], d1)}
+
+
+def _joined(srcs):
+    """Host array of the channels of srcs side by side; ("up", t, dims) is the
+    nearest upsampling of t, a (dims) volume."""
+    parts = []
+    for t in srcs:
+        if isinstance(t, tuple):
+            _, c, d = t
+            parts.append(O.up2(_host(c).reshape(*d, -1)).reshape(-1, c.shape[1]))
+        else:
+            parts.append(_host(t))
+    return np.concatenate(parts, axis=1)
 
 
 def _network_case(dims, seed=9):
@@ -443,7 +465,7 @@ def test_unet_layers_on_network_activations_match_oracle(dims):
     spec = _layer_inputs(net, feat, dims)
     for name, cin, cout, _ in LAYERS:
         srcs, dst, d = spec[name]
-        x = np.concatenate([_host(t) for t in srcs], axis=1).reshape(*d, cin)
+        x = _joined(srcs).reshape(*d, cin)
         w = _host(net.w16[name]).reshape(27, cout, cin)
         ref = O.conv3(x, w, bias=net.b[name].cpu().numpy(), relu=True).reshape(-1, cout)
         got = _host(dst)
