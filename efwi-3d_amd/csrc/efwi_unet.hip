@@ -661,6 +661,12 @@ __global__ void k_pool2(const uint16_t* __restrict__ src, int lds, uint16_t* __r
     dst[Cv.vox(z, x, y) * ldd + ch] = f2bf(0.125f * s);
   }
 }
+__global__ void k_zero_tail(float* __restrict__ dw, float* __restrict__ db, double* __restrict__ loss) {
+  if (threadIdx.x < 16) dw[threadIdx.x] = 0.f;
+  if (threadIdx.x == 16) db[0] = 0.f;
+  if (threadIdx.x == 17) loss[0] = 0.0;
+}
+
 // gradient of the pooling: d_fine[v][c] (=|+=) d_coarse[parent][c] / 8, then
 // zeroed where mask[v][c] <= 0 when a mask is given (rounded to BF16 first)
 __global__ void k_pool2_bwd(const uint16_t* __restrict__ dc, int ldc, uint16_t* __restrict__ df, int ldf, int c, Vol F,
@@ -966,6 +972,19 @@ __global__ void k_head_bwd16(const uint16_t* __restrict__ act, int ld, const flo
   }
 }
 
+// The loss and the head's gradient in one pass (16 channels): per voxel the
+// loss term d = (u - t) m (u given: the gated output) or (o - t) m (u NULL:
+// the loss on the head output), its gradient 2 d / count, through the gate
+// where u is given (zero outside the clamp), then the head gradient as
+// k_head_bwd16. The same FP32 operations per voxel as efwi_nn_mse,
+// efwi_nn_gate_bwd and efwi_nn_head_bwd one after another; the sums (loss in
+// FP64, dw, db in FP32) add in another order.
+__global__ __launch_bounds__(256) void k_loss_head16(const uint16_t* __restrict__ act, int ld, const float* __restrict__ w,
+                                                     const float* __restrict__ o, const float* __restrict__ u,
+                                                     const float* __restrict__ target, const float* __restrict__ m,
+                                                     long nvox, float inv_count, uint16_t* __restrict__ dact, int ldd,
+                                                     float* __restrict__ dw, float* __restrict__ db,
+                                                     double* __restrict__ loss);
 // multiplicative output: u[v] = g[v] exp(clamp(o[v], -4, 6)), g = channel 0
 // of the input; gradient do[v] = du[v] u[v] inside the clamp, 0 outside
 constexpr float kGateLo = -4.f, kGateHi = 6.f;
@@ -978,6 +997,78 @@ __global__ void k_gate_bwd(const float* __restrict__ o, const float* __restrict_
                            float* __restrict__ dout, long nvox) {
   for (long v = (long)blockIdx.x * blockDim.x + threadIdx.x; v < nvox; v += (long)gridDim.x * blockDim.x)
     dout[v] = (o[v] > kGateLo && o[v] < kGateHi) ? du[v] * u[v] : 0.f;
+}
+
+__global__ __launch_bounds__(256) void k_loss_head16(const uint16_t* __restrict__ act, int ld, const float* __restrict__ w,
+                                                     const float* __restrict__ o, const float* __restrict__ u,
+                                                     const float* __restrict__ target, const float* __restrict__ m,
+                                                     long nvox, float inv_count, uint16_t* __restrict__ dact, int ldd,
+                                                     float* __restrict__ dw, float* __restrict__ db,
+                                                     double* __restrict__ loss) {
+  __shared__ float red[17][256];
+  __shared__ double redl[256];
+  float acc[17];
+  for (int k = 0; k < 17; k++) acc[k] = 0.f;
+  double sl = 0.0;
+  float wr[16];
+#pragma unroll
+  for (int ch = 0; ch < 16; ch++) wr[ch] = w[ch];
+  for (long v = (long)blockIdx.x * blockDim.x + threadIdx.x; v < nvox; v += (long)gridDim.x * blockDim.x) {
+    float g;
+    if (u) {
+      const float uv = u[v];
+      const float d = (uv - target[v]) * m[v];
+      sl += (double)d * d;
+      const float dl = 2.f * d * inv_count;
+      g = (o[v] > kGateLo && o[v] < kGateHi) ? dl * uv : 0.f;
+    } else {
+      const float d = (o[v] - target[v]) * m[v];
+      sl += (double)d * d;
+      g = 2.f * d * inv_count;
+    }
+#pragma unroll
+    for (int h = 0; h < 2; h++) {
+      float a[8], dd[8];
+      unpack8(ld16(act + v * ld + 8 * h), a);
+#pragma unroll
+      for (int j = 0; j < 8; j++) {
+        dd[j] = a[j] > 0.f ? wr[8 * h + j] * g : 0.f;
+        acc[8 * h + j] += a[j] * g;
+      }
+      st16(dact + v * ldd + 8 * h, pack8(dd));
+    }
+    acc[16] += g;
+  }
+  for (int k = 0; k < 17; k++) red[k][threadIdx.x] = acc[k];
+  redl[threadIdx.x] = sl;
+  __syncthreads();
+  // 17 x 16 partial sums over 256 threads: item e = threadIdx.x + 256 r
+  float part[2] = {0.f, 0.f};
+#pragma unroll
+  for (int r = 0; r < 2; r++) {
+    const int e = (int)threadIdx.x + 256 * r;
+    if (e < 17 * 16) {
+      const int k = e >> 4, q0 = (e & 15) * 16;
+      for (int q = q0; q < q0 + 16; q++) part[r] += red[k][q];
+    }
+  }
+  for (int k = 128; k > 0; k >>= 1) {
+    if ((int)threadIdx.x < k) redl[threadIdx.x] += redl[threadIdx.x + k];
+    __syncthreads();
+  }
+#pragma unroll
+  for (int r = 0; r < 2; r++) {
+    const int e = (int)threadIdx.x + 256 * r;
+    if (e < 17 * 16) red[e >> 4][e & 15] = part[r];
+  }
+  __syncthreads();
+  if (threadIdx.x < 17) {
+    float t = 0.f;
+    for (int q = 0; q < 16; q++) t += red[threadIdx.x][q];
+    if (threadIdx.x < 16) atomicAdd(dw + threadIdx.x, t);
+    if (threadIdx.x == 16) atomicAdd(db, t);
+  }
+  if (threadIdx.x == 0) atomicAdd(loss, redl[0] * inv_count);
 }
 
 // masked mean-square loss: dout = 2 (out - target) m / sum(m); loss += sum m (out - target)^2 / sum(m)
@@ -1402,6 +1493,19 @@ int efwi_nn_mse(const float* out, const float* target, const float* mask, long n
   hipLaunchKernelGGL(k_zero, dim3(1), dim3(64), 0, s, reinterpret_cast<unsigned*>(loss), 2);
   hipLaunchKernelGGL(k_mse, dim3(grid_for(nvox, 256, 1024)), dim3(256), 0, s, out, target, mask, nvox,
                      (float)(1.0 / count), dout, loss);
+  return nn_check(s);
+}
+
+int efwi_nn_loss_head_bwd(const uint16_t* act, int ld, const float* w, const float* o, const float* u,
+                          const float* target, const float* mask, long nvox, double count, uint16_t* dact, int ldd,
+                          float* dw, float* db, double* loss, void* stream) {
+  if (!act || !w || !o || !target || !mask || !dact || !dw || !db || !loss || !(count > 0) || nvox <= 0 ||
+      !granules_ok(16, act, ld) || !granules_ok(16, dact, ldd))
+    return EFWI_NN_ERR_ARG;
+  hipStream_t s = (hipStream_t)stream;
+  hipLaunchKernelGGL(k_zero_tail, dim3(1), dim3(64), 0, s, dw, db, loss);
+  hipLaunchKernelGGL(k_loss_head16, dim3(grid_for(nvox, 256, 2048)), dim3(256), 0, s, act, ld, w, o, u, target, mask,
+                     nvox, (float)(1.0 / count), dact, ldd, dw, db, loss);
   return nn_check(s);
 }
 

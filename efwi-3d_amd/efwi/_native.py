@@ -88,6 +88,8 @@ SIGNATURES = {
                                     C.c_int, C.c_int, _vp]),
     "efwi_nn_wgrad_cat": (C.c_int, [_vp, C.c_int, C.c_int, _vp, C.c_int, C.c_int, C.c_int, _vp, C.c_int, C.c_int,
                                     _vp, C.c_long, _vp, _vp, C.c_int, C.c_int, C.c_int, C.c_int, _vp]),
+    "efwi_nn_loss_head_bwd": (C.c_int, [_vp, C.c_int, _vp, _vp, _vp, _vp, _vp, C.c_long, C.c_double, _vp, C.c_int,
+                                        _vp, _vp, _vp, _vp]),
     "efwi_nn_flip_weights": (C.c_int, [_vp, C.c_int, C.c_int, _vp, _vp]),
     "efwi_nn_flip_weights_n": (C.c_int, [C.c_int, _vp, _vp, _vp, _vp, _vp]),
     "efwi_nn_wgrad_scratch": (C.c_long, [C.c_int, C.c_int, C.c_int, C.c_int, C.c_int, C.c_int]),

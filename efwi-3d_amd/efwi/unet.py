@@ -279,13 +279,21 @@ class UNet3D:
         B = self._buffers(self._dims)
         s = self._stream()
         d1 = self._dims
-        d2, d3 = tuple(x // 2 for x in d1), tuple(x // 4 for x in d1)
         v1 = d1[0] * d1[1] * d1[2]
         if through_gate:
             nn_check(self.lib.efwi_nn_gate_bwd(_p(B["o"]), _p(B["out"]), _p(dout), _p(B["do"]), v1, s), "gate bwd")
             dout = B["do"]
         nn_check(self.lib.efwi_nn_head_bwd(_p(B["d1b"]), 16, 16, _p(self.wh), _p(dout), _p(B["g1b"]), 16,
                                            _p(self.grads["head.w"]), _p(self.grads["head.b"]), v1, s), "head bwd")
+        return self._backward_body()
+
+    def _backward_body(self):
+        """Gradients of every convolution from g1b, the gradient of the last
+        activation."""
+        B = self._buffers(self._dims)
+        s = self._stream()
+        d1 = self._dims
+        d2, d3 = tuple(x // 2 for x in d1), tuple(x // 4 for x in d1)
         # level 1 decoder
         self._wgrad("dec1b", B["g1b"], 16, B["d1a"], 16, d1)
         self._dgrad("dec1b", B["g1b"], 16, B["g1"], 16, B["d1a"], 16, d1)
@@ -341,15 +349,19 @@ class UNet3D:
     def train_step(self, feat, target, mask, count, dims, lr=1e-3, log_scaling=False):
         """One Adam step on the mean-square error of the update (target: the
         update), or of the head output (log_scaling: target = the log of the
-        per-voxel scaling update / gradient)."""
+        per-voxel scaling update / gradient). The loss, its gradient through
+        the gate and the head's gradient run as one kernel; the result is that
+        of loss() then backward()."""
         out = self.forward(feat, dims)
-        if log_scaling:
-            loss, dout = self.loss(self._buffers(dims)["o"], target, mask, count)
-        else:
-            loss, dout = self.loss(out, target, mask, count)
-        self.backward(dout, through_gate=not log_scaling)
+        B = self._buffers(dims)
+        nn_check(self.lib.efwi_nn_loss_head_bwd(_p(B["d1b"]), 16, _p(self.wh), _p(B["o"]),
+                                                None if log_scaling else _p(out), _p(target), _p(mask),
+                                                out.numel(), float(count), _p(B["g1b"]), 16,
+                                                _p(self.grads["head.w"]), _p(self.grads["head.b"]), _p(B["loss"]),
+                                                self._stream()), "loss and head gradient")
+        self._backward_body()
         self.adam(lr)
-        return loss
+        return B["loss"]
 
 
 def pack_input(g, illum, npml, device=0):

@@ -141,6 +141,14 @@ int efwi_nn_gate_bwd(const float* o, const float* u, const float* du, float* dou
 
 /* Masked mean-square loss over `count` voxels: *loss = sum m (out - t)^2 /
  * count (FP64, device), dout = 2 m (out - t) / count. */
+/* efwi_nn_mse, efwi_nn_gate_bwd and efwi_nn_head_bwd (16 channels) in one
+ * pass: with u (the gated output) the loss of u and its gradient through the
+ * gate, with u NULL the loss of the head output o itself; dact, dw, db and
+ * *loss as those calls give them (dact bit for bit; the sums in another
+ * order). act and dact 16-byte aligned, ld and ldd multiples of 8. */
+int efwi_nn_loss_head_bwd(const uint16_t* act, int ld, const float* w, const float* o, const float* u,
+                          const float* target, const float* mask, long nvox, double count, uint16_t* dact, int ldd,
+                          float* dw, float* db, double* loss, void* stream);
 int efwi_nn_mse(const float* out, const float* target, const float* mask, long nvox, double count, float* dout,
                 double* loss, void* stream);
 

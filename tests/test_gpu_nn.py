@@ -493,6 +493,38 @@ def test_unet_forward_and_weight_gradients_match_oracle():
         assert e < 5e-3, (k, e)
 
 
+@pytest.mark.parametrize("log_scaling", [False, True])
+def test_fused_loss_and_head_gradient_equal_the_separate_calls(log_scaling):
+    """train_step's single loss-and-head-gradient kernel against loss() then
+    backward(): every convolution's gradients bit for bit (the head's input
+    gradient is the same per voxel), the head's own gradients and the loss up
+    to the order of their sums."""
+    dims = (8, 16, 32)
+    net, feat, target, mask, V = _network_case(dims)
+    tg, mk = torch.from_numpy(target).to(DEV), torch.from_numpy(mask * 0.5).to(DEV)
+    print(f"[gpu] fused loss tail, log_scaling={log_scaling}", flush=True)
+    out = net.forward(feat, dims)
+    o = net._bufs[dims]["o"]
+    loss, dout = net.loss(o if log_scaling else out, tg, mk, V)
+    net.backward(dout, through_gate=not log_scaling)
+    torch.cuda.synchronize()
+    ref = {k: v.clone() for k, v in net.grads.items()}
+    ref_loss = float(loss)
+    for v in net.grads.values():
+        v.fill_(7.0)
+    # train_step = forward, the fused tail, the body, Adam: skip Adam by lr 0
+    steps = net.steps
+    got_loss = float(net.train_step(feat, tg, mk, V, dims, lr=0.0, log_scaling=log_scaling))
+    torch.cuda.synchronize()
+    assert net.steps == steps + 1
+    assert abs(got_loss - ref_loss) <= 1e-12 * abs(ref_loss)
+    for k, v in ref.items():
+        if k.startswith("head."):
+            assert torch.allclose(net.grads[k], v, rtol=1e-5, atol=1e-6 * float(v.abs().max())), k
+        else:
+            assert torch.equal(net.grads[k], v), k
+
+
 def test_unet_training_lowers_the_loss():
     """A short training run on the GPU's own kernels: learn the pseudo-Hessian
     scaling g / (sqrt(illumination) + eps) of the two input channels on random
