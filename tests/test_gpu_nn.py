@@ -57,10 +57,10 @@ def _s():
                                              ((5, 6, 33), 48, 16, 1), ((4, 4, 20), 96, 32, 1),
                                              # rows of 48 voxels or more: 64-voxel wavefront tiles, the last partial
                                              ((3, 4, 70), 16, 64, 1), ((2, 3, 130), 48, 32, 0),
-                                             # 256 bricks: enough for 2 to 4 output blocks per workgroup
+                                             # 1024 bricks: enough for 2 to 4 output blocks per workgroup
                                              # (smaller volumes take one block per workgroup)
-                                             ((16, 128, 64), 16, 64, 1), ((16, 128, 64), 32, 48, 0),
-                                             ((16, 128, 64), 96, 32, 1)])
+                                             ((32, 128, 128), 16, 64, 1), ((32, 128, 128), 32, 48, 0),
+                                             ((32, 128, 128), 32, 32, 1)])
 def test_conv3_forward_and_input_gradient(dims, kc, nc, relu):
     from efwi import _native as N
     from efwi.unet import nn_check
