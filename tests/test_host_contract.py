@@ -131,6 +131,39 @@ def test_cufd_reports_errors_instead_of_exiting(tmp_path):
     assert rc == -3 and b"z_rec/x_rec/y_rec" in lib.efwi_last_error()
 
 
+def test_nn_entry_points_refuse_invalid_arguments():
+    """The learned preconditioner's entry points check their arguments before
+    any device work: each invalid call returns EFWI_NN_ERR_ARG (1) on the host.
+    (The pointers stand for device memory that is never touched.)"""
+    from efwi import _native as N
+    lib = N.lib()
+    P = C.c_void_p(256)   # aligned, non-NULL, never dereferenced
+    Z = None
+    # convolutions: NULL input, channel count not a multiple of 16, an odd
+    # volume read through upsampling, bad second source or second output
+    assert lib.efwi_nn_conv3(Z, 16, 16, P, Z, P, 16, 16, Z, 0, 4, 4, 4, 0, 1, Z) == 1
+    assert lib.efwi_nn_conv3(P, 16, 8, P, Z, P, 16, 16, Z, 0, 4, 4, 4, 0, 1, Z) == 1
+    assert lib.efwi_nn_conv3(P, 16, 16, P, Z, P, 16, 16, P, 8, 4, 4, 4, 0, 1, Z) == 1   # mask stride < channels
+    assert lib.efwi_nn_conv3_cat(P, 32, 32, 1, P, 16, 16, P, Z, P, 16, 16, Z, 0, 0, Z, 0, 5, 4, 4, 0, 1, Z) == 1
+    assert lib.efwi_nn_conv3_cat(P, 32, 32, 0, P, 16, 8, P, Z, P, 16, 16, Z, 0, 0, Z, 0, 4, 4, 4, 0, 1, Z) == 1
+    assert lib.efwi_nn_conv3_cat(P, 32, 32, 0, Z, 0, 0, P, Z, P, 16, 16, P, 16, 8, Z, 0, 4, 4, 4, 0, 1, Z) == 1
+    # weight gradients
+    assert lib.efwi_nn_wgrad(P, 16, 16, P, 16, 16, Z, 0, P, Z, 4, 4, 4, 1, Z) == 1               # no scratch
+    assert lib.efwi_nn_wgrad(P, 16, 16, P, 16, 16, P, 1, P, Z, 4, 4, 4, 1, Z) == 1               # scratch too small
+    assert lib.efwi_nn_wgrad_cat(P, 16, 16, P, 16, 16, 0, Z, 16, 16, P, 1 << 30, P, Z, 4, 4, 4, 1, Z) == 1
+    assert lib.efwi_nn_wgrad_cat(P, 16, 16, P, 16, 16, 1, P, 16, 16, P, 1 << 30, P, Z, 3, 4, 4, 1, Z) == 1
+    assert lib.efwi_nn_wgrad_noalloc(P, 16, 16, P, 16, 8, P, Z, 4, 4, 4, 1, Z) == 1
+    # batched flip, pooling gradient, fused loss tail
+    ptrs = (C.c_void_p * 1)(256)
+    ints = (C.c_int * 1)(16)
+    assert lib.efwi_nn_flip_weights_n(0, ptrs, ints, ints, ptrs, Z) == 1
+    assert lib.efwi_nn_flip_weights_n(17, ptrs, ints, ints, ptrs, Z) == 1
+    assert lib.efwi_nn_pool2_bwd(P, 16, P, 16, 16, 3, 4, 4, 1, Z, 0, Z) == 1
+    assert lib.efwi_nn_pool2_bwd(P, 16, P, 16, 16, 4, 4, 4, 1, P, 8, Z) == 1
+    assert lib.efwi_nn_loss_head_bwd(P, 16, P, P, Z, P, P, 64, 0.0, P, 16, P, P, P, Z) == 1
+    assert lib.efwi_nn_loss_head_bwd(P, 12, P, P, Z, P, P, 64, 1.0, P, 16, P, P, P, Z) == 1
+
+
 def test_create_without_device_fails_cleanly():
     """On a machine without a HIP device, efwi_create returns an error code."""
     code = ("import sys, ctypes as C; sys.path.insert(0, %r); from efwi import _native as N; import torch;"
